@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Lloyd iterations/s of the CVT engine on one MI355X.
+
+Workload: 2048^2 disk-cut patch, 0.05 + 8 anisotropic Gaussians, 16384 sites
+rejection-sampled from the density (the 2048^2 / 16K configuration the
+metric is quoted on).  One step = one Lloyd iteration = JFA(+1) (12 passes)
++ moments and energy + site update, replayed from one captured hipGraph.
+Inputs are resident in HBM before the timed region.
+
+The JSON line carries:
+  roofline      the dominant kernel (the JFA pass): algorithmic bytes per
+                launch (read + write of the int32 label map, 8 B/pixel)
+                over its mean duration measured with HIP events on the
+                engine's stream, against the 8 TB/s HBM peak;
+  cpu_baseline  the oracle's OpenMP Lloyd (JFA(+1) and exact nearest-site
+                variants; the faster is reported) on a bounded sample of the
+                same workload, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "surface-remesher_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Lloyd iter/s (2048², 16K sites); time-to-tol: learned init vs plain Lloyd"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(w, seconds: float):
+    """Times the oracle's Lloyd on the host (the 'port' baseline)."""
+    import numpy as np
+    from oracle import oracle as orc
+    threads = min(orc.max_threads(), int(os.environ.get("OMP_NUM_THREADS", "64")))
+    orc.set_threads(threads)
+    res = {}
+    for name, exact in (("jfa", False), ("exact", True)):
+        xy = w.sites.copy()
+        orc.lloyd(w.density, xy, 1, exact=exact)           # warm-up
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < seconds / 2 and k < 50:
+            orc.lloyd(w.density, xy, 1, exact=exact)
+            k += 1
+        dt = time.perf_counter() - t0
+        res[name] = (k / dt, k)
+    best = max(res, key=lambda n: res[n][0])
+    return {
+        "value": round(res[best][0], 4),
+        "unit": "iter/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{w.name}: oracle Lloyd on the host, {res['jfa'][1]} iterations of the "
+                   f"JFA(+1) variant ({res['jfa'][0]:.3f} it/s) and {res['exact'][1]} of the "
+                   f"exact nearest-site variant ({res['exact'][0]:.3f} it/s), OpenMP over "
+                   f"{threads} threads; value = the faster ({best})"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-iters", type=int, default=10)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this engine runs one grid per device and "
+              "has no multi-GPU path (see DESIGN.md for how it would shard); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import cvtamd
+    from cvtamd import workloads as wl
+
+    torch.cuda.set_device(0)
+    w = wl.config2(args.seed)
+    n, S = w.n, w.num_sites
+    rho = torch.from_numpy(w.density).cuda()
+    xy = torch.from_numpy(w.sites).cuda()
+    stream = torch.cuda.Stream()
+    eng = cvtamd.Engine(n, n, S)
+    eng.set_stream(stream.cuda_stream)
+    eng.set_density(rho)
+    eng.set_sites(xy)
+    torch.cuda.synchronize()
+
+    # warm-up (graph capture happens on the first launch)
+    eng.lloyd(args.warmup, 1.0)
+    eng.synchronize()
+    torch.cuda.synchronize()
+
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    eng.lloyd(args.steps, 1.0)
+    end.record(stream)
+    torch.cuda.synchronize()
+    eng.synchronize()
+    ms_total = start.elapsed_time(end)
+    ms_step = ms_total / args.steps
+    value = 1000.0 / ms_step
+
+    hist = eng.history()
+    assert np.isfinite(hist).all() and hist.shape[0] == args.warmup + args.steps
+
+    # per-kernel durations (HIP events between launches, same stream)
+    prof = eng.profile(args.profile_iters, 1.0)
+    P = prof["jfa_passes"]
+    jfa_bytes = 8.0 * n * n                       # read + write int32 labels
+    jfa_gbs = jfa_bytes / (prof["jfa_pass"] * 1e-3) / 1e9
+    iter_bytes = n * n * (8.0 * P + 8.0)           # N^2 (8P + 8)
+    iter_gbs = iter_bytes / (ms_step * 1e-3) / 1e9
+    eng.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "grid": n, "sites": S, "jfa_passes": P,
+                   "moment_tile": prof["tile"], "seed": args.seed},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_jfa",
+            "achieved": round(jfa_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(jfa_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": jfa_bytes,
+            "launch_ms": round(prof["jfa_pass"], 6),
+            "iteration_achieved": round(iter_gbs, 1),
+            "iteration_frac": round(iter_gbs / HBM_PEAK_GBS, 4),
+            "kernel_ms": {k: round(v, 6) for k, v in prof.items() if isinstance(v, float)},
+        },
+        "energy": {"first": hist[0, 0], "last": hist[-1, 0]},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(w, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["vs_cpu"] = round(value / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

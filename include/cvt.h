@@ -1,0 +1,126 @@
+/*
+ * cvt.h -- C ABI of the MI355X CVT engine (libcvt_hip.so).
+ *
+ * Plain pointers and sizes only.  Every pointer argument may be a host
+ * pointer or a device pointer of the engine's device (copies use
+ * hipMemcpyDefault); calls that read results back to a host pointer
+ * synchronise the engine's stream.  Every function returns CVT_OK (0) or a
+ * negative status; cvt_last_error() gives the message of the calling
+ * thread's last failure.  A context is not re-entrant: one thread at a time.
+ *
+ * Conventions: a grid of width x height pixels, row-major (pixel (x, y) at
+ * y*width + x, the reference's TOID, source/gcvt.h:26).  Sites are fp32
+ * (x, y) pairs in pixel units; pixel (x, y) has its centre at
+ * (x + 0.5, y + 0.5).  Labels are int32 site indices, CVT_LABEL_NONE where no
+ * site reaches.  The Voronoi map is JFA with steps s0, s0/2, ..., 1 plus one
+ * extra step-1 pass (s0 = largest power of two below max(width, height)),
+ * distance d = fmaf(dx, dx, dy*dy), ties to the lower index.
+ *
+ * Which reference interface each entry point replaces:
+ *   cvt_create / cvt_destroy      gcvtInitialization / pbaCVDDeinitialization
+ *                                 (source/gcvt.cu:840-903)
+ *   cvt_set_density               pba2DInitializeInput density upload and the
+ *                                 once-per-run prefix sums
+ *                                 (source/gcvt.cu:906-914, :995-1006)
+ *   cvt_set_sites / cvt_get_sites the site texture upload / download
+ *                                 (source/gcvt.cu:1101-1103, :1152)
+ *   cvt_voronoi / cvt_get_labels  pba2DCompute (source/gcvt.cu:967-978)
+ *   cvt_moments                   pbaCVDComputeCentroid
+ *                                 (source/gcvt.cu:1008-1023)
+ *   cvt_lloyd                     the Lloyd loop body: pba2DCompute +
+ *                                 pbaCVDCalcEnergy + pbaCVDComputeCentroid +
+ *                                 pbaCVDUpdateSites
+ *                                 (source/gcvt.cu:1113-1121)
+ *   cvt_energy                    pbaCVDCalcEnergy (source/gcvt.cu:1059-1083),
+ *                                 in float64 and without its reduction bias
+ *   cvt_compat_gcvt               gCVT (source/gcvt.cu:1087-1156)
+ *   cvt_rasterize                 discretization_d
+ *                                 (source/discretization.cu:87-120)
+ *   cvt_extract_points            the site scan of delaunayInput
+ *                                 (source/delaunay.h:48-58)
+ */
+#ifndef CVT_H
+#define CVT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CVT_OK 0
+#define CVT_ERR_ARG (-1)      /* invalid argument */
+#define CVT_ERR_HIP (-2)      /* HIP runtime failure */
+#define CVT_ERR_NOMEM (-3)    /* device allocation failed */
+#define CVT_ERR_STATE (-4)    /* call out of order (e.g. no density yet) */
+#define CVT_ERR_CAPACITY (-5) /* a fixed-size table overflowed */
+
+#define CVT_LABEL_NONE 0x7fffffff
+
+typedef struct cvt_ctx cvt_ctx;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *cvt_last_error(void);
+/* ABI version (major * 100 + minor). */
+int cvt_version(void);
+
+/* A context for one grid: allocates every device buffer it will use (the
+ * reference allocates per call, source/gcvt.cu:859-874).  max_sites bounds
+ * the site count of later calls. */
+int cvt_create(cvt_ctx **ctx, int width, int height, int max_sites);
+int cvt_destroy(cvt_ctx *ctx);
+/* Launch subsequent work on this hipStream_t (NULL = the context's own
+ * non-blocking stream). */
+int cvt_set_stream(cvt_ctx *ctx, void *hip_stream);
+/* width*height fp32 densities, finite and >= 0. */
+int cvt_set_density(cvt_ctx *ctx, const float *density);
+/* num_sites (x, y) fp32 pairs in pixel units; fixed (optional, num_sites
+ * bytes) marks sites the update never moves (the reference's constraint
+ * mask, source/gcvt.h:106-122). */
+int cvt_set_sites(cvt_ctx *ctx, const float *xy, const uint8_t *fixed, int num_sites);
+int cvt_get_sites(cvt_ctx *ctx, float *xy);
+/* Current site count (>= 0) or a negative status. */
+int cvt_num_sites(cvt_ctx *ctx);
+/* Label map of the current sites. */
+int cvt_voronoi(cvt_ctx *ctx);
+/* width*height int32 labels of the last Voronoi pass. */
+int cvt_get_labels(cvt_ctx *ctx, int32_t *labels);
+/* Per-site float64 moments over the cells of the last Voronoi pass, 4 per
+ * site: W = sum rho, DX = sum rho (px - sx), DY = sum rho (py - sy),
+ * E = sum rho |p - s|^2 (pixel units); the centroid is s + (DX, DY) / W. */
+int cvt_moments(cvt_ctx *ctx, double *moments);
+/* iterations Lloyd steps: Voronoi of the current sites, moments, energy,
+ * then s <- s + omega (c - s) for every non-fixed site with W > 0.  The
+ * per-iteration energy and gradient norm are recorded (cvt_get_history). */
+int cvt_lloyd(cvt_ctx *ctx, int iterations, float omega);
+/* Number of recorded iterations (>= 0) or a negative status. */
+int cvt_history_length(cvt_ctx *ctx);
+/* The first n recorded (energy, gradient norm) pairs, 2n doubles.  Energy
+ * E = sum_p rho_p |s(p) - p|^2 / (width * height) in pixel units (the
+ * reference's normalisation, source/gcvt.cu:788-802); |grad E| =
+ * 2 / (width * height) * sqrt(sum_i DX_i^2 + DY_i^2). */
+int cvt_get_history(cvt_ctx *ctx, double *energy_gradnorm, int n);
+int cvt_clear_history(cvt_ctx *ctx);
+/* Float64 energy of the current sites over their fresh Voronoi map, and the
+ * gradient norm, reduced on the GPU in a fixed order. */
+int cvt_energy(cvt_ctx *ctx, double *energy, double *gradnorm);
+int cvt_synchronize(cvt_ctx *ctx);
+/* Runs `iterations` Lloyd steps eagerly with HIP events between launches and
+ * returns mean durations in ms: ms[0] one JFA pass, ms[1] the moment tile
+ * kernel, ms[2] the overflow kernel, ms[3] update + finalize, ms[4] 0,
+ * ms[5] the whole step, ms[6] the JFA pass count, ms[7] the moment tile.
+ * (No reference counterpart: the reference times stages with clock(),
+ * source/main.cpp:181-242.) */
+int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms);
+
+/* Tuning: moment tile side (32, 64, 128 or 256; 0 = choose from the site
+ * density).  Results do not depend on it. */
+int cvt_set_tile(cvt_ctx *ctx, int tile);
+int cvt_get_tile(cvt_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CVT_H */

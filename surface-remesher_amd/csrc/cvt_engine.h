@@ -1,0 +1,68 @@
+// cvt_engine.h -- internal state of a CVT engine context (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/cvt.h"
+#include "cvt_kernels.h"
+
+namespace cvt {
+
+int fail(int code, const char *fmt, ...);
+void clear_error();
+const char *last_error();
+int jfa_first_step(int W, int H);
+int jfa_num_passes(int W, int H);
+
+struct Ctx {
+    int W = 0, H = 0, maxS = 0, S = 0;
+    int device = 0;
+    hipStream_t stream = nullptr, own_stream = nullptr;
+    bool own = false;
+
+    float *rho = nullptr;          // [H][W]
+    int rho_exp = 0;               // rho' = rho * 2^-rho_exp <= 1
+    bool have_density = false;
+    int32_t *seed = nullptr;       // seed map [H][W]
+    bool seeds_valid = false;      // seed map holds the current sites
+    int32_t *lab[2] = {nullptr, nullptr};
+    int final_buf = 0;             // lab[final_buf] = labels of the last JFA
+    bool labels_fresh = false;
+    float2 *sites = nullptr;       // [maxS + 1]
+    uint8_t *fixed = nullptr;      // [maxS]
+
+    int tile = 0, tilesX = 0, tilesY = 0;
+    bool tile_auto = true;
+    Slot *slots = nullptr;         // [maxS][9]
+    OvfRec *ovf = nullptr;         // [tiles][ovf_cap]
+    int ovf_cap = 0;
+    int *ovf_count = nullptr;      // [tiles]
+    double *ovf_acc = nullptr;     // [maxS][4]
+    double *blockpart = nullptr;
+    double *hist = nullptr;        // [hist_cap][2]
+    int hist_cap = 0;
+    int *dstate = nullptr;         // ST_*
+    int *dflags = nullptr;         // FLAG_*
+    float *scratch = nullptr;
+    double *eg = nullptr;
+
+    hipEvent_t *prof_events = nullptr;   // per-pass events while profiling
+    hipEvent_t prof_mid = nullptr;
+
+    bool use_graph = true;
+    bool graph_valid = false;
+    float graph_omega = 0.0f;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    ~Ctx();
+    int enqueue_seed();
+    int enqueue_jfa();
+    int enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg);
+    int enqueue_iteration(float omega);
+    int build_graph(float omega);
+    int lloyd(int iterations, float omega);
+    int check_flags();
+};
+
+}  // namespace cvt

@@ -1,0 +1,629 @@
+// cvt_engine.hip -- host side of the C ABI declared in include/cvt.h.
+//
+// One cvt_ctx owns every device buffer of one grid for its whole life (the
+// reference allocates and frees about eleven N^2 buffers per gCVT call,
+// source/gcvt.cu:840-903).  A Lloyd iteration is P JFA launches, one moment
+// launch, one overflow launch, one update launch and one finalize launch; it
+// is captured once into a hipGraph and replayed, so the host does no work per
+// iteration and never synchronises inside the loop (the reference copies the
+// energy back every 10th iteration, source/gcvt.cu:1080).
+#include "cvt_engine.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+namespace cvt {
+
+static thread_local char g_err[512];
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+void clear_error() { g_err[0] = 0; }
+const char *last_error() { return g_err; }
+
+int jfa_first_step(int W, int H)
+{
+    int n = std::max(W, H), s = 1;
+    while (s * 2 < n) s *= 2;
+    return s;
+}
+
+int jfa_num_passes(int W, int H)
+{
+    int s = jfa_first_step(W, H), p = 1;
+    while (s > 1) { s >>= 1; ++p; }
+    return p + 1;
+}
+
+static int choose_tile(int W, int H, int S)
+{
+    // the 3x3 tile window around a site's home tile must hold its cell: take a
+    // tile of at least four mean cell diameters, at most 256
+    double cell = sqrt((double)W * H / std::max(S, 1));
+    int t = 32;
+    while (t < 256 && t < 4.0 * cell) t *= 2;
+    return t;
+}
+
+static int launch_moments(Ctx *c, const int32_t *lab)
+{
+    const float rs = (float)ldexp(1.0, -c->rho_exp);
+    dim3 grid(c->tilesX * c->tilesY), block(256);
+    switch (c->tile) {
+    case 32:
+        hipLaunchKernelGGL(k_moments<32>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
+                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
+                           c->ovf_count, c->dflags, c->dstate);
+        break;
+    case 64:
+        hipLaunchKernelGGL(k_moments<64>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
+                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
+                           c->ovf_count, c->dflags, c->dstate);
+        break;
+    case 128:
+        hipLaunchKernelGGL(k_moments<128>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
+                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
+                           c->ovf_count, c->dflags, c->dstate);
+        break;
+    default:
+        hipLaunchKernelGGL(k_moments<256>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
+                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
+                           c->ovf_count, c->dflags, c->dstate);
+        break;
+    }
+    if (c->prof_mid) hipEventRecord(c->prof_mid, c->stream);
+    hipLaunchKernelGGL(k_overflow, dim3(1), dim3(256), 0, c->stream, c->ovf, c->ovf_cap,
+                       c->ovf_count, c->tilesX * c->tilesY, c->ovf_acc, c->dflags);
+    return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "moment launch failed");
+}
+
+// moments with an event between the tile kernel and the overflow kernel
+static int launch_moments_timed(Ctx *c, hipEvent_t after_tiles, hipEvent_t after_ovf)
+{
+    hipEvent_t *saved = c->prof_events;
+    c->prof_events = nullptr;
+    c->prof_mid = after_tiles;
+    int rc = launch_moments(c, c->lab[c->final_buf]);
+    c->prof_mid = nullptr;
+    c->prof_events = saved;
+    hipEventRecord(after_ovf, c->stream);
+    return rc;
+}
+
+static int set_tile_internal(Ctx *c, int tile)
+{
+    if (tile == 0) tile = choose_tile(c->W, c->H, c->S);
+    if (tile != 32 && tile != 64 && tile != 128 && tile != 256)
+        return fail(CVT_ERR_ARG, "tile must be 32, 64, 128 or 256 (got %d)", tile);
+    if (tile == c->tile && c->slots) return CVT_OK;
+    c->tile = tile;
+    c->tilesX = (c->W + tile - 1) / tile;
+    c->tilesY = (c->H + tile - 1) / tile;
+    const long ntiles = (long)c->tilesX * c->tilesY;
+    // a tile holds at most CVT_HASH_CAP labels (more raises CVT_ERR_CAPACITY)
+    c->ovf_cap = CVT_HASH_CAP;
+    if (c->ovf) hipFree(c->ovf);
+    if (c->ovf_count) hipFree(c->ovf_count);
+    c->ovf = nullptr;
+    c->ovf_count = nullptr;
+    if (hipMalloc(&c->ovf, sizeof(OvfRec) * ntiles * c->ovf_cap) != hipSuccess ||
+        hipMalloc(&c->ovf_count, sizeof(int) * ntiles) != hipSuccess)
+        return fail(CVT_ERR_NOMEM, "overflow buffers (%ld tiles)", ntiles);
+    hipMemsetAsync(c->ovf_count, 0, sizeof(int) * ntiles, c->stream);
+    c->graph_valid = false;
+    return CVT_OK;
+}
+
+int Ctx::check_flags()
+{
+    int f[NUM_FLAGS];
+    if (hipMemcpyAsync(f, dflags, sizeof f, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return fail(CVT_ERR_HIP, "flag readback: %s", hipGetErrorString(hipGetLastError()));
+    if (f[FLAG_FULL]) {
+        int z = 0;
+        hipMemcpyAsync(dflags + FLAG_FULL, &z, sizeof z, hipMemcpyHostToDevice, stream);
+        hipStreamSynchronize(stream);
+        return fail(CVT_ERR_CAPACITY,
+                    "more than %d distinct sites met one %dx%d moment tile; use a smaller tile",
+                    CVT_HASH_CAP, tile, tile);
+    }
+    return CVT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one Lloyd iteration (or its parts) on the stream
+// ---------------------------------------------------------------------------
+int Ctx::enqueue_seed()
+{
+    if (S > 0)
+        hipLaunchKernelGGL(k_seed, dim3((S + 255) / 256), dim3(256), 0, stream, sites, S, seed, W,
+                           H);
+    seeds_valid = true;
+    return CVT_OK;
+}
+
+// JFA(+1) of the current sites; the labels end in lab[final_buf]
+int Ctx::enqueue_jfa()
+{
+    const int np = jfa_num_passes(W, H);
+    int step = jfa_first_step(W, H);
+    dim3 block(64, 4), grid((W + 63) / 64, (H + 3) / 4);
+    const int32_t *in = seed;
+    int cur = 0;
+    for (int k = 0; k < np; ++k) {
+        int32_t *out = lab[cur];
+        hipLaunchKernelGGL(k_jfa, grid, block, 0, stream, in, out, sites, (unsigned)S, W, H, step,
+                           k == 1 ? seed : nullptr);
+        if (prof_events) hipEventRecord(prof_events[k + 1], stream);
+        in = out;
+        cur ^= 1;
+        if (step > 1) step >>= 1;
+    }
+    final_buf = cur ^ 1;
+    seeds_valid = false;
+    return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "jfa launch failed");
+}
+
+int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg)
+{
+    const int nb = std::max(1, (S + 255) / 256);
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, ovf_acc,
+                       dflags, dstate, omega, mode, ldexp(1.0, rho_exp), mom_out, blockpart, seed,
+                       W, H);
+    const double a = (double)W * H;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, blockpart, nb, dstate, dflags,
+                       hist, hist_cap, record, ldexp(1.0, rho_exp) / a, 2.0 * ldexp(1.0, rho_exp) / a,
+                       out_eg);
+    if (mode & UPD_SEED) seeds_valid = true;
+    return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "update launch failed");
+}
+
+int Ctx::enqueue_iteration(float omega)
+{
+    int rc;
+    if ((rc = enqueue_jfa())) return rc;
+    if ((rc = launch_moments(this, lab[final_buf]))) return rc;
+    return enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr);
+}
+
+int Ctx::build_graph(float omega)
+{
+    if (graph_valid && graph_omega == omega) return CVT_OK;
+    if (graph_exec) { hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+    if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+    graph_valid = false;
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(CVT_ERR_HIP, "stream capture could not start");
+    const bool sv = seeds_valid;
+    int rc = enqueue_iteration(omega);
+    seeds_valid = sv;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess || !g) return fail(CVT_ERR_HIP, "stream capture failed");
+    if (hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(g);
+        return fail(CVT_ERR_HIP, "graph instantiation failed");
+    }
+    graph = g;
+    graph_valid = true;
+    graph_omega = omega;
+    return CVT_OK;
+}
+
+int Ctx::lloyd(int iterations, float omega)
+{
+    if (!have_density) return fail(CVT_ERR_STATE, "cvt_lloyd before cvt_set_density");
+    if (iterations <= 0) return CVT_OK;
+    if (!seeds_valid) enqueue_seed();
+    int rc;
+    if (use_graph) {
+        if ((rc = build_graph(omega))) return rc;
+        for (int k = 0; k < iterations; ++k)
+            if (hipGraphLaunch(graph_exec, stream) != hipSuccess)
+                return fail(CVT_ERR_HIP, "graph launch failed");
+        seeds_valid = true;
+    } else {
+        for (int k = 0; k < iterations; ++k)
+            if ((rc = enqueue_iteration(omega))) return rc;
+    }
+    labels_fresh = true;
+    return CVT_OK;
+}
+
+Ctx::~Ctx()
+{
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    if (graph) hipGraphDestroy(graph);
+    void *bufs[] = {rho, seed, lab[0], lab[1], sites, fixed, slots, ovf, ovf_count, ovf_acc,
+                    blockpart, hist, dstate, dflags, scratch, eg};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (own_stream && own) hipStreamDestroy(own_stream);
+}
+
+}  // namespace cvt
+
+using namespace cvt;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define CHECK_CTX(c) \
+    do { if (!(c)) return fail(CVT_ERR_ARG, "null context"); clear_error(); } while (0)
+#define HIPCHK(call, what) \
+    do { hipError_t e_ = (call); if (e_ != hipSuccess) \
+        return fail(CVT_ERR_HIP, "%s: %s", what, hipGetErrorString(e_)); } while (0)
+
+extern "C" {
+
+const char *cvt_last_error(void) { return last_error(); }
+int cvt_version(void) { return 100; }
+
+int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
+{
+    clear_error();
+    if (!out) return fail(CVT_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (W < 1 || H < 1 || W > 16384 || H > 16384)
+        return fail(CVT_ERR_ARG, "grid %dx%d outside [1, 16384]^2", W, H);
+    if (max_sites < 0 || max_sites > (1 << 28)) return fail(CVT_ERR_ARG, "max_sites %d", max_sites);
+    Ctx *c = new Ctx();
+    c->W = W; c->H = H; c->maxS = max_sites; c->S = 0;
+    hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(CVT_ERR_HIP, "stream creation failed");
+    }
+    c->own = true;
+    c->stream = c->own_stream;
+    const long n = (long)W * H;
+    const long ms = std::max(max_sites, 1);
+    c->hist_cap = 1 << 16;
+    const int nb = (int)std::max(1L, (ms + 255) / 256);
+    bool ok = hipMalloc(&c->rho, sizeof(float) * n) == hipSuccess &&
+              hipMalloc(&c->seed, sizeof(int32_t) * n) == hipSuccess &&
+              hipMalloc(&c->lab[0], sizeof(int32_t) * n) == hipSuccess &&
+              hipMalloc(&c->lab[1], sizeof(int32_t) * n) == hipSuccess &&
+              hipMalloc(&c->sites, sizeof(float2) * (ms + 1)) == hipSuccess &&
+              hipMalloc(&c->fixed, ms) == hipSuccess &&
+              hipMalloc(&c->slots, sizeof(Slot) * 9 * ms) == hipSuccess &&
+              hipMalloc(&c->ovf_acc, sizeof(double) * 4 * ms) == hipSuccess &&
+              hipMalloc(&c->blockpart, sizeof(double) * 2 * std::max(nb, 4096)) == hipSuccess &&
+              hipMalloc(&c->hist, sizeof(double) * 2 * c->hist_cap) == hipSuccess &&
+              hipMalloc(&c->dstate, sizeof(int) * NUM_STATE) == hipSuccess &&
+              hipMalloc(&c->dflags, sizeof(int) * NUM_FLAGS) == hipSuccess &&
+              hipMalloc(&c->scratch, sizeof(float) * 4096) == hipSuccess &&
+              hipMalloc(&c->eg, sizeof(double) * 4) == hipSuccess;
+    if (!ok) {
+        delete c;
+        return fail(CVT_ERR_NOMEM, "device allocation for a %dx%d grid, %d sites", W, H, max_sites);
+    }
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, c->seed, (int32_t)CVT_NONE, n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, c->lab[0], (int32_t)CVT_NONE, n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, c->lab[1], (int32_t)CVT_NONE, n);
+    hipMemsetAsync(c->rho, 0, sizeof(float) * n, s);
+    hipMemsetAsync(c->fixed, 0, ms, s);
+    hipMemsetAsync(c->slots, 0, sizeof(Slot) * 9 * ms, s);
+    hipMemsetAsync(c->ovf_acc, 0, sizeof(double) * 4 * ms, s);
+    hipMemsetAsync(c->dflags, 0, sizeof(int) * NUM_FLAGS, s);
+    int st[NUM_STATE] = {1, 0, 0, 0};
+    hipMemcpyAsync(c->dstate, st, sizeof st, hipMemcpyHostToDevice, s);
+    float2 sentinel = make_float2(__builtin_inff(), __builtin_inff());
+    hipMemcpyAsync(c->sites, &sentinel, sizeof sentinel, hipMemcpyHostToDevice, s);
+    int rc = set_tile_internal(c, 64);
+    if (rc == CVT_OK && hipStreamSynchronize(s) != hipSuccess)
+        rc = fail(CVT_ERR_HIP, "initialisation failed");
+    if (rc) { delete c; return rc; }
+    c->tile_auto = true;
+    *out = reinterpret_cast<cvt_ctx *>(c);
+    return CVT_OK;
+}
+
+int cvt_destroy(cvt_ctx *ctx)
+{
+    clear_error();
+    if (!ctx) return CVT_OK;
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    hipStreamSynchronize(c->stream);
+    delete c;
+    return CVT_OK;
+}
+
+int cvt_set_stream(cvt_ctx *ctx, void *stream)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    hipStreamSynchronize(c->stream);
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    c->graph_valid = false;
+    return CVT_OK;
+}
+
+int cvt_set_density(cvt_ctx *ctx, const float *density)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!density) return fail(CVT_ERR_ARG, "null density");
+    const long n = (long)c->W * c->H;
+    HIPCHK(hipMemcpyAsync(c->rho, density, sizeof(float) * n, hipMemcpyDefault, c->stream),
+           "density upload");
+    int zero = 0;
+    HIPCHK(hipMemcpyAsync(c->dflags + FLAG_BAD, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream),
+           "flag reset");
+    const int nb = 1024;
+    hipLaunchKernelGGL(k_density_stats, dim3(nb), dim3(256), 0, c->stream, c->rho, n, c->scratch,
+                       c->dflags);
+    std::vector<float> bm(nb);
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(bm.data(), c->scratch, sizeof(float) * nb, hipMemcpyDeviceToHost, c->stream),
+           "density stats");
+    HIPCHK(hipMemcpyAsync(&bad, c->dflags + FLAG_BAD, sizeof bad, hipMemcpyDeviceToHost, c->stream),
+           "density flag");
+    HIPCHK(hipStreamSynchronize(c->stream), "density stats");
+    if (bad) {
+        c->have_density = false;
+        return fail(CVT_ERR_ARG, "density holds a negative, NaN or infinite value");
+    }
+    float mx = 0.0f;
+    for (float v : bm) mx = std::max(mx, v);
+    // rho' = rho * 2^-rho_exp <= 1 keeps the fixed-point moment sums in range
+    int ex = 0;
+    if (mx > 0.0f) {
+        frexp((double)mx, &ex);   // mx in [2^(ex-1), 2^ex)
+    }
+    c->rho_exp = ex;
+    c->have_density = true;
+    c->graph_valid = false;
+    return CVT_OK;
+}
+
+int cvt_set_sites(cvt_ctx *ctx, const float *xy, const uint8_t *fixed, int num_sites)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (num_sites < 0 || num_sites > c->maxS)
+        return fail(CVT_ERR_ARG, "num_sites %d outside [0, %d]", num_sites, c->maxS);
+    if (num_sites > 0 && !xy) return fail(CVT_ERR_ARG, "null site array");
+    // the seed map is all CVT_NONE unless seeds are pending: clear them first
+    if (c->seeds_valid && c->S > 0)
+        hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, c->stream, c->seed,
+                           (int32_t)CVT_NONE, (long)c->W * c->H);
+    if (num_sites > 0) {
+        HIPCHK(hipMemcpyAsync(c->sites, xy, sizeof(float2) * num_sites, hipMemcpyDefault, c->stream),
+               "site upload");
+        if (fixed)
+            HIPCHK(hipMemcpyAsync(c->fixed, fixed, num_sites, hipMemcpyDefault, c->stream),
+                   "fixed upload");
+        else
+            HIPCHK(hipMemsetAsync(c->fixed, 0, num_sites, c->stream), "fixed clear");
+    }
+    float2 sentinel = make_float2(__builtin_inff(), __builtin_inff());
+    HIPCHK(hipMemcpyAsync(c->sites + num_sites, &sentinel, sizeof sentinel, hipMemcpyHostToDevice,
+                          c->stream), "sentinel");
+    HIPCHK(hipStreamSynchronize(c->stream), "site upload");
+    const int oldS = c->S;
+    c->S = num_sites;
+    c->seeds_valid = false;
+    c->enqueue_seed();
+    if (c->tile_auto) {
+        int rc = set_tile_internal(c, 0);
+        if (rc) return rc;
+    }
+    if (oldS != num_sites) c->graph_valid = false;
+    c->labels_fresh = false;
+    HIPCHK(hipGetLastError(), "seed launch");
+    return CVT_OK;
+}
+
+int cvt_get_sites(cvt_ctx *ctx, float *xy)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (c->S == 0) return CVT_OK;
+    if (!xy) return fail(CVT_ERR_ARG, "null output");
+    HIPCHK(hipMemcpyAsync(xy, c->sites, sizeof(float2) * c->S, hipMemcpyDefault, c->stream),
+           "site download");
+    HIPCHK(hipStreamSynchronize(c->stream), "site download");
+    return CVT_OK;
+}
+
+int cvt_num_sites(cvt_ctx *ctx)
+{
+    CHECK_CTX(ctx);
+    return reinterpret_cast<Ctx *>(ctx)->S;
+}
+
+int cvt_voronoi(cvt_ctx *ctx)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->seeds_valid) c->enqueue_seed();
+    int rc = c->enqueue_jfa();
+    if (rc) return rc;
+    c->labels_fresh = true;
+    HIPCHK(hipGetLastError(), "voronoi");
+    return CVT_OK;
+}
+
+int cvt_get_labels(cvt_ctx *ctx, int32_t *labels)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!labels) return fail(CVT_ERR_ARG, "null output");
+    if (!c->labels_fresh) return fail(CVT_ERR_STATE, "no label map yet (call cvt_voronoi)");
+    HIPCHK(hipMemcpyAsync(labels, c->lab[c->final_buf], sizeof(int32_t) * (long)c->W * c->H,
+                          hipMemcpyDefault, c->stream), "label download");
+    HIPCHK(hipStreamSynchronize(c->stream), "label download");
+    return CVT_OK;
+}
+
+int cvt_moments(cvt_ctx *ctx, double *m)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->have_density) return fail(CVT_ERR_STATE, "cvt_moments before cvt_set_density");
+    if (!c->labels_fresh) return fail(CVT_ERR_STATE, "no label map yet (call cvt_voronoi)");
+    if (!m) return fail(CVT_ERR_ARG, "null output");
+    double *dm = nullptr;
+    HIPCHK(hipMalloc(&dm, sizeof(double) * 4 * std::max(c->S, 1)), "moment buffer");
+    int rc = launch_moments(c, c->lab[c->final_buf]);
+    if (!rc) rc = c->enqueue_update(0, 1.0f, dm, 0, nullptr);
+    if (!rc && c->S > 0 &&
+        hipMemcpyAsync(m, dm, sizeof(double) * 4 * c->S, hipMemcpyDefault, c->stream) != hipSuccess)
+        rc = fail(CVT_ERR_HIP, "moment download");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(CVT_ERR_HIP, "moments");
+    hipFree(dm);
+    if (!rc) rc = c->check_flags();
+    return rc;
+}
+
+int cvt_lloyd(cvt_ctx *ctx, int iterations, float omega)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (iterations < 0) return fail(CVT_ERR_ARG, "iterations %d", iterations);
+    if (!(omega > 0.0f && omega < 2.0f + 1e-6f)) return fail(CVT_ERR_ARG, "omega %g", omega);
+    return c->lloyd(iterations, omega);
+}
+
+int cvt_history_length(cvt_ctx *ctx)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    int st[NUM_STATE];
+    HIPCHK(hipMemcpyAsync(st, c->dstate, sizeof st, hipMemcpyDeviceToHost, c->stream), "state");
+    HIPCHK(hipStreamSynchronize(c->stream), "state");
+    return std::min(st[ST_HIST], c->hist_cap);
+}
+
+int cvt_get_history(cvt_ctx *ctx, double *eg, int n)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (n < 0 || n > c->hist_cap) return fail(CVT_ERR_ARG, "n %d", n);
+    if (n == 0) return CVT_OK;
+    if (!eg) return fail(CVT_ERR_ARG, "null output");
+    int rc = c->check_flags();
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(eg, c->hist, sizeof(double) * 2 * n, hipMemcpyDefault, c->stream),
+           "history");
+    HIPCHK(hipStreamSynchronize(c->stream), "history");
+    return CVT_OK;
+}
+
+int cvt_clear_history(cvt_ctx *ctx)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    HIPCHK(hipMemsetAsync(c->dstate + ST_HIST, 0, sizeof(int), c->stream), "history clear");
+    HIPCHK(hipStreamSynchronize(c->stream), "history clear");
+    return CVT_OK;
+}
+
+int cvt_energy(cvt_ctx *ctx, double *energy, double *gradnorm)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->have_density) return fail(CVT_ERR_STATE, "cvt_energy before cvt_set_density");
+    int rc = cvt_voronoi(ctx);
+    if (rc) return rc;
+    if ((rc = launch_moments(c, c->lab[c->final_buf]))) return rc;
+    if ((rc = c->enqueue_update(0, 1.0f, nullptr, 0, c->eg))) return rc;
+    const long n = (long)c->W * c->H;
+    const int nb = 1024;
+    hipLaunchKernelGGL(k_energy64, dim3(nb), dim3(256), 0, c->stream, c->lab[c->final_buf], c->rho,
+                       c->sites, c->S, c->W, c->H, c->blockpart);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->blockpart, nb, c->dstate,
+                       c->dflags, c->hist, c->hist_cap, 0, 1.0 / (double)n, 0.0, c->eg + 2);
+    double eg[4];
+    HIPCHK(hipMemcpyAsync(eg, c->eg, sizeof eg, hipMemcpyDeviceToHost, c->stream), "energy");
+    HIPCHK(hipStreamSynchronize(c->stream), "energy");
+    if ((rc = c->check_flags())) return rc;
+    if (energy) *energy = eg[2];
+    if (gradnorm) *gradnorm = eg[1];
+    // the voronoi above consumed the seeds; re-seed for the next iteration
+    c->enqueue_seed();
+    return CVT_OK;
+}
+
+int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->have_density) return fail(CVT_ERR_STATE, "cvt_profile before cvt_set_density");
+    if (iterations < 1 || !ms) return fail(CVT_ERR_ARG, "iterations %d", iterations);
+    const int np = jfa_num_passes(c->W, c->H);
+    const int nev = np + 5;
+    std::vector<hipEvent_t> ev(nev);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e), "event");
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int rc = CVT_OK;
+    for (int it = 0; it < iterations && !rc; ++it) {
+        if (!c->seeds_valid) c->enqueue_seed();
+        c->prof_events = ev.data();
+        hipEventRecord(ev[0], c->stream);
+        rc = c->enqueue_jfa();                      // records ev[1..np]
+        if (!rc) rc = launch_moments_timed(c, ev[np + 1], ev[np + 2]);
+        if (!rc) rc = c->enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr);
+        hipEventRecord(ev[np + 3], c->stream);
+        c->prof_events = nullptr;
+        if (hipEventSynchronize(ev[np + 3]) != hipSuccess) { rc = fail(CVT_ERR_HIP, "profile"); break; }
+        float t;
+        for (int k = 0; k < np; ++k) { hipEventElapsedTime(&t, ev[k], ev[k + 1]); acc[0] += t; }
+        hipEventElapsedTime(&t, ev[np], ev[np + 1]); acc[1] += t;      // moments
+        hipEventElapsedTime(&t, ev[np + 1], ev[np + 2]); acc[2] += t;  // overflow
+        hipEventElapsedTime(&t, ev[np + 2], ev[np + 3]); acc[3] += t;  // update + finalize
+        hipEventElapsedTime(&t, ev[0], ev[np + 3]); acc[5] += t;       // whole iteration
+    }
+    for (auto &e : ev) hipEventDestroy(e);
+    if (rc) return rc;
+    c->labels_fresh = true;
+    ms[0] = acc[0] / (iterations * np);
+    ms[1] = acc[1] / iterations;
+    ms[2] = acc[2] / iterations;
+    ms[3] = acc[3] / iterations;
+    ms[4] = 0.0;
+    ms[5] = acc[5] / iterations;
+    ms[6] = np;
+    ms[7] = c->tile;
+    return c->check_flags();
+}
+
+int cvt_synchronize(cvt_ctx *ctx)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    HIPCHK(hipStreamSynchronize(c->stream), "synchronize");
+    return c->check_flags();
+}
+
+int cvt_set_tile(cvt_ctx *ctx, int tile)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    c->tile_auto = (tile == 0);
+    return set_tile_internal(c, tile);
+}
+
+int cvt_get_tile(cvt_ctx *ctx)
+{
+    CHECK_CTX(ctx);
+    return reinterpret_cast<Ctx *>(ctx)->tile;
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+"""Python host API over the C ABI of include/cvt.h (one grid per Engine).
+
+Inputs may be numpy arrays (host) or torch tensors (host or the engine's
+device); outputs come back as numpy arrays unless a destination tensor is
+given.  The Python layer holds no compute: every call goes to
+libcvt_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import LABEL_NONE, call, lib
+
+
+def _ptr(a, dtype=None):
+    """(pointer, keep-alive) for a numpy array or torch tensor."""
+    if a is None:
+        return None, None
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(a.data_ptr()), a
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return arr.ctypes.data_as(C.c_void_p), arr
+
+
+class Engine:
+    """A CVT engine context for a width x height grid and up to max_sites sites."""
+
+    def __init__(self, width: int, height: int | None = None, max_sites: int = 0):
+        height = width if height is None else height
+        h = C.c_void_p()
+        call("cvt_create", C.byref(h), int(width), int(height), int(max_sites))
+        self._h = h
+        self.width, self.height, self.max_sites = int(width), int(height), int(max_sites)
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cvt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- inputs ----------------------------------------------------------
+    def set_stream(self, stream_handle: int | None):
+        call("cvt_set_stream", self._h, C.c_void_p(stream_handle or 0))
+
+    def set_density(self, density):
+        if hasattr(density, "numel"):
+            assert density.numel() == self.width * self.height
+        else:
+            density = np.ascontiguousarray(density, np.float32)
+            assert density.size == self.width * self.height, density.shape
+        p, keep = _ptr(density, np.float32)
+        call("cvt_set_density", self._h, p)
+
+    def set_sites(self, xy, fixed=None):
+        if hasattr(xy, "numel"):
+            n = xy.numel() // 2
+        else:
+            xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+            n = xy.shape[0]
+        p, k1 = _ptr(xy, np.float32)
+        f, k2 = _ptr(fixed, np.uint8)
+        call("cvt_set_sites", self._h, p, f, int(n))
+
+    # -- queries ---------------------------------------------------------
+    @property
+    def num_sites(self) -> int:
+        return call("cvt_num_sites", self._h)
+
+    def sites(self, out=None):
+        n = self.num_sites
+        if out is None:
+            out = np.empty((n, 2), np.float32)
+        p, keep = _ptr(out)
+        call("cvt_get_sites", self._h, p)
+        return out
+
+    def voronoi(self):
+        call("cvt_voronoi", self._h)
+
+    def labels(self, out=None):
+        if out is None:
+            out = np.empty((self.height, self.width), np.int32)
+        p, keep = _ptr(out)
+        call("cvt_get_labels", self._h, p)
+        return out
+
+    def moments(self):
+        n = self.num_sites
+        out = np.zeros((max(n, 1), 4), np.float64)
+        p, keep = _ptr(out)
+        call("cvt_moments", self._h, p)
+        return out[:n]
+
+    def lloyd(self, iterations: int, omega: float = 1.0):
+        call("cvt_lloyd", self._h, int(iterations), float(omega))
+
+    def history(self):
+        n = call("cvt_history_length", self._h)
+        out = np.zeros((max(n, 1), 2), np.float64)
+        p, keep = _ptr(out)
+        call("cvt_get_history", self._h, p, n)
+        return out[:n]
+
+    def clear_history(self):
+        call("cvt_clear_history", self._h)
+
+    def energy(self):
+        e = C.c_double()
+        g = C.c_double()
+        call("cvt_energy", self._h, C.byref(e), C.byref(g))
+        return e.value, g.value
+
+    def profile(self, iterations: int = 5, omega: float = 1.0) -> dict:
+        """Eager Lloyd steps timed per kernel with HIP events (ms)."""
+        ms = np.zeros(8, np.float64)
+        p, keep = _ptr(ms)
+        call("cvt_profile", self._h, int(iterations), float(omega), p)
+        return {"jfa_pass": ms[0], "moments": ms[1], "overflow": ms[2], "update": ms[3],
+                "step": ms[5], "jfa_passes": int(ms[6]), "tile": int(ms[7])}
+
+    def synchronize(self):
+        call("cvt_synchronize", self._h)
+
+    @property
+    def tile(self) -> int:
+        return call("cvt_get_tile", self._h)
+
+    @tile.setter
+    def tile(self, t: int):
+        call("cvt_set_tile", self._h, int(t))
+
+
+__all__ = ["Engine", "LABEL_NONE"]

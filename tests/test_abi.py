@@ -1,0 +1,97 @@
+"""CPU-side checks of the C ABI and host logic (no GPU compute).
+
+The engine library must load and export every entry point include/*.h
+declares; argument validation that happens before any device call is
+exercised here; the workload generators are deterministic.
+"""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(cvt_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for n in ["cvt_create", "cvt_destroy", "cvt_set_density", "cvt_set_sites", "cvt_voronoi",
+              "cvt_moments", "cvt_lloyd", "cvt_energy", "cvt_get_labels"]:
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    import cvtamd
+    L = C.CDLL(str(cvtamd.LIB_PATH))
+    missing = [n for n in declared_symbols() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    from cvtamd import _lib
+    assert declared_symbols() <= set(_lib.SIGNATURES), declared_symbols() - set(_lib.SIGNATURES)
+
+
+def test_version_and_argument_errors_without_gpu():
+    import cvtamd
+    L = cvtamd.lib()
+    assert L.cvt_version() == 100
+    h = C.c_void_p()
+    assert L.cvt_create(C.byref(h), 0, 64, 10) == -1
+    assert b"grid" in L.cvt_last_error()
+    assert L.cvt_create(C.byref(h), 64, 64, -5) == -1
+    assert L.cvt_create(None, 64, 64, 5) == -1
+    assert L.cvt_lloyd(None, 1, 1.0) == -1
+    assert L.cvt_destroy(None) == 0
+
+
+def test_python_error_type():
+    import cvtamd
+    with pytest.raises(cvtamd.CvtError) as e:
+        cvtamd.Engine(0, 0, 1)
+    assert e.value.status == -1
+
+
+def test_workloads_deterministic():
+    from cvtamd import workloads as wl
+    a = wl.config2(5, n=256, num_sites=300)
+    b = wl.config2(5, n=256, num_sites=300)
+    assert np.array_equal(a.density, b.density) and np.array_equal(a.sites, b.sites)
+    c = wl.config2(6, n=256, num_sites=300)
+    assert not np.array_equal(a.sites, c.sites)
+
+
+def test_workload_shapes_and_domains():
+    from cvtamd import workloads as wl
+    w1 = wl.config1()
+    assert w1.n == 512 and w1.num_sites == 1024 and w1.iterations == 50
+    assert (w1.density == 1).all() and w1.sites.dtype == np.float32
+    assert (w1.sites >= 0).all() and (w1.sites < 512).all()
+    w2 = wl.config2(2, n=256, num_sites=500)
+    m = wl.disk_mask(256)
+    assert (w2.density[~m] == 0).all() and (w2.density[m] >= 0.05).all()
+    px = np.floor(w2.sites).astype(int)
+    assert (w2.density[px[:, 1], px[:, 0]] > 0).all()
+    w3 = wl.config3(3, n=256, num_sites=400)
+    assert (w3.density > 0).sum() > 0.3 * 256 * 256
+    assert (w3.density >= 0).all()
+
+
+def test_batch_patch_rules():
+    from cvtamd import workloads as wl
+    for s in range(6):
+        p = wl.batch_patch(s)
+        assert p.n % 64 == 0 and 256 <= p.n <= 2048
+        assert 256 <= p.num_sites <= 16384
+        assert 4 <= p.meta["components"] <= 12

@@ -1,0 +1,263 @@
+"""GPU parity of the Lloyd hot path (JFA(+1), moments, update, energy) against the
+CPU oracle, through the C ABI.
+
+Bar: labels bit-exact; per-site moments within the stated fp32 bound of the
+oracle's float64 (centroid offset 1e-4 px, cell energy 1e-4 relative); energy
+per iteration 1e-5 relative; the float64 energy check 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make_engine(n, xy, rho, H=None, tile=0):
+    import cvtamd
+    H = n if H is None else H
+    e = cvtamd.Engine(n, H, max(len(xy), 1))
+    e.set_density(rho)
+    if tile:
+        e.tile = tile
+    e.set_sites(xy)
+    return e
+
+
+def rand_sites(seed, W, H, S, lo=0.0):
+    g = np.random.default_rng(seed)
+    xy = g.random((S, 2))
+    xy[:, 0] = lo + xy[:, 0] * (W - 2 * lo)
+    xy[:, 1] = lo + xy[:, 1] * (H - 2 * lo)
+    return xy.astype(np.float32)
+
+
+def check_moments(m, ref, W_tol=1e-6):
+    w, wr = m[:, 0], ref[:, 0]
+    assert np.allclose(w, wr, rtol=W_tol, atol=1e-9)
+    nz = wr > 0
+    off = m[nz, 1:3] / w[nz, None]
+    off_r = ref[nz, 1:3] / wr[nz, None]
+    assert np.abs(off - off_r).max() < 1e-4
+    assert np.allclose(m[nz, 3], ref[nz, 3], rtol=1e-4, atol=1e-6)
+    assert np.all(m[~nz] == 0)
+
+
+# ----------------------------------------------------------------- labels --
+@pytest.mark.parametrize("W,H,S,seed", [
+    (512, 512, 1024, 1),      # config 1 shape
+    (256, 256, 4096, 2),      # dense: many seed collisions
+    (576, 576, 700, 3),       # non power of two (batched patch side)
+    (320, 200, 333, 4),       # rectangular, not a multiple of 64 in y
+    (100, 70, 1, 5),          # one site
+    (64, 64, 64, 6),
+])
+def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
+    xy = rand_sites(seed, W, H, S)
+    rho = np.ones((H, W), np.float32)
+    with make_engine(W, xy, rho, H) as e:
+        e.voronoi()
+        lab = e.labels()
+    assert np.array_equal(lab, orc.jfa(W, H, xy))
+
+
+def test_jfa_awkward_sites(gpu, orc):
+    n = 200
+    xy = rand_sites(7, n, n, 300)
+    xy[:20] = xy[20:40]                                   # coincident sites
+    xy[40:60] = np.floor(xy[40:60]) + 0.5                 # pixel centres: exact ties
+    xy[60] = [-30.0, 50.0]                                # outside the grid
+    xy[61] = [n + 10.5, n + 3.0]
+    with make_engine(n, xy, np.ones((n, n), np.float32)) as e:
+        e.voronoi()
+        assert np.array_equal(e.labels(), orc.jfa(n, n, xy))
+
+
+def test_no_sites(gpu, orc):
+    import cvtamd
+    n = 64
+    with cvtamd.Engine(n, n, 8) as e:
+        e.set_density(np.ones((n, n), np.float32))
+        e.set_sites(np.zeros((0, 2), np.float32))
+        e.voronoi()
+        assert (e.labels() == cvtamd.LABEL_NONE).all()
+        assert e.moments().shape == (0, 4)
+        e.lloyd(2)
+        h = e.history()
+        assert h.shape == (2, 2) and (h == 0).all()
+
+
+# ---------------------------------------------------------------- moments --
+@pytest.mark.parametrize("tile", [32, 64, 128, 256])
+def test_moments_each_tile(gpu, orc, tile):
+    from cvtamd import workloads as wl
+    w = wl.config2(11, n=512, num_sites=1024)
+    with make_engine(w.n, w.sites, w.density, tile=tile) as e:
+        assert e.tile == tile
+        e.voronoi()
+        lab = e.labels()
+        m = e.moments()
+    assert np.array_equal(lab, orc.jfa(w.n, w.n, w.sites))
+    check_moments(m, orc.moments(w.density, lab, w.sites))
+
+
+def test_moments_overflow_path(gpu, orc):
+    # 40 sites on a 512 grid: cells reach far beyond a 32-px tile's 3x3 window
+    n = 512
+    xy = rand_sites(12, n, n, 40)
+    g = np.random.default_rng(13)
+    rho = (0.1 + g.random((n, n))).astype(np.float32)
+    with make_engine(n, xy, rho, tile=32) as e:
+        e.voronoi()
+        lab = e.labels()
+        m = e.moments()
+    check_moments(m, orc.moments(rho, lab, xy))
+
+
+def test_moments_lattice_exact(gpu, orc):
+    # a x a lattice at block centres: W = a^2, DX = DY = 0, E = a^2 (a^2 - 1) / 6
+    a, k = 16, 8
+    n = a * k
+    c = (np.arange(k) * a + a / 2).astype(np.float32)
+    xy = np.stack(np.meshgrid(c, c), -1).reshape(-1, 2).astype(np.float32)
+    with make_engine(n, xy, np.ones((n, n), np.float32)) as e:
+        e.voronoi()
+        m = e.moments()
+        e.lloyd(3)
+        out = e.sites()
+    assert np.all(m[:, 0] == a * a)
+    assert np.abs(m[:, 1:3]).max() < 1e-9
+    assert np.allclose(m[:, 3], a * a * (a * a - 1) / 6, rtol=1e-9)
+    assert np.array_equal(out, xy)
+
+
+def test_capacity_error(gpu):
+    import cvtamd
+    # 2000 sites inside one 64x64 tile exceed its 1024-entry table
+    n = 256
+    g = np.random.default_rng(3)
+    xy = (g.random((2000, 2)) * 64).astype(np.float32)
+    with make_engine(n, xy, np.ones((n, n), np.float32), tile=64) as e:
+        e.voronoi()
+        with pytest.raises(cvtamd.CvtError) as err:
+            e.moments()
+        assert err.value.status == -5
+
+
+# ------------------------------------------------------------------ lloyd --
+def test_lloyd_config1_trajectory(gpu, orc):
+    from cvtamd import workloads as wl
+    w = wl.config1()
+    with make_engine(w.n, w.sites, w.density) as e:
+        e.lloyd(w.iterations, 1.0)
+        h = e.history()
+        out = e.sites()
+    sites_r, e_r, g_r = orc.lloyd(w.density, w.sites, w.iterations)
+    assert h.shape == (w.iterations, 2)
+    assert np.abs(h[:, 0] - e_r).max() / e_r.max() < 1e-5
+    assert np.all(np.abs(h[:, 1] - g_r) <= 1e-3 * g_r + 1e-9)
+    assert np.abs(out - sites_r).max() < 1e-2
+    assert h[-1, 0] < h[0, 0]
+
+
+def test_lloyd_one_step_matches_update(gpu, orc):
+    from cvtamd import workloads as wl
+    w = wl.config2(21, n=1024, num_sites=4096)
+    with make_engine(w.n, w.sites, w.density) as e:
+        e.lloyd(1, 1.0)
+        out = e.sites()
+    lab = orc.jfa(w.n, w.n, w.sites)
+    ref = orc.update(orc.moments(w.density, lab, w.sites), w.sites, 1.0)
+    assert np.abs(out - ref).max() < 1e-4
+
+
+def test_lloyd_deterministic_and_graph_equals_eager(gpu):
+    from cvtamd import workloads as wl
+    w = wl.config2(5, n=512, num_sites=2000)
+    runs = []
+    for _ in range(2):
+        with make_engine(w.n, w.sites, w.density) as e:
+            e.lloyd(7, 1.0)
+            runs.append((e.sites(), e.history()))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_fixed_and_zero_density(gpu, orc):
+    n, S = 128, 50
+    xy = rand_sites(9, n, n, S)
+    fixed = np.zeros(S, np.uint8)
+    fixed[::7] = 1
+    rho = np.ones((n, n), np.float32)
+    rho[:, :40] = 0.0
+    import cvtamd
+    with cvtamd.Engine(n, n, S) as e:
+        e.set_density(rho)
+        e.set_sites(xy, fixed)
+        e.lloyd(4)
+        out = e.sites()
+    ref, _, _ = orc.lloyd(rho, xy, 4, fixed=fixed)
+    assert np.array_equal(out[fixed == 1], xy[fixed == 1])
+    assert np.abs(out - ref).max() < 1e-3
+
+
+def test_energy_fp64(gpu, orc):
+    from cvtamd import workloads as wl
+    w = wl.config2(8, n=1024, num_sites=3000)
+    with make_engine(w.n, w.sites, w.density) as e:
+        E, G = e.energy()
+        e.lloyd(1)
+        h = e.history()
+    lab = orc.jfa(w.n, w.n, w.sites)
+    Er, Gr = orc.energy_from_moments(w.n, w.n, orc.moments(w.density, lab, w.sites))
+    assert E == pytest.approx(Er, rel=1e-12)
+    assert G == pytest.approx(Gr, rel=1e-6)
+    assert h[0, 0] == pytest.approx(Er, rel=1e-6)
+
+
+def test_density_validation(gpu):
+    import cvtamd
+    n = 32
+    with cvtamd.Engine(n, n, 4) as e:
+        bad = np.ones((n, n), np.float32)
+        bad[3, 3] = np.nan
+        with pytest.raises(cvtamd.CvtError):
+            e.set_density(bad)
+        bad[3, 3] = -1
+        with pytest.raises(cvtamd.CvtError):
+            e.set_density(bad)
+        with pytest.raises(cvtamd.CvtError):
+            e.lloyd(1)
+
+
+def test_device_pointers(gpu, orc):
+    import torch
+    from cvtamd import workloads as wl
+    w = wl.config2(31, n=256, num_sites=256)
+    rho = torch.from_numpy(w.density).cuda()
+    xy = torch.from_numpy(w.sites).cuda()
+    import cvtamd
+    with cvtamd.Engine(w.n, w.n, w.num_sites) as e:
+        e.set_density(rho)
+        e.set_sites(xy)
+        e.voronoi()
+        lab = torch.empty((w.n, w.n), dtype=torch.int32, device="cuda")
+        e.labels(lab)
+        out = torch.empty_like(xy)
+        e.sites(out)
+    assert np.array_equal(lab.cpu().numpy(), orc.jfa(w.n, w.n, w.sites))
+    assert torch.equal(out, xy)
+
+
+# ------------------------------------------------ full-size property checks --
+@pytest.mark.parametrize("cfg", ["config2", "config3"])
+def test_full_size_labels_and_mass(gpu, orc, cfg):
+    from cvtamd import workloads as wl
+    w = getattr(wl, cfg)()
+    with make_engine(w.n, w.sites, w.density) as e:
+        e.voronoi()
+        lab = e.labels()
+        m = e.moments()
+    assert np.array_equal(lab, orc.jfa(w.n, w.n, w.sites))
+    # mass conservation and centroid inside the grid
+    assert m[:, 0].sum() == pytest.approx(float(w.density.astype(np.float64).sum()), rel=1e-9)
+    ref = orc.moments(w.density, lab, w.sites)
+    check_moments(m, ref)
