@@ -65,9 +65,10 @@ const char *cvt_last_error(void);
 /* ABI version (major * 100 + minor). */
 int cvt_version(void);
 
-/* A context for one grid: allocates every device buffer it will use (the
- * reference allocates per call, source/gcvt.cu:859-874).  max_sites bounds
- * the site count of later calls. */
+/* A context for one grid of at most 8192 x 8192 pixels (the reference's
+ * limit, source/gcvt.cu:842-847): allocates every device buffer it will use
+ * (the reference allocates per call, source/gcvt.cu:859-874).  max_sites
+ * bounds the site count of later calls. */
 int cvt_create(cvt_ctx **ctx, int width, int height, int max_sites);
 int cvt_destroy(cvt_ctx *ctx);
 /* Launch subsequent work on this hipStream_t (NULL = the context's own
@@ -75,9 +76,10 @@ int cvt_destroy(cvt_ctx *ctx);
 int cvt_set_stream(cvt_ctx *ctx, void *hip_stream);
 /* width*height fp32 densities, finite and >= 0. */
 int cvt_set_density(cvt_ctx *ctx, const float *density);
-/* num_sites (x, y) fp32 pairs in pixel units; fixed (optional, num_sites
- * bytes) marks sites the update never moves (the reference's constraint
- * mask, source/gcvt.h:106-122). */
+/* num_sites (x, y) fp32 pairs in pixel units, finite and inside
+ * [-width, 2 width] x [-height, 2 height]; fixed (optional, num_sites bytes)
+ * marks sites the update never moves (the reference's constraint mask,
+ * source/gcvt.h:106-122). */
 int cvt_set_sites(cvt_ctx *ctx, const float *xy, const uint8_t *fixed, int num_sites);
 int cvt_get_sites(cvt_ctx *ctx, float *xy);
 /* Current site count (>= 0) or a negative status. */

@@ -5,6 +5,7 @@
 
 #include "../../include/cvt.h"
 #include "cvt_kernels.h"
+#include "jfa.h"
 
 namespace cvt {
 
@@ -23,12 +24,15 @@ struct Ctx {
     float *rho = nullptr;          // [H][W]
     int rho_exp = 0;               // rho' = rho * 2^-rho_exp <= 1
     bool have_density = false;
-    int32_t *seed = nullptr;       // seed map [H][W]
+    uint32_t *seed = nullptr;      // seed map [H][W]
     bool seeds_valid = false;      // seed map holds the current sites
-    int32_t *lab[2] = {nullptr, nullptr};
+    uint32_t *lab[2] = {nullptr, nullptr};
     int final_buf = 0;             // lab[final_buf] = labels of the last JFA
     bool labels_fresh = false;
-    float2 *sites = nullptr;       // [maxS + 1]
+    float2 *sites = nullptr;       // [capS + 1], sentinel at capS
+    int capS = 1;                  // max(maxS, 1)
+    uint32_t none() const { return 8u * (uint32_t)capS; }
+    JfaPlan jplan;                 // launch shape of the JFA passes
     uint8_t *fixed = nullptr;      // [maxS]
 
     int tile = 0, tilesX = 0, tilesY = 0;
@@ -37,7 +41,7 @@ struct Ctx {
     OvfRec *ovf = nullptr;         // [tiles][ovf_cap]
     int ovf_cap = 0;
     int *ovf_count = nullptr;      // [tiles]
-    double *ovf_acc = nullptr;     // [maxS][4]
+    Acc128 *ovf_acc = nullptr;     // [maxS]
     double *blockpart = nullptr;
     double *hist = nullptr;        // [hist_cap][2]
     int hist_cap = 0;

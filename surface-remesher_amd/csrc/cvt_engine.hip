@@ -57,7 +57,7 @@ static int choose_tile(int W, int H, int S)
     return t;
 }
 
-static int launch_moments(Ctx *c, const int32_t *lab)
+static int launch_moments(Ctx *c, const uint32_t *lab)
 {
     const float rs = (float)ldexp(1.0, -c->rho_exp);
     dim3 grid(c->tilesX * c->tilesY), block(256);
@@ -160,13 +160,11 @@ int Ctx::enqueue_jfa()
 {
     const int np = jfa_num_passes(W, H);
     int step = jfa_first_step(W, H);
-    dim3 block(64, 4), grid((W + 63) / 64, (H + 3) / 4);
-    const int32_t *in = seed;
+    const uint32_t *in = seed;
     int cur = 0;
     for (int k = 0; k < np; ++k) {
-        int32_t *out = lab[cur];
-        hipLaunchKernelGGL(k_jfa, grid, block, 0, stream, in, out, sites, (unsigned)S, W, H, step,
-                           k == 1 ? seed : nullptr);
+        uint32_t *out = lab[cur];
+        launch_jfa_pass(stream, jplan, in, out, sites, W, H, step, k == 1 ? seed : nullptr, none());
         if (prof_events) hipEventRecord(prof_events[k + 1], stream);
         in = out;
         cur ^= 1;
@@ -278,11 +276,13 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     clear_error();
     if (!out) return fail(CVT_ERR_ARG, "null output pointer");
     *out = nullptr;
-    if (W < 1 || H < 1 || W > 16384 || H > 16384)
-        return fail(CVT_ERR_ARG, "grid %dx%d outside [1, 16384]^2", W, H);
+    if (W < 1 || H < 1 || W > 8192 || H > 8192)
+        return fail(CVT_ERR_ARG, "grid %dx%d outside [1, 8192]^2", W, H);
     if (max_sites < 0 || max_sites > (1 << 28)) return fail(CVT_ERR_ARG, "max_sites %d", max_sites);
     Ctx *c = new Ctx();
     c->W = W; c->H = H; c->maxS = max_sites; c->S = 0;
+    c->capS = std::max(max_sites, 1);
+    c->jplan = plan_jfa(W, H, c->capS);
     hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -295,13 +295,13 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     c->hist_cap = 1 << 16;
     const int nb = (int)std::max(1L, (ms + 255) / 256);
     bool ok = hipMalloc(&c->rho, sizeof(float) * n) == hipSuccess &&
-              hipMalloc(&c->seed, sizeof(int32_t) * n) == hipSuccess &&
-              hipMalloc(&c->lab[0], sizeof(int32_t) * n) == hipSuccess &&
-              hipMalloc(&c->lab[1], sizeof(int32_t) * n) == hipSuccess &&
-              hipMalloc(&c->sites, sizeof(float2) * (ms + 1)) == hipSuccess &&
+              hipMalloc(&c->seed, sizeof(uint32_t) * n) == hipSuccess &&
+              hipMalloc(&c->lab[0], sizeof(uint32_t) * n) == hipSuccess &&
+              hipMalloc(&c->lab[1], sizeof(uint32_t) * n) == hipSuccess &&
+              hipMalloc(&c->sites, sizeof(float2) * (c->capS + 2)) == hipSuccess &&
               hipMalloc(&c->fixed, ms) == hipSuccess &&
               hipMalloc(&c->slots, sizeof(Slot) * 9 * ms) == hipSuccess &&
-              hipMalloc(&c->ovf_acc, sizeof(double) * 4 * ms) == hipSuccess &&
+              hipMalloc(&c->ovf_acc, sizeof(Acc128) * ms) == hipSuccess &&
               hipMalloc(&c->blockpart, sizeof(double) * 2 * std::max(nb, 4096)) == hipSuccess &&
               hipMalloc(&c->hist, sizeof(double) * 2 * c->hist_cap) == hipSuccess &&
               hipMalloc(&c->dstate, sizeof(int) * NUM_STATE) == hipSuccess &&
@@ -313,18 +313,18 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
         return fail(CVT_ERR_NOMEM, "device allocation for a %dx%d grid, %d sites", W, H, max_sites);
     }
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, c->seed, (int32_t)CVT_NONE, n);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, c->lab[0], (int32_t)CVT_NONE, n);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, c->lab[1], (int32_t)CVT_NONE, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->seed, c->none(), n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[0], c->none(), n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[1], c->none(), n);
     hipMemsetAsync(c->rho, 0, sizeof(float) * n, s);
     hipMemsetAsync(c->fixed, 0, ms, s);
     hipMemsetAsync(c->slots, 0, sizeof(Slot) * 9 * ms, s);
-    hipMemsetAsync(c->ovf_acc, 0, sizeof(double) * 4 * ms, s);
+    hipMemsetAsync(c->ovf_acc, 0, sizeof(Acc128) * ms, s);
     hipMemsetAsync(c->dflags, 0, sizeof(int) * NUM_FLAGS, s);
     int st[NUM_STATE] = {1, 0, 0, 0};
     hipMemcpyAsync(c->dstate, st, sizeof st, hipMemcpyHostToDevice, s);
     float2 sentinel = make_float2(__builtin_inff(), __builtin_inff());
-    hipMemcpyAsync(c->sites, &sentinel, sizeof sentinel, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(c->sites + c->capS, &sentinel, sizeof sentinel, hipMemcpyHostToDevice, s);
     int rc = set_tile_internal(c, 64);
     if (rc == CVT_OK && hipStreamSynchronize(s) != hipSuccess)
         rc = fail(CVT_ERR_HIP, "initialisation failed");
@@ -401,8 +401,8 @@ int cvt_set_sites(cvt_ctx *ctx, const float *xy, const uint8_t *fixed, int num_s
     if (num_sites > 0 && !xy) return fail(CVT_ERR_ARG, "null site array");
     // the seed map is all CVT_NONE unless seeds are pending: clear them first
     if (c->seeds_valid && c->S > 0)
-        hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, c->stream, c->seed,
-                           (int32_t)CVT_NONE, (long)c->W * c->H);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, c->seed, c->none(),
+                           (long)c->W * c->H);
     if (num_sites > 0) {
         HIPCHK(hipMemcpyAsync(c->sites, xy, sizeof(float2) * num_sites, hipMemcpyDefault, c->stream),
                "site upload");
@@ -412,10 +412,22 @@ int cvt_set_sites(cvt_ctx *ctx, const float *xy, const uint8_t *fixed, int num_s
         else
             HIPCHK(hipMemsetAsync(c->fixed, 0, num_sites, c->stream), "fixed clear");
     }
-    float2 sentinel = make_float2(__builtin_inff(), __builtin_inff());
-    HIPCHK(hipMemcpyAsync(c->sites + num_sites, &sentinel, sizeof sentinel, hipMemcpyHostToDevice,
-                          c->stream), "sentinel");
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(c->dflags + FLAG_BAD, &bad, sizeof bad, hipMemcpyHostToDevice, c->stream),
+           "flag reset");
+    if (num_sites > 0)
+        hipLaunchKernelGGL(k_site_check, dim3((num_sites + 255) / 256), dim3(256), 0, c->stream,
+                           c->sites, num_sites, c->W, c->H, c->dflags);
+    HIPCHK(hipMemcpyAsync(&bad, c->dflags + FLAG_BAD, sizeof bad, hipMemcpyDeviceToHost, c->stream),
+           "site check");
     HIPCHK(hipStreamSynchronize(c->stream), "site upload");
+    if (bad) {
+        c->S = 0;
+        c->seeds_valid = false;
+        c->labels_fresh = false;
+        c->graph_valid = false;
+        return fail(CVT_ERR_ARG, "a site is not finite or lies outside [-W, 2W] x [-H, 2H]");
+    }
     const int oldS = c->S;
     c->S = num_sites;
     c->seeds_valid = false;
@@ -466,8 +478,12 @@ int cvt_get_labels(cvt_ctx *ctx, int32_t *labels)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!labels) return fail(CVT_ERR_ARG, "null output");
     if (!c->labels_fresh) return fail(CVT_ERR_STATE, "no label map yet (call cvt_voronoi)");
-    HIPCHK(hipMemcpyAsync(labels, c->lab[c->final_buf], sizeof(int32_t) * (long)c->W * c->H,
-                          hipMemcpyDefault, c->stream), "label download");
+    const long n = (long)c->W * c->H;
+    int32_t *tmp = reinterpret_cast<int32_t *>(c->lab[c->final_buf ^ 1]);
+    hipLaunchKernelGGL(k_export_labels, dim3(1024), dim3(256), 0, c->stream, c->lab[c->final_buf],
+                       tmp, n, c->none());
+    HIPCHK(hipMemcpyAsync(labels, tmp, sizeof(int32_t) * n, hipMemcpyDefault, c->stream),
+           "label download");
     HIPCHK(hipStreamSynchronize(c->stream), "label download");
     return CVT_OK;
 }

@@ -3,12 +3,15 @@
 // Data layout in HBM (one grid of W x H pixels, S sites):
 //   rho      fp32 [H][W]      density, unnormalised (scaled by rho_scale = 2^-k
 //                             in-kernel so that every scaled value is <= 1)
-//   seed     int32 [H][W]     seed map: site index on the pixel holding a site,
-//                             CVT_NONE elsewhere; read by JFA pass 0, cleared by
-//                             pass 1, re-seeded by the site update
-//   lab[2]   int32 [H][W]     JFA ping-pong label maps
-//   sites    float2 [S + 1]   site table, pixel units; sites[S] = (inf, inf) is
-//                             the sentinel every CVT_NONE label gathers
+//   seed     uint32 [H][W]    seed map: label of the site on that pixel, `none`
+//                             elsewhere; read by JFA pass 0, cleared by pass 1,
+//                             re-seeded by the site update
+//   lab[2]   uint32 [H][W]    JFA ping-pong label maps
+//   sites    float2 [capS+1]  site table, pixel units; sites[capS] = (inf, inf)
+//                             is the sentinel
+// A label is the byte offset of its site in the table (8 * index) and `none`
+// = 8 * capS, so the JFA gather needs no address arithmetic; labels leave the
+// engine as int32 indices (CVT_NONE for none).
 //   slots    Slot [S][9]      per-site partial moments, one per tile of the 3x3
 //                             tile window around the site's home tile (unique
 //                             writer per slot, tagged with the epoch)
@@ -26,16 +29,31 @@
 
 #define CVT_NONE 0x7fffffff
 
+// Fixed-point moment sums of one (tile, site) pair: W = sum rho', DX = sum
+// rho' (px - sx), DY = sum rho' (py - sy), E = sum rho' |p - s|^2, scaled by
+// 2^MOM_FW, 2^MOM_FX, 2^MOM_FX, 2^MOM_FE.  rho' = rho * 2^-k <= 1, a tile has
+// at most 256^2 = 2^16 pixels and |p - s| < 2^15 (sites stay inside
+// [-W, 2W] x [-H, 2H], W, H <= 8192), so each tile sum fits in int64; the
+// per-site totals are summed in 128 bits.  Integer sums are exact, hence
+// independent of order and grouping.
+#define MOM_FW 46
+#define MOM_FX 31
+#define MOM_FE 15
+
 struct __align__(8) Slot {
-    double w, dx, dy, e;   // sum rho', sum rho'(px - sx), sum rho'(py - sy), sum rho'|p - s|^2
+    long long w, dx, dy, e;
     int epoch;
     int pad;
 };
 
 struct __align__(8) OvfRec {
-    double w, dx, dy, e;
+    long long w, dx, dy, e;
     int label;
     int pad;
+};
+
+struct __align__(16) Acc128 {   // one 128-bit accumulator per moment
+    __int128 v[4];
 };
 
 // device flags
@@ -56,7 +74,7 @@ __device__ __forceinline__ float cvt_dist(float2 s, float px, float py)
 // ---------------------------------------------------------------------------
 // fills, seeding
 // ---------------------------------------------------------------------------
-__global__ void k_fill_i32(int32_t *__restrict__ p, int32_t v, long n)
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, long n)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long stride = (long)gridDim.x * blockDim.x;
@@ -70,56 +88,40 @@ __device__ __forceinline__ int cvt_pixel_of(float v, int n)
     return (int)f;
 }
 
+// Sites must be finite and inside [-W, 2W] x [-H, 2H] (keeps |p - s| < 2^15
+// for the fixed-point moments); a violation raises FLAG_BAD.
+__global__ void k_site_check(const float2 *__restrict__ sites, int S, int W, int H,
+                             int *__restrict__ dflags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    const float2 s = sites[i];
+    const bool ok = s.x >= -(float)W && s.x <= 2.0f * W && s.y >= -(float)H && s.y <= 2.0f * H;
+    if (!ok) dflags[FLAG_BAD] = 1;
+}
+
 // Every site claims its pixel; the lowest index wins a shared pixel.  The
 // integer atomicMin makes the result independent of thread order.
-__global__ void k_seed(const float2 *__restrict__ sites, int S, int32_t *__restrict__ seed,
+__global__ void k_seed(const float2 *__restrict__ sites, int S, uint32_t *__restrict__ seed,
                        int W, int H)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S) return;
     float2 s = sites[i];
     int x = cvt_pixel_of(s.x, W), y = cvt_pixel_of(s.y, H);
-    atomicMin(&seed[(long)y * W + x], i);
+    atomicMin(&seed[(long)y * W + x], 8u * (uint32_t)i);
 }
 
-// ---------------------------------------------------------------------------
-// JFA pass (reference replacement for pba2DCompute, source/gcvt.cu:967-978)
-// ---------------------------------------------------------------------------
-// One Jacobi pass: out(p) = argmin over the 9 labels at p + step*{-1,0,1}^2
-// inside the grid, order (d, index).  Block 64 x 4: a wavefront is one
-// 64-pixel row segment, so the 9 label reads of a wave are three coalesced
-// rows and the y bounds test is wave-uniform.  If clear is non-null the
-// pass also resets that map to CVT_NONE (pass 1 clears the seed map read by
-// pass 0).
-__global__ __launch_bounds__(256) void k_jfa(const int32_t *__restrict__ in,
-                                              int32_t *__restrict__ out,
-                                              const float2 *__restrict__ sites, unsigned S,
-                                              int W, int H, int step,
-                                              int32_t *__restrict__ clear)
+// engine labels (byte offsets) -> int32 indices, CVT_NONE for none
+__global__ void k_export_labels(const uint32_t *__restrict__ lab, int32_t *__restrict__ out, long n,
+                                uint32_t none)
 {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
-    if (x >= W || y >= H) return;
-    const float px = (float)x + 0.5f, py = (float)y + 0.5f;
-    float bd = __builtin_inff();
-    int bl = CVT_NONE;
-#pragma unroll
-    for (int oy = -1; oy <= 1; ++oy) {
-        const int yy = y + oy * step;
-        if ((unsigned)yy >= (unsigned)H) continue;
-        const int32_t *row = in + (long)yy * W;
-#pragma unroll
-        for (int ox = -1; ox <= 1; ++ox) {
-            const int xx = x + ox * step;
-            const int l = ((unsigned)xx < (unsigned)W) ? row[xx] : CVT_NONE;
-            const float2 s = sites[min((unsigned)l, S)];
-            const float d = cvt_dist(s, px, py);
-            if (d < bd || (d == bd && l < bl)) { bd = d; bl = l; }
-        }
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const uint32_t l = lab[i];
+        out[i] = (l >= none) ? CVT_NONE : (int32_t)(l >> 3);
     }
-    const long p = (long)y * W + x;
-    out[p] = bl;
-    if (clear) clear[p] = CVT_NONE;
 }
 
 // ---------------------------------------------------------------------------
@@ -132,10 +134,6 @@ template <int T> struct TileCfg {
     static constexpr int RPW = 64 / SEG;              // rows per wave step
     static constexpr int SPR = T / SEG;               // segments per row
     static constexpr int GROUPS = T * T / 64;         // wave steps per tile
-    // fixed-point scales: tile-relative sums fit in int64 when rho' <= 1
-    static constexpr int FW = 62 - 2 * LOG;
-    static constexpr int FB = 63 - 3 * LOG;
-    static constexpr int FE = 63 - 4 * LOG;
 };
 
 #define CVT_HASH_CAP 1024
@@ -147,7 +145,7 @@ __device__ __forceinline__ long long cvt_fix(float v, int f)
 
 template <int T>
 __global__ __launch_bounds__(256) void k_moments(
-    const int32_t *__restrict__ lab, const float *__restrict__ rho,
+    const uint32_t *__restrict__ lab, const float *__restrict__ rho,
     const float2 *__restrict__ sites, int S, int W, int H, float rho_scale, int tilesX,
     int tilesY, Slot *__restrict__ slots, OvfRec *__restrict__ ovf, int ovf_cap,
     int *__restrict__ ovf_count, int *__restrict__ dflags, const int *__restrict__ dstate)
@@ -167,7 +165,6 @@ __global__ __launch_bounds__(256) void k_moments(
     if (threadIdx.x == 0) { n_over = 0; full = 0; }
     __syncthreads();
 
-    const float cx0 = (float)tx0 + 0.5f * T, cy0 = (float)ty0 + 0.5f * T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int seg_lane = lane % C::SEG;
 
@@ -179,13 +176,14 @@ __global__ __launch_bounds__(256) void k_moments(
         float a = 0.f, b = 0.f, cc = 0.f, e = 0.f;
         if (x < W && y < H) {
             const long p = (long)y * W + x;
-            const int l = lab[p];
+            const int l = (int)(lab[p] >> 3);
             const float rv = rho[p];
             if ((unsigned)l < (unsigned)S && rv != 0.0f) {
                 key = l;
+                const float2 st = sites[l];
                 const float rs = rv * rho_scale;
-                const float dx = ((float)x + 0.5f) - cx0;
-                const float dy = ((float)y + 0.5f) - cy0;
+                const float dx = ((float)x + 0.5f) - st.x;
+                const float dy = ((float)y + 0.5f) - st.y;
                 a = rs;
                 b = rs * dx;
                 cc = rs * dy;
@@ -218,10 +216,10 @@ __global__ __launch_bounds__(256) void k_moments(
                 if (old == -1 || old == key) { slot = h; break; }
             }
             if (slot >= 0) {
-                atomicAdd(&acc[slot][0], (unsigned long long)cvt_fix(a, C::FW));
-                atomicAdd(&acc[slot][1], (unsigned long long)cvt_fix(b, C::FB));
-                atomicAdd(&acc[slot][2], (unsigned long long)cvt_fix(cc, C::FB));
-                atomicAdd(&acc[slot][3], (unsigned long long)cvt_fix(e, C::FE));
+                atomicAdd(&acc[slot][0], (unsigned long long)cvt_fix(a, MOM_FW));
+                atomicAdd(&acc[slot][1], (unsigned long long)cvt_fix(b, MOM_FX));
+                atomicAdd(&acc[slot][2], (unsigned long long)cvt_fix(cc, MOM_FX));
+                atomicAdd(&acc[slot][3], (unsigned long long)cvt_fix(e, MOM_FE));
             } else {
                 full = 1;
             }
@@ -234,15 +232,8 @@ __global__ __launch_bounds__(256) void k_moments(
         const int l = keys[j];
         if (l < 0) continue;
         const float2 s = sites[l];
-        const double wt = ldexp((double)(long long)acc[j][0], -C::FW);
-        const double bx = ldexp((double)(long long)acc[j][1], -C::FB);
-        const double by = ldexp((double)(long long)acc[j][2], -C::FB);
-        const double et = ldexp((double)(long long)acc[j][3], -C::FE);
-        // tile-relative -> site-relative
-        const double ox = (double)cx0 - (double)s.x, oy = (double)cy0 - (double)s.y;
-        const double dx = bx + wt * ox;
-        const double dy = by + wt * oy;
-        const double ee = et + 2.0 * (ox * bx + oy * by) + wt * (ox * ox + oy * oy);
+        const long long wt = (long long)acc[j][0], dx = (long long)acc[j][1];
+        const long long dy = (long long)acc[j][2], ee = (long long)acc[j][3];
         const int hx = min(max((int)floorf(s.x / (float)T), 0), tilesX - 1);
         const int hy = min(max((int)floorf(s.y / (float)T), 0), tilesY - 1);
         const int rx = tX - hx, ry = tY - hy;
@@ -272,7 +263,7 @@ __global__ __launch_bounds__(256) void k_moments(
 // free when no tile overflowed.
 __global__ __launch_bounds__(256) void k_overflow(const OvfRec *__restrict__ ovf, int ovf_cap,
                                                   const int *__restrict__ ovf_count, int ntiles,
-                                                  double *__restrict__ ovf_acc,
+                                                  Acc128 *__restrict__ ovf_acc,
                                                   const int *__restrict__ dflags)
 {
     if (dflags[FLAG_OVF] == 0) return;
@@ -281,11 +272,19 @@ __global__ __launch_bounds__(256) void k_overflow(const OvfRec *__restrict__ ovf
         // records of one tile carry distinct labels: no two threads collide
         for (int k = threadIdx.x; k < n; k += blockDim.x) {
             const OvfRec o = ovf[(long)t * ovf_cap + k];
-            double *a = ovf_acc + 4L * o.label;
-            a[0] += o.w; a[1] += o.dx; a[2] += o.dy; a[3] += o.e;
+            Acc128 &a = ovf_acc[o.label];
+            a.v[0] += o.w; a.v[1] += o.dx; a.v[2] += o.dy; a.v[3] += o.e;
         }
         __syncthreads();
     }
+}
+
+// exact 128-bit fixed point -> double (value * 2^-f)
+__device__ __forceinline__ double cvt_i128_to_double(__int128 v, int f)
+{
+    const long long hi = (long long)(v >> 64);
+    const unsigned long long lo = (unsigned long long)v;
+    return ldexp((double)hi, 64 - f) + ldexp((double)lo, -f);
 }
 
 // ---------------------------------------------------------------------------
@@ -296,29 +295,33 @@ enum { UPD_MOVE = 1, UPD_SEED = 2 };
 
 __global__ __launch_bounds__(256) void k_update(
     float2 *__restrict__ sites, const uint8_t *__restrict__ fixed, int S,
-    const Slot *__restrict__ slots, double *__restrict__ ovf_acc,
+    const Slot *__restrict__ slots, Acc128 *__restrict__ ovf_acc,
     const int *__restrict__ dflags, const int *__restrict__ dstate, float omega, int mode,
     double rho_unscale, double *__restrict__ mom_out, double *__restrict__ blockpart,
-    int32_t *__restrict__ seed, int W, int H)
+    uint32_t *__restrict__ seed, int W, int H)
 {
     __shared__ double red_e[256], red_g[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0.0, g = 0.0;
     if (i < S) {
         const int epoch = dstate[ST_EPOCH];
-        double w = 0.0, dx = 0.0, dy = 0.0, ee = 0.0;
+        __int128 sw = 0, sx = 0, sy = 0, se = 0;
         const Slot *sl = slots + (long)i * 9;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             if (sl[k].epoch == epoch) {
-                w += sl[k].w; dx += sl[k].dx; dy += sl[k].dy; ee += sl[k].e;
+                sw += sl[k].w; sx += sl[k].dx; sy += sl[k].dy; se += sl[k].e;
             }
         }
         if (dflags[FLAG_OVF]) {
-            double *a = ovf_acc + 4L * i;
-            w += a[0]; dx += a[1]; dy += a[2]; ee += a[3];
-            a[0] = a[1] = a[2] = a[3] = 0.0;
+            Acc128 &a = ovf_acc[i];
+            sw += a.v[0]; sx += a.v[1]; sy += a.v[2]; se += a.v[3];
+            a.v[0] = a.v[1] = a.v[2] = a.v[3] = 0;
         }
+        const double w = cvt_i128_to_double(sw, MOM_FW);
+        const double dx = cvt_i128_to_double(sx, MOM_FX);
+        const double dy = cvt_i128_to_double(sy, MOM_FX);
+        const double ee = cvt_i128_to_double(se, MOM_FE);
         e = ee;
         g = dx * dx + dy * dy;
         if (mom_out) {
@@ -335,7 +338,7 @@ __global__ __launch_bounds__(256) void k_update(
         }
         if (mode & UPD_SEED) {
             const int x = cvt_pixel_of(s.x, W), y = cvt_pixel_of(s.y, H);
-            atomicMin(&seed[(long)y * W + x], i);
+            atomicMin(&seed[(long)y * W + x], 8u * (uint32_t)i);
         }
     }
     red_e[threadIdx.x] = e;
@@ -396,7 +399,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ blo
 // ---------------------------------------------------------------------------
 // float64 energy over pixels (the fp64 convergence check)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_energy64(const int32_t *__restrict__ lab,
+__global__ __launch_bounds__(256) void k_energy64(const uint32_t *__restrict__ lab,
                                                   const float *__restrict__ rho,
                                                   const float2 *__restrict__ sites, int S,
                                                   int W, int H, double *__restrict__ blockpart)
@@ -405,7 +408,7 @@ __global__ __launch_bounds__(256) void k_energy64(const int32_t *__restrict__ la
     const long n = (long)W * H;
     double e = 0.0;
     for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256) {
-        const int l = lab[p];
+        const int l = (int)(lab[p] >> 3);
         const float r = rho[p];
         if ((unsigned)l < (unsigned)S && r != 0.0f) {
             const int y = (int)(p / W), x = (int)(p - (long)y * W);

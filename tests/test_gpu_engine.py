@@ -49,6 +49,9 @@ def check_moments(m, ref, W_tol=1e-6):
     (320, 200, 333, 4),       # rectangular, not a multiple of 64 in y
     (100, 70, 1, 5),          # one site
     (64, 64, 64, 6),
+    (98, 77, 150, 7),         # width not a multiple of 4: one pixel per lane
+    (1024, 1024, 30000, 8),   # site table too large for LDS: global gathers
+    (12, 9, 5, 9),            # tiny grid
 ])
 def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
     xy = rand_sites(seed, W, H, S)
@@ -97,6 +100,18 @@ def test_moments_each_tile(gpu, orc, tile):
         m = e.moments()
     assert np.array_equal(lab, orc.jfa(w.n, w.n, w.sites))
     check_moments(m, orc.moments(w.density, lab, w.sites))
+
+
+def test_moments_independent_of_tile(gpu):
+    # tiles >= 64 see the same 64-pixel runs and sum them exactly: identical bits
+    from cvtamd import workloads as wl
+    w = wl.config2(17, n=512, num_sites=800)
+    res = []
+    for tile in (64, 128, 256):
+        with make_engine(w.n, w.sites, w.density, tile=tile) as e:
+            e.voronoi()
+            res.append(e.moments())
+    assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
 
 
 def test_moments_overflow_path(gpu, orc):
@@ -211,6 +226,20 @@ def test_energy_fp64(gpu, orc):
     assert E == pytest.approx(Er, rel=1e-12)
     assert G == pytest.approx(Gr, rel=1e-6)
     assert h[0, 0] == pytest.approx(Er, rel=1e-6)
+
+
+def test_site_validation(gpu):
+    import cvtamd
+    n = 32
+    with cvtamd.Engine(n, n, 4) as e:
+        e.set_density(np.ones((n, n), np.float32))
+        for bad in ([[1.0, np.nan]], [[3 * n, 1.0]], [[1.0, -n - 1.0]]):
+            with pytest.raises(cvtamd.CvtError):
+                e.set_sites(np.array(bad, np.float32))
+            assert e.num_sites == 0
+        e.set_sites(np.array([[-n + 0.5, 2 * n - 1.0]], np.float32))   # edge of the allowed box
+        e.voronoi()
+        assert (e.labels() == 0).all()
 
 
 def test_density_validation(gpu):
