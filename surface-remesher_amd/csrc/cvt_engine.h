@@ -38,10 +38,9 @@ struct Ctx {
     int tile = 0, tilesX = 0, tilesY = 0;
     bool tile_auto = true;
     Slot *slots = nullptr;         // [maxS][9]
-    OvfRec *ovf = nullptr;         // [tiles][ovf_cap]
+    OvfRec *ovf = nullptr;         // [ovf_cap] record pool
     int ovf_cap = 0;
-    int *ovf_count = nullptr;      // [tiles]
-    Acc128 *ovf_acc = nullptr;     // [maxS]
+    int *ovf_head = nullptr;       // [maxS] first record of each site, -1 none
     double *blockpart = nullptr;
     double *hist = nullptr;        // [hist_cap][2]
     int hist_cap = 0;

@@ -65,27 +65,25 @@ static int launch_moments(Ctx *c, const uint32_t *lab)
     case 32:
         hipLaunchKernelGGL(k_moments<32>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
                            c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
-                           c->ovf_count, c->dflags, c->dstate);
+                           c->ovf_head, c->dflags, c->dstate);
         break;
     case 64:
         hipLaunchKernelGGL(k_moments<64>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
                            c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
-                           c->ovf_count, c->dflags, c->dstate);
+                           c->ovf_head, c->dflags, c->dstate);
         break;
     case 128:
         hipLaunchKernelGGL(k_moments<128>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
                            c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
-                           c->ovf_count, c->dflags, c->dstate);
+                           c->ovf_head, c->dflags, c->dstate);
         break;
     default:
         hipLaunchKernelGGL(k_moments<256>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
                            c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
-                           c->ovf_count, c->dflags, c->dstate);
+                           c->ovf_head, c->dflags, c->dstate);
         break;
     }
     if (c->prof_mid) hipEventRecord(c->prof_mid, c->stream);
-    hipLaunchKernelGGL(k_overflow, dim3(1), dim3(256), 0, c->stream, c->ovf, c->ovf_cap,
-                       c->ovf_count, c->tilesX * c->tilesY, c->ovf_acc, c->dflags);
     return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "moment launch failed");
 }
 
@@ -112,16 +110,13 @@ static int set_tile_internal(Ctx *c, int tile)
     c->tilesX = (c->W + tile - 1) / tile;
     c->tilesY = (c->H + tile - 1) / tile;
     const long ntiles = (long)c->tilesX * c->tilesY;
-    // a tile holds at most CVT_HASH_CAP labels (more raises CVT_ERR_CAPACITY)
-    c->ovf_cap = CVT_HASH_CAP;
+    // a tile holds at most CVT_HASH_CAP labels, so at most that many records
+    // per tile can overflow (more raises CVT_ERR_CAPACITY)
+    c->ovf_cap = (int)std::min<long>(ntiles * (long)CVT_HASH_CAP, 1L << 26);
     if (c->ovf) hipFree(c->ovf);
-    if (c->ovf_count) hipFree(c->ovf_count);
     c->ovf = nullptr;
-    c->ovf_count = nullptr;
-    if (hipMalloc(&c->ovf, sizeof(OvfRec) * ntiles * c->ovf_cap) != hipSuccess ||
-        hipMalloc(&c->ovf_count, sizeof(int) * ntiles) != hipSuccess)
+    if (hipMalloc(&c->ovf, sizeof(OvfRec) * (size_t)c->ovf_cap) != hipSuccess)
         return fail(CVT_ERR_NOMEM, "overflow buffers (%ld tiles)", ntiles);
-    hipMemsetAsync(c->ovf_count, 0, sizeof(int) * ntiles, c->stream);
     c->graph_valid = false;
     return CVT_OK;
 }
@@ -178,9 +173,9 @@ int Ctx::enqueue_jfa()
 int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg)
 {
     const int nb = std::max(1, (S + 255) / 256);
-    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, ovf_acc,
-                       dflags, dstate, omega, mode, ldexp(1.0, rho_exp), mom_out, blockpart, seed,
-                       W, H);
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, ovf,
+                       ovf_head, dflags, dstate, omega, mode, ldexp(1.0, rho_exp), mom_out, blockpart,
+                       seed, W, H);
     const double a = (double)W * H;
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, blockpart, nb, dstate, dflags,
                        hist, hist_cap, record, ldexp(1.0, rho_exp) / a, 2.0 * ldexp(1.0, rho_exp) / a,
@@ -246,7 +241,7 @@ Ctx::~Ctx()
 {
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
-    void *bufs[] = {rho, seed, lab[0], lab[1], sites, fixed, slots, ovf, ovf_count, ovf_acc,
+    void *bufs[] = {rho, seed, lab[0], lab[1], sites, fixed, slots, ovf, ovf_head,
                     blockpart, hist, dstate, dflags, scratch, eg};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -301,7 +296,7 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
               hipMalloc(&c->sites, sizeof(float2) * (c->capS + 2)) == hipSuccess &&
               hipMalloc(&c->fixed, ms) == hipSuccess &&
               hipMalloc(&c->slots, sizeof(Slot) * 9 * ms) == hipSuccess &&
-              hipMalloc(&c->ovf_acc, sizeof(Acc128) * ms) == hipSuccess &&
+              hipMalloc(&c->ovf_head, sizeof(int) * ms) == hipSuccess &&
               hipMalloc(&c->blockpart, sizeof(double) * 2 * std::max(nb, 4096)) == hipSuccess &&
               hipMalloc(&c->hist, sizeof(double) * 2 * c->hist_cap) == hipSuccess &&
               hipMalloc(&c->dstate, sizeof(int) * NUM_STATE) == hipSuccess &&
@@ -319,7 +314,7 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     hipMemsetAsync(c->rho, 0, sizeof(float) * n, s);
     hipMemsetAsync(c->fixed, 0, ms, s);
     hipMemsetAsync(c->slots, 0, sizeof(Slot) * 9 * ms, s);
-    hipMemsetAsync(c->ovf_acc, 0, sizeof(Acc128) * ms, s);
+    hipMemsetAsync(c->ovf_head, 0xff, sizeof(int) * ms, s);   // -1: empty chains
     hipMemsetAsync(c->dflags, 0, sizeof(int) * NUM_FLAGS, s);
     int st[NUM_STATE] = {1, 0, 0, 0};
     hipMemcpyAsync(c->dstate, st, sizeof st, hipMemcpyHostToDevice, s);

@@ -15,7 +15,8 @@
 //   slots    Slot [S][9]      per-site partial moments, one per tile of the 3x3
 //                             tile window around the site's home tile (unique
 //                             writer per slot, tagged with the epoch)
-//   ovf      OvfRec [tiles][cap]  partial moments of tiles outside that window
+//   ovf      OvfRec [cap]     partial moments of tiles outside that window,
+//                             chained per site from ovf_head[S]
 //
 // The moment reduction is deterministic and uses no global atomics: per
 // 64-pixel row segment a wavefront-level segmented scan sums each run of equal
@@ -48,18 +49,14 @@ struct __align__(8) Slot {
 
 struct __align__(8) OvfRec {
     long long w, dx, dy, e;
-    int label;
+    int next;   // next record of the same site, -1 ends the chain
     int pad;
 };
 
-struct __align__(16) Acc128 {   // one 128-bit accumulator per moment
-    __int128 v[4];
-};
-
 // device flags
-enum { FLAG_OVF = 0, FLAG_FULL = 1, FLAG_BAD = 2, NUM_FLAGS = 4 };
+enum { FLAG_FULL = 1, FLAG_BAD = 2, NUM_FLAGS = 4 };
 // device state words
-enum { ST_EPOCH = 0, ST_HIST = 1, NUM_STATE = 4 };
+enum { ST_EPOCH = 0, ST_HIST = 1, ST_OVFN = 2, NUM_STATE = 4 };
 
 // ---------------------------------------------------------------------------
 // distance order (must round exactly like the oracle's orc_dist / orc_better)
@@ -129,7 +126,6 @@ __global__ void k_export_labels(const uint32_t *__restrict__ lab, int32_t *__res
 // texture, source/gcvt.cu:562-732, :788-802)
 // ---------------------------------------------------------------------------
 template <int T> struct TileCfg {
-    static constexpr int LOG = (T == 32) ? 5 : (T == 64) ? 6 : (T == 128) ? 7 : 8;
     static constexpr int SEG = T < 64 ? T : 64;       // pixels per row segment
     static constexpr int RPW = 64 / SEG;              // rows per wave step
     static constexpr int SPR = T / SEG;               // segments per row
@@ -138,9 +134,45 @@ template <int T> struct TileCfg {
 
 #define CVT_HASH_CAP 1024
 
-__device__ __forceinline__ long long cvt_fix(float v, int f)
+__device__ __forceinline__ long long cvt_fix(double v, int f)
 {
-    return __double2ll_rn(ldexp((double)v, f));
+    return __double2ll_rn(ldexp(v, f));
+}
+
+// --- cross-lane moves (DPP, no LDS traffic) --------------------------------
+// value of lane - 1 over the whole wave (lane 0 gets fill)
+__device__ __forceinline__ int dpp_wave_shr1(int v, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xf, 0xf, false);
+}
+// value of lane + 1 over the whole wave (lane 63 gets fill)
+__device__ __forceinline__ int dpp_wave_shl1(int v, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xf, 0xf, false);
+}
+// value of lane - K inside the 16-lane row (0 where that leaves the row)
+template <int K> __device__ __forceinline__ int dpp_row_shr(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x110 + K, 0xf, 0xf, false);
+}
+// lane 15 of rows 0 / 2 to every lane of rows 1 / 3 (0 elsewhere)
+__device__ __forceinline__ int dpp_bcast15(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+}
+// lane 31 to every lane of rows 2 and 3 (0 elsewhere)
+__device__ __forceinline__ int dpp_bcast31(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+}
+__device__ __forceinline__ float fbits(int v) { return __int_as_float(v); }
+__device__ __forceinline__ int ibits(float v) { return __float_as_int(v); }
+
+// One step of a segmented inclusive scan: add the value from lane `src` when
+// that lane lies in this lane's run (src >= start).  t is the moved value.
+__device__ __forceinline__ float seg_add(float v, int t, int src, int start)
+{
+    return (src >= start) ? v + fbits(t) : v;
 }
 
 template <int T>
@@ -148,12 +180,12 @@ __global__ __launch_bounds__(256) void k_moments(
     const uint32_t *__restrict__ lab, const float *__restrict__ rho,
     const float2 *__restrict__ sites, int S, int W, int H, float rho_scale, int tilesX,
     int tilesY, Slot *__restrict__ slots, OvfRec *__restrict__ ovf, int ovf_cap,
-    int *__restrict__ ovf_count, int *__restrict__ dflags, const int *__restrict__ dstate)
+    int *__restrict__ ovf_head, int *__restrict__ dflags, int *__restrict__ dstate)
 {
     using C = TileCfg<T>;
     __shared__ int keys[CVT_HASH_CAP];
     __shared__ unsigned long long acc[CVT_HASH_CAP][4];
-    __shared__ int n_over, full;
+    __shared__ int full;
 
     const int tile = blockIdx.x;
     const int tX = tile % tilesX, tY = tile / tilesX;
@@ -162,52 +194,68 @@ __global__ __launch_bounds__(256) void k_moments(
         keys[j] = -1;
         acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0ull;
     }
-    if (threadIdx.x == 0) { n_over = 0; full = 0; }
+    if (threadIdx.x == 0) full = 0;
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int seg_lane = lane % C::SEG;
+    const int row16 = lane & 15;
 
     for (int g = wave; g < C::GROUPS; g += 4) {
         const int r = (g / C::SPR) * C::RPW + lane / C::SEG;
-        const int c = (g % C::SPR) * C::SEG + seg_lane;
-        const int x = tx0 + c, y = ty0 + r;
+        const int c0 = (g % C::SPR) * C::SEG;
+        const int x = tx0 + c0 + seg_lane, y = ty0 + r;
         int key = -1;
-        float a = 0.f, b = 0.f, cc = 0.f, e = 0.f;
+        float rs = 0.0f;
         if (x < W && y < H) {
             const long p = (long)y * W + x;
             const int l = (int)(lab[p] >> 3);
             const float rv = rho[p];
             if ((unsigned)l < (unsigned)S && rv != 0.0f) {
                 key = l;
-                const float2 st = sites[l];
-                const float rs = rv * rho_scale;
-                const float dx = ((float)x + 0.5f) - st.x;
-                const float dy = ((float)y + 0.5f) - st.y;
-                a = rs;
-                b = rs * dx;
-                cc = rs * dy;
-                e = rs * __builtin_fmaf(dx, dx, dy * dy);
+                rs = rv * rho_scale;
             }
         }
-        // segmented inclusive scan over runs of equal key within the segment
-        const int kprev = __shfl_up(key, 1);
+        // runs of equal key inside the segment; start = first lane of the run
+        const int kprev = dpp_wave_shr1(key, -2);
         const bool head = (seg_lane == 0) || (kprev != key);
         int start = head ? lane : 0;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int o = __shfl_up(start, off);
-            if (lane >= off) start = max(start, o);
+        start = max(start, dpp_row_shr<1>(start));
+        start = max(start, dpp_row_shr<2>(start));
+        start = max(start, dpp_row_shr<4>(start));
+        start = max(start, dpp_row_shr<8>(start));
+        start = max(start, dpp_bcast15(start));
+        start = max(start, dpp_bcast31(start));
+        // per pixel, relative to the run's first pixel: u = lane - start
+        const float u = (float)(lane - start);
+        float a = rs, b = rs * u, cc = rs * (u * u);
+        // segmented inclusive scan of (a, b, cc)
+#define CVT_SCAN_STEP(MOVE, SRC)                                                   \
+        {                                                                          \
+            const int ta = MOVE(ibits(a)), tb = MOVE(ibits(b)), tc = MOVE(ibits(cc)); \
+            a = seg_add(a, ta, SRC, start);                                        \
+            b = seg_add(b, tb, SRC, start);                                        \
+            cc = seg_add(cc, tc, SRC, start);                                      \
         }
-#pragma unroll
-        for (int off = 1; off < C::SEG; off <<= 1) {
-            const float ua = __shfl_up(a, off), ub = __shfl_up(b, off);
-            const float uc = __shfl_up(cc, off), ue = __shfl_up(e, off);
-            if (lane - off >= start) { a += ua; b += ub; cc += uc; e += ue; }
-        }
-        const int knext = __shfl_down(key, 1);
+        CVT_SCAN_STEP(dpp_row_shr<1>, row16 >= 1 ? lane - 1 : -1)
+        CVT_SCAN_STEP(dpp_row_shr<2>, row16 >= 2 ? lane - 2 : -1)
+        CVT_SCAN_STEP(dpp_row_shr<4>, row16 >= 4 ? lane - 4 : -1)
+        CVT_SCAN_STEP(dpp_row_shr<8>, row16 >= 8 ? lane - 8 : -1)
+        CVT_SCAN_STEP(dpp_bcast15, ((lane >> 4) & 1) ? (lane & ~15) - 1 : -1)
+        CVT_SCAN_STEP(dpp_bcast31, (lane >= 32) ? 31 : -1)
+#undef CVT_SCAN_STEP
+        const int knext = dpp_wave_shl1(key, -2);
         const bool tail = (seg_lane == C::SEG - 1) || (knext != key);
         if (tail && key >= 0) {
+            // run -> site-relative sums in float64 (one site read per run)
+            const float2 st = sites[key];
+            const double A = a, B = b, Cc = cc;
+            const double ox = ((double)(x - (lane - start)) + 0.5) - (double)st.x;
+            const double oy = ((double)y + 0.5) - (double)st.y;
+            const double W_ = A;
+            const double DX = B + A * ox;
+            const double DY = A * oy;
+            const double E = Cc + 2.0 * ox * B + A * (ox * ox + oy * oy);
             const unsigned hsh = ((unsigned)key * 2654435761u) >> (32 - 10);
             int slot = -1;
             for (int probe = 0; probe < CVT_HASH_CAP; ++probe) {
@@ -216,10 +264,10 @@ __global__ __launch_bounds__(256) void k_moments(
                 if (old == -1 || old == key) { slot = h; break; }
             }
             if (slot >= 0) {
-                atomicAdd(&acc[slot][0], (unsigned long long)cvt_fix(a, MOM_FW));
-                atomicAdd(&acc[slot][1], (unsigned long long)cvt_fix(b, MOM_FX));
-                atomicAdd(&acc[slot][2], (unsigned long long)cvt_fix(cc, MOM_FX));
-                atomicAdd(&acc[slot][3], (unsigned long long)cvt_fix(e, MOM_FE));
+                atomicAdd(&acc[slot][0], (unsigned long long)cvt_fix(W_, MOM_FW));
+                atomicAdd(&acc[slot][1], (unsigned long long)cvt_fix(DX, MOM_FX));
+                atomicAdd(&acc[slot][2], (unsigned long long)cvt_fix(DY, MOM_FX));
+                atomicAdd(&acc[slot][3], (unsigned long long)cvt_fix(E, MOM_FE));
             } else {
                 full = 1;
             }
@@ -242,41 +290,22 @@ __global__ __launch_bounds__(256) void k_moments(
             sl.w = wt; sl.dx = dx; sl.dy = dy; sl.e = ee; sl.epoch = epoch; sl.pad = 0;
             slots[(long)l * 9 + (ry + 1) * 3 + (rx + 1)] = sl;
         } else {
-            const int k = atomicAdd(&n_over, 1);
+            // a cell reaching beyond its home tile's 3x3 window: chain a record
+            // to the site (integer sums: the chain order cannot change them)
+            const int k = atomicAdd(&dstate[ST_OVFN], 1);
             if (k < ovf_cap) {
                 OvfRec o;
-                o.w = wt; o.dx = dx; o.dy = dy; o.e = ee; o.label = l; o.pad = 0;
-                ovf[(long)tile * ovf_cap + k] = o;
+                o.w = wt; o.dx = dx; o.dy = dy; o.e = ee;
+                o.next = atomicExch(&ovf_head[l], k);
+                o.pad = 0;
+                ovf[k] = o;
+            } else {
+                full = 1;
             }
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        ovf_count[tile] = n_over;
-        if (n_over) dflags[FLAG_OVF] = 1;
-        if (full || n_over > ovf_cap) dflags[FLAG_FULL] = 1;
-    }
-}
-
-// Overflow records (cells reaching beyond the 3x3 window of their home tile)
-// are added to ovf_acc in tile order by a single workgroup: deterministic, and
-// free when no tile overflowed.
-__global__ __launch_bounds__(256) void k_overflow(const OvfRec *__restrict__ ovf, int ovf_cap,
-                                                  const int *__restrict__ ovf_count, int ntiles,
-                                                  Acc128 *__restrict__ ovf_acc,
-                                                  const int *__restrict__ dflags)
-{
-    if (dflags[FLAG_OVF] == 0) return;
-    for (int t = 0; t < ntiles; ++t) {
-        const int n = min(ovf_count[t], ovf_cap);
-        // records of one tile carry distinct labels: no two threads collide
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            const OvfRec o = ovf[(long)t * ovf_cap + k];
-            Acc128 &a = ovf_acc[o.label];
-            a.v[0] += o.w; a.v[1] += o.dx; a.v[2] += o.dy; a.v[3] += o.e;
-        }
-        __syncthreads();
-    }
+    if (threadIdx.x == 0 && full) dflags[FLAG_FULL] = 1;
 }
 
 // exact 128-bit fixed point -> double (value * 2^-f)
@@ -295,8 +324,8 @@ enum { UPD_MOVE = 1, UPD_SEED = 2 };
 
 __global__ __launch_bounds__(256) void k_update(
     float2 *__restrict__ sites, const uint8_t *__restrict__ fixed, int S,
-    const Slot *__restrict__ slots, Acc128 *__restrict__ ovf_acc,
-    const int *__restrict__ dflags, const int *__restrict__ dstate, float omega, int mode,
+    const Slot *__restrict__ slots, const OvfRec *__restrict__ ovf, int *__restrict__ ovf_head,
+    const int *__restrict__ dflags, int *__restrict__ dstate, float omega, int mode,
     double rho_unscale, double *__restrict__ mom_out, double *__restrict__ blockpart,
     uint32_t *__restrict__ seed, int W, int H)
 {
@@ -313,11 +342,10 @@ __global__ __launch_bounds__(256) void k_update(
                 sw += sl[k].w; sx += sl[k].dx; sy += sl[k].dy; se += sl[k].e;
             }
         }
-        if (dflags[FLAG_OVF]) {
-            Acc128 &a = ovf_acc[i];
-            sw += a.v[0]; sx += a.v[1]; sy += a.v[2]; se += a.v[3];
-            a.v[0] = a.v[1] = a.v[2] = a.v[3] = 0;
+        for (int k = ovf_head[i]; k >= 0; k = ovf[k].next) {
+            sw += ovf[k].w; sx += ovf[k].dx; sy += ovf[k].dy; se += ovf[k].e;
         }
+        ovf_head[i] = -1;
         const double w = cvt_i128_to_double(sw, MOM_FW);
         const double dx = cvt_i128_to_double(sx, MOM_FX);
         const double dy = cvt_i128_to_double(sy, MOM_FX);
@@ -355,6 +383,8 @@ __global__ __launch_bounds__(256) void k_update(
         blockpart[2 * blockIdx.x] = red_e[0];
         blockpart[2 * blockIdx.x + 1] = red_g[0];
     }
+    // every chain was consumed above: the record pool restarts empty
+    if (blockIdx.x == 0 && threadIdx.x == 0) dstate[ST_OVFN] = 0;
 }
 
 // Sums the block partials in a fixed order, appends (E, |grad E|) to the
@@ -392,7 +422,6 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ blo
         }
         if (out_eg) { out_eg[0] = E; out_eg[1] = G; }
         dstate[ST_EPOCH] = dstate[ST_EPOCH] + 1;
-        if (dflags[FLAG_OVF]) dflags[FLAG_OVF] = 0;
     }
 }
 
