@@ -116,6 +116,67 @@ int cvt_synchronize(cvt_ctx *ctx);
  * source/main.cpp:181-242.) */
 int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms);
 
+/* Reference-compatible integer-site CVT (the semantics of gCVT,
+ * source/gcvt.cu:1087-1156): voronoi is the size x size map of interleaved
+ * (x, y) int16 labels (MARKER = -32768), read for the initial sites (pixels
+ * labelled with their own coordinates) and overwritten with the final label
+ * map; density is size x size fp32; mask (optional, one byte per pixel)
+ * marks fixed sites.  depth > 1 adds coarse levels down to 256 on a 2x2-mean
+ * density pyramid.  iterations / num_sites (optional) receive the iteration
+ * count and the final site count.  Differences from the reference: JFA(+1)
+ * instead of exact PBA+ for the Voronoi map, float64 moments and energy, any
+ * size in [2, 8192] (the reference needs a power of two >= 256). */
+int cvt_compat_gcvt(int16_t *voronoi, const float *density, const uint8_t *mask, int size,
+                    int depth, int max_iter, int *iterations, int *num_sites);
+
+/* Density rasterizer (the semantics of discretization_d,
+ * source/discretization.cu:57-120): density[y*n + x] for the sample point
+ * (x*scale, y*scale) is the float64 barycentric interpolation of weight over
+ * the first triangle (index order) containing it, 0 outside every triangle.
+ * points holds num_point (x, y) pairs already shifted to the origin,
+ * triangle 3 vertex indices per triangle. */
+int cvt_rasterize(const double *points, const double *weight, int num_point, const int *triangle,
+                  int num_tri, float *density, double scale, int n);
+
+/* CDT input points (the site scan of delaunayInput, source/delaunay.h:48-58):
+ * pixels (x, y) whose label is their own coordinates and whose mask byte is
+ * 0 (mask may be null), in the reference's order (x outer, y inner), as
+ * (x*scale + l, y*scale + b) float64 pairs.  *count receives the number
+ * found; at most capacity pairs are written (CVT_ERR_CAPACITY if more). */
+int cvt_extract_points(const int16_t *voronoi, const uint8_t *mask, int n, double scale, double l,
+                       double b, double *out, long capacity, long *count);
+
+/* ---------------------------------------------------------------------------
+ * Batched CVT: many independent patches of different sizes advanced
+ * together, each launch covering every patch (no reference counterpart: the
+ * reference runs one mesh per gCVT call).  A patch's results equal a
+ * single-patch context's on the same inputs (labels bit for bit; moments bit
+ * for bit at the same moment tile).  Patch widths must be multiples of 4 in
+ * [8, 8192].
+ * ------------------------------------------------------------------------- */
+typedef struct cvt_batch cvt_batch;
+
+int cvt_batch_create(cvt_batch **batch, int num_patches, const int *widths, const int *heights,
+                     const int *max_sites);
+int cvt_batch_destroy(cvt_batch *batch);
+int cvt_batch_set_stream(cvt_batch *batch, void *hip_stream);
+int cvt_batch_set_density(cvt_batch *batch, int patch, const float *density);
+int cvt_batch_set_sites(cvt_batch *batch, int patch, const float *xy, const uint8_t *fixed,
+                        int num_sites);
+int cvt_batch_get_sites(cvt_batch *batch, int patch, float *xy);
+/* iterations Lloyd steps of every patch (one captured graph per step). */
+int cvt_batch_lloyd(cvt_batch *batch, int iterations, float omega);
+/* Label maps of every patch for their current sites. */
+int cvt_batch_voronoi(cvt_batch *batch);
+int cvt_batch_get_labels(cvt_batch *batch, int patch, int32_t *labels);
+/* Recorded (energy, gradient norm) pairs of a patch: copies min(n, length)
+ * pairs and returns that count (n = 0 returns the length). */
+int cvt_batch_history(cvt_batch *batch, int patch, double *energy_gradnorm, int n);
+/* Moment tile of a patch (64, 128 or 256): results do not depend on it. */
+int cvt_batch_set_tile(cvt_batch *batch, int patch, int tile);
+int cvt_batch_get_tile(cvt_batch *batch, int patch);
+int cvt_batch_synchronize(cvt_batch *batch);
+
 /* Tuning: moment tile side (32, 64, 128 or 256; 0 = choose from the site
  * density).  Results do not depend on it. */
 int cvt_set_tile(cvt_ctx *ctx, int tile);

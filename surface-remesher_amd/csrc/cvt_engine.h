@@ -4,7 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/cvt.h"
-#include "cvt_kernels.h"
+#include "cvt_types.h"
 #include "jfa.h"
 
 namespace cvt {
@@ -61,7 +61,9 @@ struct Ctx {
     ~Ctx();
     int enqueue_seed();
     int enqueue_jfa();
-    int enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg);
+    int enqueue_moments();
+    int enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg,
+                       const float *domega = nullptr, CompatArgs ca = CompatArgs{});
     int enqueue_iteration(float omega);
     int build_graph(float omega);
     int lloyd(int iterations, float omega);

@@ -8,6 +8,7 @@
 // iteration and never synchronises inside the loop (the reference copies the
 // energy back every 10th iteration, source/gcvt.cu:1080).
 #include "cvt_engine.h"
+#include "cvt_kernels.h"
 
 #include <math.h>
 #include <stdarg.h>
@@ -19,6 +20,7 @@
 
 namespace cvt {
 
+static int launch_moments_on(Ctx *c, const uint32_t *lab);
 static thread_local char g_err[512];
 
 int fail(int code, const char *fmt, ...)
@@ -57,7 +59,9 @@ static int choose_tile(int W, int H, int S)
     return t;
 }
 
-static int launch_moments(Ctx *c, const uint32_t *lab)
+int Ctx::enqueue_moments() { return launch_moments_on(this, lab[final_buf]); }
+
+static int launch_moments_on(Ctx *c, const uint32_t *lab)
 {
     const float rs = (float)ldexp(1.0, -c->rho_exp);
     dim3 grid(c->tilesX * c->tilesY), block(256);
@@ -93,7 +97,7 @@ static int launch_moments_timed(Ctx *c, hipEvent_t after_tiles, hipEvent_t after
     hipEvent_t *saved = c->prof_events;
     c->prof_events = nullptr;
     c->prof_mid = after_tiles;
-    int rc = launch_moments(c, c->lab[c->final_buf]);
+    int rc = launch_moments_on(c, c->lab[c->final_buf]);
     c->prof_mid = nullptr;
     c->prof_events = saved;
     hipEventRecord(after_ovf, c->stream);
@@ -170,12 +174,13 @@ int Ctx::enqueue_jfa()
     return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "jfa launch failed");
 }
 
-int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg)
+int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg,
+                        const float *domega, CompatArgs ca)
 {
     const int nb = std::max(1, (S + 255) / 256);
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, ovf,
-                       ovf_head, dflags, dstate, omega, mode, ldexp(1.0, rho_exp), mom_out, blockpart,
-                       seed, W, H);
+                       ovf_head, dflags, dstate, omega, domega, mode, ldexp(1.0, rho_exp), mom_out,
+                       blockpart, seed, W, H, ca);
     const double a = (double)W * H;
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, blockpart, nb, dstate, dflags,
                        hist, hist_cap, record, ldexp(1.0, rho_exp) / a, 2.0 * ldexp(1.0, rho_exp) / a,
@@ -188,7 +193,7 @@ int Ctx::enqueue_iteration(float omega)
 {
     int rc;
     if ((rc = enqueue_jfa())) return rc;
-    if ((rc = launch_moments(this, lab[final_buf]))) return rc;
+    if ((rc = enqueue_moments())) return rc;
     return enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr);
 }
 
@@ -492,7 +497,7 @@ int cvt_moments(cvt_ctx *ctx, double *m)
     if (!m) return fail(CVT_ERR_ARG, "null output");
     double *dm = nullptr;
     HIPCHK(hipMalloc(&dm, sizeof(double) * 4 * std::max(c->S, 1)), "moment buffer");
-    int rc = launch_moments(c, c->lab[c->final_buf]);
+    int rc = launch_moments_on(c, c->lab[c->final_buf]);
     if (!rc) rc = c->enqueue_update(0, 1.0f, dm, 0, nullptr);
     if (!rc && c->S > 0 &&
         hipMemcpyAsync(m, dm, sizeof(double) * 4 * c->S, hipMemcpyDefault, c->stream) != hipSuccess)
@@ -553,7 +558,7 @@ int cvt_energy(cvt_ctx *ctx, double *energy, double *gradnorm)
     if (!c->have_density) return fail(CVT_ERR_STATE, "cvt_energy before cvt_set_density");
     int rc = cvt_voronoi(ctx);
     if (rc) return rc;
-    if ((rc = launch_moments(c, c->lab[c->final_buf]))) return rc;
+    if ((rc = launch_moments_on(c, c->lab[c->final_buf]))) return rc;
     if ((rc = c->enqueue_update(0, 1.0f, nullptr, 0, c->eg))) return rc;
     const long n = (long)c->W * c->H;
     const int nb = 1024;

@@ -28,35 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define CVT_NONE 0x7fffffff
-
-// Fixed-point moment sums of one (tile, site) pair: W = sum rho', DX = sum
-// rho' (px - sx), DY = sum rho' (py - sy), E = sum rho' |p - s|^2, scaled by
-// 2^MOM_FW, 2^MOM_FX, 2^MOM_FX, 2^MOM_FE.  rho' = rho * 2^-k <= 1, a tile has
-// at most 256^2 = 2^16 pixels and |p - s| < 2^15 (sites stay inside
-// [-W, 2W] x [-H, 2H], W, H <= 8192), so each tile sum fits in int64; the
-// per-site totals are summed in 128 bits.  Integer sums are exact, hence
-// independent of order and grouping.
-#define MOM_FW 46
-#define MOM_FX 31
-#define MOM_FE 15
-
-struct __align__(8) Slot {
-    long long w, dx, dy, e;
-    int epoch;
-    int pad;
-};
-
-struct __align__(8) OvfRec {
-    long long w, dx, dy, e;
-    int next;   // next record of the same site, -1 ends the chain
-    int pad;
-};
-
-// device flags
-enum { FLAG_FULL = 1, FLAG_BAD = 2, NUM_FLAGS = 4 };
-// device state words
-enum { ST_EPOCH = 0, ST_HIST = 1, ST_OVFN = 2, NUM_STATE = 4 };
+#include "cvt_types.h"
 
 // ---------------------------------------------------------------------------
 // distance order (must round exactly like the oracle's orc_dist / orc_better)
@@ -71,7 +43,7 @@ __device__ __forceinline__ float cvt_dist(float2 s, float px, float py)
 // ---------------------------------------------------------------------------
 // fills, seeding
 // ---------------------------------------------------------------------------
-__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, long n)
+static __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, long n)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long stride = (long)gridDim.x * blockDim.x;
@@ -87,7 +59,7 @@ __device__ __forceinline__ int cvt_pixel_of(float v, int n)
 
 // Sites must be finite and inside [-W, 2W] x [-H, 2H] (keeps |p - s| < 2^15
 // for the fixed-point moments); a violation raises FLAG_BAD.
-__global__ void k_site_check(const float2 *__restrict__ sites, int S, int W, int H,
+static __global__ void k_site_check(const float2 *__restrict__ sites, int S, int W, int H,
                              int *__restrict__ dflags)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -99,7 +71,7 @@ __global__ void k_site_check(const float2 *__restrict__ sites, int S, int W, int
 
 // Every site claims its pixel; the lowest index wins a shared pixel.  The
 // integer atomicMin makes the result independent of thread order.
-__global__ void k_seed(const float2 *__restrict__ sites, int S, uint32_t *__restrict__ seed,
+static __global__ void k_seed(const float2 *__restrict__ sites, int S, uint32_t *__restrict__ seed,
                        int W, int H)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -110,7 +82,7 @@ __global__ void k_seed(const float2 *__restrict__ sites, int S, uint32_t *__rest
 }
 
 // engine labels (byte offsets) -> int32 indices, CVT_NONE for none
-__global__ void k_export_labels(const uint32_t *__restrict__ lab, int32_t *__restrict__ out, long n,
+static __global__ void k_export_labels(const uint32_t *__restrict__ lab, int32_t *__restrict__ out, long n,
                                 uint32_t none)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -176,7 +148,7 @@ __device__ __forceinline__ float seg_add(float v, int t, int src, int start)
 }
 
 template <int T>
-__global__ __launch_bounds__(256) void k_moments(
+static __global__ __launch_bounds__(256) void k_moments(
     const uint32_t *__restrict__ lab, const float *__restrict__ rho,
     const float2 *__restrict__ sites, int S, int W, int H, float rho_scale, int tilesX,
     int tilesY, Slot *__restrict__ slots, OvfRec *__restrict__ ovf, int ovf_cap,
@@ -320,36 +292,55 @@ __device__ __forceinline__ double cvt_i128_to_double(__int128 v, int f)
 // site update (reference replacement for pbaCVDUpdateSites,
 // source/gcvt.cu:753-781, continuous form) + energy / gradient partials
 // ---------------------------------------------------------------------------
-enum { UPD_MOVE = 1, UPD_SEED = 2 };
+enum {
+    UPD_MOVE = 1,     // continuous update s += omega (c - s)
+    UPD_SEED = 2,     // scatter the (new) site into the seed map
+    UPD_COMPAT = 4,   // integer-site rule of the reference: targets into tgt / owner
+    UPD_DEV_S = 8,    // the live site count is dstate[ST_NSITES] (<= S)
+};
 
-__global__ __launch_bounds__(256) void k_update(
+// Moment totals of site i: the 9 window slots of this epoch plus its overflow
+// chain, summed exactly in 128 bits.  Consumes the chain.
+__device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__restrict__ slots,
+                                                const OvfRec *__restrict__ ovf,
+                                                int *__restrict__ ovf_head, double &w, double &dx,
+                                                double &dy, double &ee)
+{
+    __int128 sw = 0, sx = 0, sy = 0, se = 0;
+    const Slot *sl = slots + (long)i * 9;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        if (sl[k].epoch == epoch) {
+            sw += sl[k].w; sx += sl[k].dx; sy += sl[k].dy; se += sl[k].e;
+        }
+    }
+    for (int k = ovf_head[i]; k >= 0; k = ovf[k].next) {
+        sw += ovf[k].w; sx += ovf[k].dx; sy += ovf[k].dy; se += ovf[k].e;
+    }
+    ovf_head[i] = -1;
+    w = cvt_i128_to_double(sw, MOM_FW);
+    dx = cvt_i128_to_double(sx, MOM_FX);
+    dy = cvt_i128_to_double(sy, MOM_FX);
+    ee = cvt_i128_to_double(se, MOM_FE);
+}
+
+
+
+static __global__ __launch_bounds__(256) void k_update(
     float2 *__restrict__ sites, const uint8_t *__restrict__ fixed, int S,
     const Slot *__restrict__ slots, const OvfRec *__restrict__ ovf, int *__restrict__ ovf_head,
-    const int *__restrict__ dflags, int *__restrict__ dstate, float omega, int mode,
-    double rho_unscale, double *__restrict__ mom_out, double *__restrict__ blockpart,
-    uint32_t *__restrict__ seed, int W, int H)
+    const int *__restrict__ dflags, int *__restrict__ dstate, float omega_arg,
+    const float *__restrict__ domega, int mode, double rho_unscale, double *__restrict__ mom_out,
+    double *__restrict__ blockpart, uint32_t *__restrict__ seed, int W, int H, CompatArgs ca)
 {
     __shared__ double red_e[256], red_g[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int live = (mode & UPD_DEV_S) ? dstate[ST_NSITES] : S;
+    const float omega = domega ? *domega : omega_arg;
     double e = 0.0, g = 0.0;
-    if (i < S) {
-        const int epoch = dstate[ST_EPOCH];
-        __int128 sw = 0, sx = 0, sy = 0, se = 0;
-        const Slot *sl = slots + (long)i * 9;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            if (sl[k].epoch == epoch) {
-                sw += sl[k].w; sx += sl[k].dx; sy += sl[k].dy; se += sl[k].e;
-            }
-        }
-        for (int k = ovf_head[i]; k >= 0; k = ovf[k].next) {
-            sw += ovf[k].w; sx += ovf[k].dx; sy += ovf[k].dy; se += ovf[k].e;
-        }
-        ovf_head[i] = -1;
-        const double w = cvt_i128_to_double(sw, MOM_FW);
-        const double dx = cvt_i128_to_double(sx, MOM_FX);
-        const double dy = cvt_i128_to_double(sy, MOM_FX);
-        const double ee = cvt_i128_to_double(se, MOM_FE);
+    if (i < live) {
+        double w, dx, dy, ee;
+        cvt_site_totals(i, dstate[ST_EPOCH], slots, ovf, ovf_head, w, dx, dy, ee);
         e = ee;
         g = dx * dx + dy * dy;
         if (mom_out) {
@@ -363,6 +354,25 @@ __global__ __launch_bounds__(256) void k_update(
             s.x = (float)((double)s.x + (double)omega * (dx / w));
             s.y = (float)((double)s.y + (double)omega * (dy / w));
             sites[i] = s;
+        }
+        if (mode & UPD_COMPAT) {
+            // source/gcvt.cu:753-781: target trunc(s + omega (c - s) + 0.5)
+            // (toward zero) per axis, clamped; a site on a mask pixel, or a
+            // target on rho == 0, stays.  Sites sit at pixel centres.
+            const int sx = (int)floorf(s.x), sy = (int)floorf(s.y);
+            int nx = sx, ny = sy;
+            const bool fx = ca.mask && ca.mask[(long)sy * W + sx];
+            if (!fx && w > 0.0) {
+                const double vx = (double)sx + (double)omega * (dx / w) + 0.5;
+                const double vy = (double)sy + (double)omega * (dy / w) + 0.5;
+                int cx = (int)vx, cy = (int)vy;
+                cx = min(max(cx, 0), W - 1);
+                cy = min(max(cy, 0), H - 1);
+                if (ca.rho[(long)cy * W + cx] != 0.0f) { nx = cx; ny = cy; }
+            }
+            const int t = ny * W + nx;
+            ca.tgt[i] = t;
+            atomicMin(&ca.owner[t], (uint32_t)i);
         }
         if (mode & UPD_SEED) {
             const int x = cvt_pixel_of(s.x, W), y = cvt_pixel_of(s.y, H);
@@ -387,9 +397,65 @@ __global__ __launch_bounds__(256) void k_update(
     if (blockIdx.x == 0 && threadIdx.x == 0) dstate[ST_OVFN] = 0;
 }
 
+// Integer-site merge (one workgroup): sites whose target pixel they own
+// survive, in their old relative order; survivors become the new site table
+// (pixel centres) and their seeds; the owner map is cleared again.
+static __global__ __launch_bounds__(1024) void k_compat_compact(float2 *__restrict__ sites,
+                                                         const int *__restrict__ tgt,
+                                                         uint32_t *__restrict__ owner,
+                                                         uint32_t *__restrict__ seed, int W,
+                                                         int *__restrict__ dstate)
+{
+    __shared__ int part[1024];
+    const int S = dstate[ST_NSITES];
+    const int per = (S + 1023) / 1024;
+    const int i0 = min(S, (int)threadIdx.x * per), i1 = min(S, i0 + per);
+    int cnt = 0;
+    for (int i = i0; i < i1; ++i) cnt += (owner[tgt[i]] == (uint32_t)i);
+    part[threadIdx.x] = cnt;
+    __syncthreads();
+    // inclusive scan of the per-thread counts (Hillis-Steele)
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int pos = part[threadIdx.x] - cnt;
+    const int total = part[1023];
+    for (int i = i0; i < i1; ++i) {
+        const int t = tgt[i];
+        if (owner[t] == (uint32_t)i) {
+            const int x = t % W, y = t / W;
+            sites[pos] = make_float2((float)x + 0.5f, (float)y + 0.5f);
+            seed[t] = 8u * (uint32_t)pos;
+            ++pos;
+        }
+    }
+    __syncthreads();
+    for (int i = i0; i < i1; ++i) owner[tgt[i]] = CVT_NONE;
+    if (threadIdx.x == 0) dstate[ST_NSITES] = total;
+}
+
+// labels -> the reference's coordinate map: interleaved (x, y) int16 of the
+// pixel's site, MARKER (-32768) where no site reaches
+static __global__ void k_labels_to_coords(const uint32_t *__restrict__ lab,
+                                   const float2 *__restrict__ sites, uint32_t none, long n,
+                                   short2 *__restrict__ out)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const uint32_t l = lab[i];
+        if (l >= none) { out[i] = make_short2(-32768, -32768); continue; }
+        const float2 s = sites[l >> 3];
+        out[i] = make_short2((short)floorf(s.x), (short)floorf(s.y));
+    }
+}
+
 // Sums the block partials in a fixed order, appends (E, |grad E|) to the
 // history when record != 0, advances the epoch and clears the flags.
-__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ blockpart,
+static __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ blockpart,
                                                   int nblocks, int *__restrict__ dstate,
                                                   int *__restrict__ dflags,
                                                   double *__restrict__ hist, int hist_cap,
@@ -428,7 +494,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ blo
 // ---------------------------------------------------------------------------
 // float64 energy over pixels (the fp64 convergence check)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_energy64(const uint32_t *__restrict__ lab,
+static __global__ __launch_bounds__(256) void k_energy64(const uint32_t *__restrict__ lab,
                                                   const float *__restrict__ rho,
                                                   const float2 *__restrict__ sites, int S,
                                                   int W, int H, double *__restrict__ blockpart)
@@ -457,7 +523,7 @@ __global__ __launch_bounds__(256) void k_energy64(const uint32_t *__restrict__ l
 }
 
 // density check: max over rho and a flag for NaN / inf / negative entries
-__global__ __launch_bounds__(256) void k_density_stats(const float *__restrict__ rho, long n,
+static __global__ __launch_bounds__(256) void k_density_stats(const float *__restrict__ rho, long n,
                                                        float *__restrict__ block_max,
                                                        int *__restrict__ dflags)
 {

@@ -1,0 +1,43 @@
+// cvt_types.h -- types shared by the engine's kernels and its host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CVT_NONE 0x7fffffff
+
+// Fixed-point moment sums of one (tile, site) pair: W = sum rho', DX = sum
+// rho' (px - sx), DY = sum rho' (py - sy), E = sum rho' |p - s|^2, scaled by
+// 2^MOM_FW, 2^MOM_FX, 2^MOM_FX, 2^MOM_FE.  rho' = rho * 2^-k <= 1, a tile has
+// at most 256^2 = 2^16 pixels and |p - s| < 2^15 (sites stay inside
+// [-W, 2W] x [-H, 2H], W, H <= 8192), so each tile sum fits in int64; the
+// per-site totals are summed in 128 bits.  Integer sums are exact, hence
+// independent of order and grouping.
+#define MOM_FW 46
+#define MOM_FX 31
+#define MOM_FE 15
+
+struct __align__(8) Slot {
+    long long w, dx, dy, e;
+    int epoch;
+    int pad;
+};
+
+struct __align__(8) OvfRec {
+    long long w, dx, dy, e;
+    int next;   // next record of the same site, -1 ends the chain
+    int pad;
+};
+
+// device flags
+enum { FLAG_FULL = 1, FLAG_BAD = 2, NUM_FLAGS = 4 };
+// device state words
+enum { ST_EPOCH = 0, ST_HIST = 1, ST_OVFN = 2, ST_NSITES = 3, NUM_STATE = 4 };
+
+
+struct CompatArgs {
+    const uint8_t *mask;   // [H][W] fixed-site pixels (may be null)
+    const float *rho;      // [H][W]
+    int *tgt;              // [S] target pixel index per site
+    uint32_t *owner;       // [H][W] lowest site index landing on a pixel, CVT_NONE else
+};

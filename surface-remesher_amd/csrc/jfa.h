@@ -23,4 +23,30 @@ JfaPlan plan_jfa(int W, int H, int capS);
 void launch_jfa_pass(hipStream_t st, const JfaPlan &plan, const uint32_t *in, uint32_t *out,
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none);
 
+// A patch of a batch, as the batched kernels see it.  Maps are concatenated
+// (pix_off), site tables too (site_off; entry site_off + cap is the patch's
+// sentinel, so `none` = 8 * (site_off + cap)); labels are byte offsets into
+// the concatenated table.
+struct BatchPatch {
+    int W, H;
+    long pix_off;
+    int site_off, cap, S;      // table base, capacity (sentinel index), live sites
+    int passes, first_step;    // JFA schedule
+    int tile, tiles_x, tiles_y, tile_off;   // moment tiles
+    int blk_off, nblk;         // update blocks [blk_off, blk_off + nblk)
+    float rho_scale;           // 2^-k
+    float pad;
+};
+// a workgroup's share of a JFA pass: rows [y0, y1) of one patch
+struct BatchUnit {
+    int patch, y0, y1, pad;
+};
+
+// Pass `pass` (0 .. pmax-1) of the batched JFA over all units; lds_bytes = 0
+// gathers sites from global memory, otherwise every patch's table (at most
+// lds_bytes) is staged in LDS.
+void launch_jfa_batch(hipStream_t st, const BatchPatch *patches, const BatchUnit *units, int nunits,
+                      uint32_t *seed, uint32_t *lab0, uint32_t *lab1, const float2 *sites, int pass,
+                      int pmax, size_t lds_bytes);
+
 }  // namespace cvt

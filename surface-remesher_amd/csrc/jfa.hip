@@ -155,27 +155,16 @@ __device__ __forceinline__ uint32_t small_right(const u32x4 &nb, const u32x4 &c,
     return nb_ok ? q_el(nb, k - 4) : q_el(c, j);
 }
 
-template <bool SITES_IN_LDS>
-__global__ __launch_bounds__(1024) void k_jfa_quad(const uint32_t *__restrict__ in,
-                                                   uint32_t *__restrict__ out,
-                                                   const float2 *__restrict__ sites,
-                                                   int table_f4, int W, int H, int step,
-                                                   uint32_t *__restrict__ clear, uint32_t none)
+// Rows [y0, y1) of one W x H map, all waves of the block.  tab is the site
+// table (LDS or global) and tab_base the byte offset its first entry has in
+// label space (labels are byte offsets into the full table).
+__device__ __forceinline__ void jfa_quad_rows(const uint32_t *__restrict__ in,
+                                              uint32_t *__restrict__ out,
+                                              uint32_t *__restrict__ clear, const char *tab,
+                                              uint32_t tab_base, int W, int H, int y0, int y1,
+                                              int step, uint32_t none)
 {
-    extern __shared__ float4 lds_sites[];
-    if (SITES_IN_LDS) {
-        const float4 *g4 = reinterpret_cast<const float4 *>(sites);
-        for (int i = threadIdx.x; i < table_f4; i += blockDim.x) lds_sites[i] = g4[i];
-        __syncthreads();
-    }
-    const char *tab = SITES_IN_LDS ? reinterpret_cast<const char *>(lds_sites)
-                                   : reinterpret_cast<const char *>(sites);
-    // band of rows of this workgroup; bands are dealt so that each XCD (blocks
-    // go to the 8 XCDs round robin) owns a contiguous stretch of rows
-    const int G = gridDim.x, g = blockIdx.x;
-    const int q8 = G >> 3, r8 = G & 7, xcd = g & 7;
-    const int band = xcd * q8 + min(xcd, r8) + (g >> 3);
-    const int y0 = (int)((long)H * band / G), y1 = (int)((long)H * (band + 1) / G);
+    tab -= tab_base;
     const int nseg = (W + 255) >> 8;              // 256-pixel segments per row
     const int nitems = (y1 - y0) * nseg;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -252,6 +241,91 @@ __global__ __launch_bounds__(1024) void k_jfa_quad(const uint32_t *__restrict__ 
                 __builtin_amdgcn_raw_buffer_store_b128(n4, buf_rsrc(clear, map_bytes), o1, s1, 0);
             }
         }
+    }
+}
+
+template <bool SITES_IN_LDS>
+__global__ __launch_bounds__(1024) void k_jfa_quad(const uint32_t *__restrict__ in,
+                                                   uint32_t *__restrict__ out,
+                                                   const float2 *__restrict__ sites,
+                                                   int table_f4, int W, int H, int step,
+                                                   uint32_t *__restrict__ clear, uint32_t none)
+{
+    extern __shared__ float4 lds_sites[];
+    if (SITES_IN_LDS) {
+        const float4 *g4 = reinterpret_cast<const float4 *>(sites);
+        for (int i = threadIdx.x; i < table_f4; i += blockDim.x) lds_sites[i] = g4[i];
+        __syncthreads();
+    }
+    const char *tab = SITES_IN_LDS ? reinterpret_cast<const char *>(lds_sites)
+                                   : reinterpret_cast<const char *>(sites);
+    // band of rows of this workgroup; bands are dealt so that each XCD (blocks
+    // go to the 8 XCDs round robin) owns a contiguous stretch of rows
+    const int G = gridDim.x, g = blockIdx.x;
+    const int q8 = G >> 3, r8 = G & 7, xcd = g & 7;
+    const int band = xcd * q8 + min(xcd, r8) + (g >> 3);
+    const int y0 = (int)((long)H * band / G), y1 = (int)((long)H * (band + 1) / G);
+    jfa_quad_rows(in, out, clear, tab, 0u, W, H, y0, y1, step, none);
+}
+
+// ---------------------------------------------------------------------------
+// batched pass: one launch over many patches.  Each workgroup takes one work
+// unit (patch, band of rows); patches run their passes right-aligned, so pass
+// k of the batch is pass k - (Pmax - P) of a patch with P passes (skipped
+// when negative) and every patch ends its JFA in the same launch.
+// ---------------------------------------------------------------------------
+template <bool SITES_IN_LDS>
+__global__ __launch_bounds__(1024) void k_jfa_batch(const BatchPatch *__restrict__ patches,
+                                                    const BatchUnit *__restrict__ units,
+                                                    uint32_t *__restrict__ seed,
+                                                    uint32_t *__restrict__ lab0,
+                                                    uint32_t *__restrict__ lab1,
+                                                    const float2 *__restrict__ sites, int pass,
+                                                    int pmax)
+{
+    extern __shared__ float4 lds_sites[];
+    const BatchUnit u = units[blockIdx.x];
+    const BatchPatch P = patches[u.patch];
+    const int local = pass - (pmax - P.passes);
+    if (local < 0) return;                                   // uniform per block
+    const int step = (local < P.passes - 1) ? (P.first_step >> local) : 1;
+    const char *tab;
+    uint32_t base;
+    if (SITES_IN_LDS) {
+        const float4 *g4 = reinterpret_cast<const float4 *>(sites + P.site_off);
+        const int nf4 = (P.cap + 2) / 2;
+        for (int i = threadIdx.x; i < nf4; i += blockDim.x) lds_sites[i] = g4[i];
+        __syncthreads();
+        tab = reinterpret_cast<const char *>(lds_sites);
+        base = 8u * (uint32_t)P.site_off;
+    } else {
+        tab = reinterpret_cast<const char *>(sites);
+        base = 0u;
+    }
+    uint32_t *bufs[2] = {lab0 + P.pix_off, lab1 + P.pix_off};
+    const uint32_t *in = (local == 0) ? seed + P.pix_off : bufs[(local - 1) & 1];
+    uint32_t *out = bufs[local & 1];
+    uint32_t *clear = (local == 1) ? seed + P.pix_off : nullptr;
+    const uint32_t none = 8u * (uint32_t)(P.site_off + P.cap);
+    jfa_quad_rows(in, out, clear, tab, base, P.W, P.H, u.y0, u.y1, step, none);
+}
+
+void launch_jfa_batch(hipStream_t st, const BatchPatch *patches, const BatchUnit *units, int nunits,
+                      uint32_t *seed, uint32_t *lab0, uint32_t *lab1, const float2 *sites, int pass,
+                      int pmax, size_t lds_bytes)
+{
+    if (lds_bytes) {
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void *)k_jfa_batch<true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr = true;
+        }
+        hipLaunchKernelGGL(k_jfa_batch<true>, dim3(nunits), dim3(1024), lds_bytes, st, patches,
+                           units, seed, lab0, lab1, sites, pass, pmax);
+    } else {
+        hipLaunchKernelGGL(k_jfa_batch<false>, dim3(nunits), dim3(1024), 0, st, patches, units,
+                           seed, lab0, lab1, sites, pass, pmax);
     }
 }
 

@@ -1,0 +1,253 @@
+// raster.hip -- density rasterizer (replaces the reference's discretization_d,
+// source/discretization.cu:57-120).
+//
+// Rule (the reference's, source/discretization.cu:57-85): pixel (tx, ty)
+// samples the point (tx * scale, ty * scale) of the shifted UV frame; the first
+// triangle in index order whose three float64 barycentric weights are all >= 0
+// interpolates the vertex weights, rounded once to float; no hit gives 0.
+// Degenerate triangles (zero denominator) never hit.
+//
+// Instead of testing every triangle at every pixel (O(n^2 F)), triangles are
+// binned into 16 x 16 pixel tiles by a conservative bounding box, and each
+// tile keeps its list in increasing triangle order, so the first hit in the
+// list is the first hit overall.  A triangle so thin that rounding could
+// accept points well outside its box (sin^2 of its angle at vertex 1 below
+// 1e-12) goes to an "always" list that every pixel tests, merged in index
+// order with its tile list.  The barycentric arithmetic is the oracle's,
+// operation for operation (contraction is off for this file).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/cvt.h"
+#include "cvt_engine.h"
+
+namespace cvt {
+
+#define RT 16   // tile side in pixels
+
+struct TriBox {
+    int x0, x1, y0, y1;   // inclusive pixel-index range the triangle may touch; x0 > x1 = none
+};
+
+__device__ __forceinline__ void bary(double x1, double y1, double x2, double y2, double x3,
+                                     double y3, double x0, double y0, double &b1, double &b2,
+                                     double &b3)
+{
+    const double v0x = x2 - x1, v0y = y2 - y1;
+    const double v1x = x3 - x1, v1y = y3 - y1;
+    const double v2x = x0 - x1, v2y = y0 - y1;
+    const double d00 = v0x * v0x + v0y * v0y;
+    const double d01 = v0x * v1x + v0y * v1y;
+    const double d11 = v1x * v1x + v1y * v1y;
+    const double d20 = v2x * v0x + v2y * v0y;
+    const double d21 = v2x * v1x + v2y * v1y;
+    const double den = d00 * d11 - d01 * d01;
+    if (den == 0) { b1 = b2 = b3 = -1; return; }
+    const double u = (d11 * d20 - d01 * d21) / den;
+    const double v = (d00 * d21 - d01 * d20) / den;
+    b2 = u;
+    b3 = v;
+    b1 = 1.0 - u - v;
+}
+
+// per triangle: index validation, the conservative pixel box and the
+// ill-conditioned flag
+__global__ void k_tri_boxes(const double *__restrict__ pts, int nv, const int *__restrict__ tri,
+                            int nt, double scale, int n, TriBox *__restrict__ box,
+                            uint8_t *__restrict__ always, int *__restrict__ bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const int a = tri[3 * i], b = tri[3 * i + 1], c = tri[3 * i + 2];
+    TriBox bx = {1, 0, 1, 0};
+    always[i] = 0;
+    if ((unsigned)a >= (unsigned)nv || (unsigned)b >= (unsigned)nv || (unsigned)c >= (unsigned)nv) {
+        *bad = 1;
+        box[i] = bx;
+        return;
+    }
+    const double x1 = pts[2 * a], y1 = pts[2 * a + 1];
+    const double x2 = pts[2 * b], y2 = pts[2 * b + 1];
+    const double x3 = pts[2 * c], y3 = pts[2 * c + 1];
+    const double v0x = x2 - x1, v0y = y2 - y1, v1x = x3 - x1, v1y = y3 - y1;
+    const double d00 = v0x * v0x + v0y * v0y, d11 = v1x * v1x + v1y * v1y;
+    const double d01 = v0x * v1x + v0y * v1y;
+    const double den = d00 * d11 - d01 * d01;
+    if (den == 0) { box[i] = bx; return; }          // never hits
+    if (!(den > 1e-12 * d00 * d11)) {                // thin or non-finite: test everywhere
+        always[i] = 1;
+        box[i] = bx;
+        return;
+    }
+    const double xmin = fmin(x1, fmin(x2, x3)), xmax = fmax(x1, fmax(x2, x3));
+    const double ymin = fmin(y1, fmin(y2, y3)), ymax = fmax(y1, fmax(y2, y3));
+    const double eps = 1e-9 * fmax(xmax - xmin, ymax - ymin) + 1e-300;
+    const double lo_x = (xmin - eps) / scale, hi_x = (xmax + eps) / scale;
+    const double lo_y = (ymin - eps) / scale, hi_y = (ymax + eps) / scale;
+    bx.x0 = (int)fmax(0.0, floor(lo_x));
+    bx.x1 = (int)fmin((double)(n - 1), ceil(hi_x));
+    bx.y0 = (int)fmax(0.0, floor(lo_y));
+    bx.y1 = (int)fmin((double)(n - 1), ceil(hi_y));
+    if (!(hi_x >= 0.0) || !(hi_y >= 0.0) || !(lo_x <= n - 1.0) || !(lo_y <= n - 1.0))
+        bx = TriBox{1, 0, 1, 0};
+    box[i] = bx;
+}
+
+// per tile: the triangles whose box meets the tile, in index order
+// (ballot + prefix count keeps the order); a tile with more than cap
+// triangles is flagged and later tests every triangle
+__global__ __launch_bounds__(256) void k_tile_lists(const TriBox *__restrict__ box, int nt,
+                                                    int tiles_x, int *__restrict__ list,
+                                                    int *__restrict__ count, int cap)
+{
+    const int tile = blockIdx.x;
+    const int tx0 = (tile % tiles_x) * RT, ty0 = (tile / tiles_x) * RT;
+    const int tx1 = tx0 + RT - 1, ty1 = ty0 + RT - 1;
+    __shared__ int wave_n[4];
+    __shared__ int total;
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int base = 0; base < nt; base += 256) {
+        const int i = base + threadIdx.x;
+        bool hit = false;
+        if (i < nt) {
+            const TriBox b = box[i];
+            hit = b.x0 <= b.x1 && b.x0 <= tx1 && b.x1 >= tx0 && b.y0 <= ty1 && b.y1 >= ty0;
+        }
+        const unsigned long long m = __ballot(hit);
+        if (lane == 0) wave_n[wave] = __popcll(m);
+        __syncthreads();
+        int off = total;
+        for (int w = 0; w < wave; ++w) off += wave_n[w];
+        if (hit) {
+            const int k = off + __popcll(m & ((1ull << lane) - 1ull));
+            if (k < cap) list[(long)tile * cap + k] = i;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) total += wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) count[tile] = total;
+}
+
+// one workgroup per tile, one thread per pixel: walk the tile list merged with
+// the "always" list in index order until the first hit
+__global__ __launch_bounds__(256) void k_raster(const double *__restrict__ pts,
+                                                const double *__restrict__ wt,
+                                                const int *__restrict__ tri, int nt,
+                                                const int *__restrict__ list,
+                                                const int *__restrict__ count, int cap,
+                                                const int *__restrict__ alw, int nalw, int tiles_x,
+                                                double scale, int n, float *__restrict__ density)
+{
+    const int tile = blockIdx.x;
+    const int tx = (tile % tiles_x) * RT + (threadIdx.x & (RT - 1));
+    const int ty = (tile / tiles_x) * RT + (threadIdx.x / RT);
+    const int cnt = count[tile];
+    const bool all = cnt > cap;                 // list overflowed: test everything
+    const int len = all ? nt : cnt;
+    if (tx >= n || ty >= n) return;
+    const double x = tx * scale, y = ty * scale;
+    float res = 0.0f;
+    int ia = 0, il = 0;
+    while (true) {
+        // next triangle in index order from the two sorted sources
+        int cand_l = (il < len) ? (all ? il : list[(long)tile * cap + il]) : 0x7fffffff;
+        int cand_a = (!all && ia < nalw) ? alw[ia] : 0x7fffffff;
+        int i;
+        if (cand_l == 0x7fffffff && cand_a == 0x7fffffff) break;
+        if (cand_l <= cand_a) { i = cand_l; ++il; if (cand_a == cand_l) ++ia; }
+        else { i = cand_a; ++ia; }
+        const int p1 = tri[3 * i], p2 = tri[3 * i + 1], p3 = tri[3 * i + 2];
+        double b1, b2, b3;
+        bary(pts[2 * p1], pts[2 * p1 + 1], pts[2 * p2], pts[2 * p2 + 1], pts[2 * p3],
+             pts[2 * p3 + 1], x, y, b1, b2, b3);
+        if (b1 < 0 || b2 < 0 || b3 < 0) continue;
+        res = (float)(b1 * wt[p1] + b2 * wt[p2] + b3 * wt[p3]);
+        break;
+    }
+    density[(long)ty * n + tx] = res;
+}
+
+}  // namespace cvt
+
+using namespace cvt;
+
+#define RCHK(call, what)                                                               \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) { rc = fail(CVT_ERR_HIP, "%s: %s", what, hipGetErrorString(e_)); goto done; } \
+    } while (0)
+
+extern "C" int cvt_rasterize(const double *points, const double *weight, int num_point,
+                             const int *triangle, int num_tri, float *density, double scale, int n)
+{
+    clear_error();
+    if (n < 1 || n > 16384) return fail(CVT_ERR_ARG, "n = %d outside [1, 16384]", n);
+    if (num_point < 0 || num_tri < 0) return fail(CVT_ERR_ARG, "negative sizes");
+    if (!density || (num_point > 0 && (!points || !weight)) || (num_tri > 0 && !triangle))
+        return fail(CVT_ERR_ARG, "null array");
+    if (!(scale > 0.0) || !isfinite(scale)) return fail(CVT_ERR_ARG, "scale %g", scale);
+    const int tiles_x = (n + RT - 1) / RT, ntiles = tiles_x * tiles_x;
+    const int cap = 256;
+    int rc = CVT_OK;
+    hipStream_t st = nullptr;
+    double *d_pts = nullptr, *d_wt = nullptr;
+    int *d_tri = nullptr, *d_list = nullptr, *d_count = nullptr, *d_alw = nullptr, *d_bad = nullptr;
+    TriBox *d_box = nullptr;
+    uint8_t *d_always = nullptr;
+    float *d_den = nullptr;
+    int bad = 0, nalw = 0;
+    std::vector<uint8_t> always;
+    std::vector<int> alw;
+    const long npx = (long)n * n;
+    RCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    RCHK(hipMalloc(&d_pts, sizeof(double) * 2 * std::max(num_point, 1)), "alloc");
+    RCHK(hipMalloc(&d_wt, sizeof(double) * std::max(num_point, 1)), "alloc");
+    RCHK(hipMalloc(&d_tri, sizeof(int) * 3 * std::max(num_tri, 1)), "alloc");
+    RCHK(hipMalloc(&d_box, sizeof(TriBox) * std::max(num_tri, 1)), "alloc");
+    RCHK(hipMalloc(&d_always, std::max(num_tri, 1)), "alloc");
+    RCHK(hipMalloc(&d_alw, sizeof(int) * std::max(num_tri, 1)), "alloc");
+    RCHK(hipMalloc(&d_list, sizeof(int) * (size_t)ntiles * cap), "alloc");
+    RCHK(hipMalloc(&d_count, sizeof(int) * ntiles), "alloc");
+    RCHK(hipMalloc(&d_bad, sizeof(int)), "alloc");
+    RCHK(hipMalloc(&d_den, sizeof(float) * npx), "alloc");
+    if (num_point > 0) {
+        RCHK(hipMemcpyAsync(d_pts, points, sizeof(double) * 2 * num_point, hipMemcpyDefault, st), "upload");
+        RCHK(hipMemcpyAsync(d_wt, weight, sizeof(double) * num_point, hipMemcpyDefault, st), "upload");
+    }
+    if (num_tri > 0)
+        RCHK(hipMemcpyAsync(d_tri, triangle, sizeof(int) * 3 * num_tri, hipMemcpyDefault, st), "upload");
+    RCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st), "memset");
+    if (num_tri > 0)
+        hipLaunchKernelGGL(k_tri_boxes, dim3((num_tri + 255) / 256), dim3(256), 0, st, d_pts,
+                           num_point, d_tri, num_tri, scale, n, d_box, d_always, d_bad);
+    RCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st), "flag");
+    always.resize(std::max(num_tri, 1));
+    if (num_tri > 0)
+        RCHK(hipMemcpyAsync(always.data(), d_always, num_tri, hipMemcpyDeviceToHost, st), "flags");
+    RCHK(hipStreamSynchronize(st), "boxes");
+    if (bad) { rc = fail(CVT_ERR_ARG, "a triangle indexes a vertex outside [0, %d)", num_point); goto done; }
+    for (int i = 0; i < num_tri; ++i)
+        if (always[i]) alw.push_back(i);
+    nalw = (int)alw.size();
+    if (nalw) RCHK(hipMemcpyAsync(d_alw, alw.data(), sizeof(int) * nalw, hipMemcpyHostToDevice, st), "upload");
+    hipLaunchKernelGGL(k_tile_lists, dim3(ntiles), dim3(256), 0, st, d_box, num_tri, tiles_x, d_list,
+                       d_count, cap);
+    hipLaunchKernelGGL(k_raster, dim3(ntiles), dim3(256), 0, st, d_pts, d_wt, d_tri, num_tri, d_list,
+                       d_count, cap, d_alw, nalw, tiles_x, scale, n, d_den);
+    RCHK(hipGetLastError(), "raster launch");
+    RCHK(hipMemcpyAsync(density, d_den, sizeof(float) * npx, hipMemcpyDefault, st), "download");
+    RCHK(hipStreamSynchronize(st), "raster");
+done:
+    void *bufs[] = {d_pts, d_wt, d_tri, d_box, d_always, d_alw, d_list, d_count, d_bad, d_den};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}

@@ -5,7 +5,7 @@ ABI of include/cvt.h); this package loads it with ctypes and provides the
 seeded workload generators used by the tests and the benchmark.
 """
 from ._lib import LIB_PATH, LABEL_NONE, CvtError, lib
-from .engine import Engine
-from . import workloads
+from .engine import Batch, Engine
+from . import compat, workloads
 
-__all__ = ["Engine", "CvtError", "LABEL_NONE", "LIB_PATH", "lib", "workloads"]
+__all__ = ["Batch", "Engine", "CvtError", "LABEL_NONE", "LIB_PATH", "lib", "compat", "workloads"]

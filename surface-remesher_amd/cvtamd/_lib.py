@@ -57,6 +57,22 @@ SIGNATURES = {
     "cvt_energy": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
     "cvt_synchronize": (_i, [_vp]),
     "cvt_profile": (_i, [_vp, _i, _f, _vp]),
+    "cvt_compat_gcvt": (_i, [_vp, _vp, _vp, _i, _i, _i, C.POINTER(_i), C.POINTER(_i)]),
+    "cvt_rasterize": (_i, [_vp, _vp, _i, _vp, _i, _vp, _d, _i]),
+    "cvt_extract_points": (_i, [_vp, _vp, _i, _d, _d, _d, _vp, C.c_long, C.POINTER(C.c_long)]),
+    "cvt_batch_create": (_i, [C.POINTER(_vp), _i, _vp, _vp, _vp]),
+    "cvt_batch_destroy": (_i, [_vp]),
+    "cvt_batch_set_stream": (_i, [_vp, _vp]),
+    "cvt_batch_set_density": (_i, [_vp, _i, _vp]),
+    "cvt_batch_set_sites": (_i, [_vp, _i, _vp, _vp, _i]),
+    "cvt_batch_get_sites": (_i, [_vp, _i, _vp]),
+    "cvt_batch_lloyd": (_i, [_vp, _i, _f]),
+    "cvt_batch_voronoi": (_i, [_vp]),
+    "cvt_batch_get_labels": (_i, [_vp, _i, _vp]),
+    "cvt_batch_history": (_i, [_vp, _i, _vp, _i]),
+    "cvt_batch_set_tile": (_i, [_vp, _i, _i]),
+    "cvt_batch_get_tile": (_i, [_vp, _i]),
+    "cvt_batch_synchronize": (_i, [_vp]),
     "cvt_set_tile": (_i, [_vp, _i]),
     "cvt_get_tile": (_i, [_vp]),
 }

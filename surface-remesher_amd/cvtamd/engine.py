@@ -147,3 +147,89 @@ class Engine:
 
 
 __all__ = ["Engine", "LABEL_NONE"]
+
+
+class Batch:
+    """Many patches advanced together (cvt_batch_* of include/cvt.h)."""
+
+    def __init__(self, sizes, max_sites):
+        """sizes: [(width, height)], max_sites: [int] per patch."""
+        w = np.array([s[0] for s in sizes], np.int32)
+        h = np.array([s[1] for s in sizes], np.int32)
+        m = np.array(max_sites, np.int32)
+        handle = C.c_void_p()
+        call("cvt_batch_create", C.byref(handle), len(sizes), w.ctypes.data_as(C.c_void_p),
+             h.ctypes.data_as(C.c_void_p), m.ctypes.data_as(C.c_void_p))
+        self._h = handle
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        self._S = [0] * len(sizes)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cvt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def set_stream(self, stream_handle):
+        call("cvt_batch_set_stream", self._h, C.c_void_p(stream_handle or 0))
+
+    def set_density(self, p, density):
+        pd, keep = _ptr(density, np.float32)
+        call("cvt_batch_set_density", self._h, int(p), pd)
+
+    def set_sites(self, p, xy, fixed=None):
+        if hasattr(xy, "numel"):
+            n = xy.numel() // 2
+        else:
+            xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+            n = xy.shape[0]
+        px, k1 = _ptr(xy, np.float32)
+        pf, k2 = _ptr(fixed, np.uint8)
+        call("cvt_batch_set_sites", self._h, int(p), px, pf, int(n))
+        self._S[p] = int(n)
+
+    def sites(self, p):
+        out = np.empty((self._S[p], 2), np.float32)
+        call("cvt_batch_get_sites", self._h, int(p), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def lloyd(self, iterations, omega=1.0):
+        call("cvt_batch_lloyd", self._h, int(iterations), float(omega))
+
+    def voronoi(self):
+        call("cvt_batch_voronoi", self._h)
+
+    def labels(self, p):
+        w, h = self.sizes[p]
+        out = np.empty((h, w), np.int32)
+        call("cvt_batch_get_labels", self._h, int(p), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def history(self, p):
+        n = call("cvt_batch_history", self._h, int(p), None, 0)
+        out = np.zeros((max(n, 1), 2), np.float64)
+        k = call("cvt_batch_history", self._h, int(p), out.ctypes.data_as(C.c_void_p), n)
+        return out[:k]
+
+    def set_tile(self, p, tile):
+        call("cvt_batch_set_tile", self._h, int(p), int(tile))
+
+    def tile(self, p):
+        return call("cvt_batch_get_tile", self._h, int(p))
+
+    def synchronize(self):
+        call("cvt_batch_synchronize", self._h)

@@ -38,6 +38,14 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_reference_signature_shims_exported():
+    # C++ linkage, the reference's exact parameter lists (include/gcvt_compat.h)
+    import cvtamd
+    L = C.CDLL(str(cvtamd.LIB_PATH))
+    assert hasattr(L, "_Z4gCVTPsPfPbiii")                  # gCVT(short*, float*, bool*, int, int, int)
+    assert hasattr(L, "_Z16discretization_dPdS_iPiiPfdi")  # discretization_d(double*, double*, int, int*, int, float*, double, int)
+
+
 def test_python_binding_covers_header():
     from cvtamd import _lib
     assert declared_symbols() <= set(_lib.SIGNATURES), declared_symbols() - set(_lib.SIGNATURES)
