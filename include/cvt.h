@@ -142,7 +142,8 @@ int cvt_rasterize(const double *points, const double *weight, int num_point, con
  * pixels (x, y) whose label is their own coordinates and whose mask byte is
  * 0 (mask may be null), in the reference's order (x outer, y inner), as
  * (x*scale + l, y*scale + b) float64 pairs.  *count receives the number
- * found; at most capacity pairs are written (CVT_ERR_CAPACITY if more). */
+ * found; at most capacity pairs are written (CVT_ERR_CAPACITY if more);
+ * out = NULL only counts. */
 int cvt_extract_points(const int16_t *voronoi, const uint8_t *mask, int n, double scale, double l,
                        double b, double *out, long capacity, long *count);
 

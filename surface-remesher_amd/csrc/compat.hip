@@ -394,7 +394,8 @@ extern "C" int cvt_extract_points(const int16_t *voronoi, const uint8_t *mask, i
                             hipMemcpyDefault, st), "download");
         CCHK(hipStreamSynchronize(st), "write pass");
     }
-    if (total > capacity) rc = fail(CVT_ERR_CAPACITY, "%ld points, capacity %ld", total, capacity);
+    // out == NULL is a count query; otherwise every point must fit
+    if (out && total > capacity) rc = fail(CVT_ERR_CAPACITY, "%ld points, capacity %ld", total, capacity);
 done:
     void *bufs[] = {dv, dm, dcnt, doff, dout};
     for (void *p : bufs)
