@@ -178,6 +178,44 @@ int cvt_batch_set_tile(cvt_batch *batch, int patch, int tile);
 int cvt_batch_get_tile(cvt_batch *batch, int patch);
 int cvt_batch_synchronize(cvt_batch *batch);
 
+/* ---------------------------------------------------------------------------
+ * Learned warm start (no reference counterpart): a 4-level convolutional
+ * U-Net, 32 / 64 / 96 / 128 channels, BF16 weights and activations with FP32
+ * accumulation on MFMA, FP32 master weights, Adam.  Input R x R (R = 256 for
+ * the warm start): density box mean / mean, initial-site count per cell *
+ * R^2 / S, and the domain indicator; output a 2-channel displacement field
+ * (units of the domain side) sampled bilinearly at the sites.  All buffers
+ * may be host or device pointers.
+ * ------------------------------------------------------------------------- */
+typedef struct cvt_unet cvt_unet;
+
+int cvt_unet_create(cvt_unet **net, int R, int max_batch, unsigned long long seed);
+int cvt_unet_destroy(cvt_unet *net);
+int cvt_unet_set_stream(cvt_unet *net, void *hip_stream);
+long cvt_unet_num_params(cvt_unet *net);
+int cvt_unet_get_params(cvt_unet *net, float *params);
+int cvt_unet_set_params(cvt_unet *net, const float *params);
+int cvt_unet_get_grads(cvt_unet *net, float *grads);
+/* Input of batch slot `slot` (dst = NULL) or into dst (R*R*32 bf16) from an
+ * n x n density (n a multiple of R) and S sites in pixel units. */
+int cvt_unet_prepare(cvt_unet *net, int slot, const float *density, int n, const float *xy, int S,
+                     uint16_t *dst);
+/* nb inputs (nb * R * R * 32 bf16, NHWC) into the batch slots. */
+int cvt_unet_set_input(cvt_unet *net, const uint16_t *x, int nb);
+int cvt_unet_forward(cvt_unet *net, int nb);
+int cvt_unet_get_field(cvt_unet *net, float *field, int nb);
+/* Loss (mean |field(uv) - target|^2 over nb * S sites, uv = position / side)
+ * and the backward pass into the gradients; *loss (optional) synchronises. */
+int cvt_unet_loss_backward(cvt_unet *net, int nb, const float *uv, const float *target, int S,
+                           float *loss);
+int cvt_unet_adam(cvt_unet *net, float lr, float beta1, float beta2, float eps);
+/* Displacements of S sites of batch slot `slot` from the last forward pass. */
+int cvt_unet_sample(cvt_unet *net, int slot, const float *uv, int S, float *disp);
+/* Weights as a raw bf16 blob ("CVTUNET1", R, count, values). */
+int cvt_unet_save(cvt_unet *net, const char *path);
+int cvt_unet_load(cvt_unet *net, const char *path);
+int cvt_unet_synchronize(cvt_unet *net);
+
 /* Tuning: moment tile side (32, 64, 128 or 256; 0 = choose from the site
  * density).  Results do not depend on it. */
 int cvt_set_tile(cvt_ctx *ctx, int tile);

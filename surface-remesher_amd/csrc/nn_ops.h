@@ -1,0 +1,49 @@
+// nn_ops.h -- launchers of the warm-start network's ops (nn_ops.hip).
+// Tensors NHWC, activations bf16 (uint16_t bits), parameters fp32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cvt {
+
+// 3x3 conv, padding 1: out = act(bias + conv(in, w)); w [9][Cout][Cin] bf16;
+// Cin % 32 == 0, Cout % 32 == 0
+void nn_conv3x3(hipStream_t s, const uint16_t *in, const uint16_t *w, const float *bias,
+                uint16_t *out, int N, int H, int W, int Cin, int Cout, int relu);
+// weight gradient dw[9][Cout][Cin] (+= when accumulate) of out-gradient g;
+// work holds nn_dw_splits(M) * 9 * Cout * Cin floats
+void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *dw, float *work,
+                   int N, int H, int W, int Cin, int Cout, int accumulate);
+int nn_dw_splits(long M);
+// db[c] = sum_m g[m][c]; work holds nn_dw_splits(M) * C floats
+void nn_colsum(hipStream_t s, const uint16_t *g, long M, int C, float *db, float *work);
+// fp32 weights -> bf16 forward copy and flipped/transposed data-gradient copy
+void nn_weights_bf16(hipStream_t s, const float *w, int Cin, int Cout, uint16_t *wf, uint16_t *wb);
+void nn_relu_bwd(hipStream_t s, uint16_t *g, const uint16_t *y, long n);
+void nn_avgpool2(hipStream_t s, const uint16_t *in, uint16_t *out, int N, int H, int W, int C);
+void nn_avgpool2_bwd(hipStream_t s, const uint16_t *gout, uint16_t *gin, int N, int H, int W, int C,
+                     int accumulate);
+// (N, H, W, C1) low upsampled x2 into channels [0, C1), skip (N, 2H, 2W, C2)
+// into [C1, C1 + C2)
+void nn_up_concat(hipStream_t s, const uint16_t *low, const uint16_t *skip, uint16_t *out, int N,
+                  int H, int W, int C1, int C2);
+void nn_up_concat_bwd(hipStream_t s, const uint16_t *gout, uint16_t *glow, uint16_t *gskip, int N,
+                      int H, int W, int C1, int C2, int acc_skip);
+// 1x1 head C -> 2 (fp32 out); dwh_db = [dwh (2C), dbh (2)]; work 256 * (2C + 2)
+void nn_head_fwd(hipStream_t s, const uint16_t *x, const float *wh, const float *bh, float *f,
+                 long M, int C);
+void nn_head_bwd(hipStream_t s, const uint16_t *x, const float *gf, const float *wh, uint16_t *gx,
+                 float *dwh_db, float *work, long M, int C);
+// bilinear sample of (nb, R, R, 2) at uv [nb][S][2]; loss partials per 256
+// sites; gfield (zeroed by the caller) receives the gradient
+void nn_sample_loss(hipStream_t s, const float *field, int R, const float *uv, const float *target,
+                    int S, int nb, float scale, float *pred, float *gfield, float *loss_part);
+void nn_adam(hipStream_t s, float *w, const float *g, float *m1, float *m2, long n, float lr,
+             float b1, float b2, float eps, int t);
+// network input (R, R, C) from an n x n density and S sites (pixel units);
+// count is an R*R float scratch
+void nn_prepare(hipStream_t s, const float *rho, int n, float inv_mean, const float *xy, int S,
+                int R, int C, uint16_t *out, float *count);
+
+}  // namespace cvt

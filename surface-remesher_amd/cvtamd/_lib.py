@@ -73,6 +73,23 @@ SIGNATURES = {
     "cvt_batch_set_tile": (_i, [_vp, _i, _i]),
     "cvt_batch_get_tile": (_i, [_vp, _i]),
     "cvt_batch_synchronize": (_i, [_vp]),
+    "cvt_unet_create": (_i, [C.POINTER(_vp), _i, _i, C.c_ulonglong]),
+    "cvt_unet_destroy": (_i, [_vp]),
+    "cvt_unet_set_stream": (_i, [_vp, _vp]),
+    "cvt_unet_num_params": (C.c_long, [_vp]),
+    "cvt_unet_get_params": (_i, [_vp, _vp]),
+    "cvt_unet_set_params": (_i, [_vp, _vp]),
+    "cvt_unet_get_grads": (_i, [_vp, _vp]),
+    "cvt_unet_prepare": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp]),
+    "cvt_unet_set_input": (_i, [_vp, _vp, _i]),
+    "cvt_unet_forward": (_i, [_vp, _i]),
+    "cvt_unet_get_field": (_i, [_vp, _vp, _i]),
+    "cvt_unet_loss_backward": (_i, [_vp, _i, _vp, _vp, _i, C.POINTER(_f)]),
+    "cvt_unet_adam": (_i, [_vp, _f, _f, _f, _f]),
+    "cvt_unet_sample": (_i, [_vp, _i, _vp, _i, _vp]),
+    "cvt_unet_save": (_i, [_vp, C.c_char_p]),
+    "cvt_unet_load": (_i, [_vp, C.c_char_p]),
+    "cvt_unet_synchronize": (_i, [_vp]),
     "cvt_set_tile": (_i, [_vp, _i]),
     "cvt_get_tile": (_i, [_vp]),
 }
