@@ -108,10 +108,11 @@ int cvt_clear_history(cvt_ctx *ctx);
  * gradient norm, reduced on the GPU in a fixed order. */
 int cvt_energy(cvt_ctx *ctx, double *energy, double *gradnorm);
 int cvt_synchronize(cvt_ctx *ctx);
-/* Runs `iterations` Lloyd steps eagerly with HIP events between launches and
- * returns mean durations in ms: ms[0] one JFA pass, ms[1] the moment tile
- * kernel, ms[2] the overflow kernel, ms[3] update + finalize, ms[4] 0,
- * ms[5] the whole step, ms[6] the JFA pass count, ms[7] the moment tile.
+/* Runs `iterations` (1..1000) Lloyd steps eagerly with HIP events around
+ * each phase and returns mean durations in ms: ms[0] one JFA pass (the
+ * back-to-back pass sequence over the pass count), ms[1] the moment kernel,
+ * ms[2] 0, ms[3] update + finalize, ms[4] 0, ms[5] the whole step, ms[6] the
+ * JFA pass count, ms[7] the moment tile.
  * (No reference counterpart: the reference times stages with clock(),
  * source/main.cpp:181-242.) */
 int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms);
@@ -211,13 +212,23 @@ int cvt_unet_loss_backward(cvt_unet *net, int nb, const float *uv, const float *
 int cvt_unet_adam(cvt_unet *net, float lr, float beta1, float beta2, float eps);
 /* Displacements of S sites of batch slot `slot` from the last forward pass. */
 int cvt_unet_sample(cvt_unet *net, int slot, const float *uv, int S, float *disp);
+/* Warm start of an engine context in place, on the context's stream and
+ * without host round trips: the input is built from the context's density
+ * and current sites (the grid must be square, a multiple of R), the network
+ * runs, and each free site moves by the field sampled at its position.  A
+ * site keeps its position when it is fixed or when the target pixel lies off
+ * the grid or has zero density.  Uses batch slot 0.  Plain Lloyd continues
+ * from the moved sites; the caller keeps the old sites to fall back to. */
+int cvt_unet_warm_start(cvt_unet *net, cvt_ctx *ctx);
 /* Weights as a raw bf16 blob ("CVTUNET1", R, count, values). */
 int cvt_unet_save(cvt_unet *net, const char *path);
 int cvt_unet_load(cvt_unet *net, const char *path);
 int cvt_unet_synchronize(cvt_unet *net);
 
 /* Tuning: moment tile side (32, 64, 128 or 256; 0 = choose from the site
- * density).  Results do not depend on it. */
+ * density).  Results do not depend on it (for sides >= 64 they are identical
+ * bit for bit).  A tile can meet at most 1024, 256, 512 or 1024 distinct
+ * sites respectively; more is reported as CVT_ERR_CAPACITY. */
 int cvt_set_tile(cvt_ctx *ctx, int tile);
 int cvt_get_tile(cvt_ctx *ctx);
 

@@ -26,8 +26,8 @@
 
 namespace cvt {
 
-// moments of one tile of one patch (runtime tile side T >= 64: 64-pixel row
-// segments, one per wave step); same arithmetic as k_moments
+// moments of one tile of one patch (tile side T >= 64): the single-map tile
+// routine with 64-pixel segments, so a patch's sums equal its single-map run
 static __global__ __launch_bounds__(256) void k_moments_batch(
     const BatchPatch *__restrict__ patches, const int *__restrict__ tile_patch,
     const uint32_t *__restrict__ lab0, const uint32_t *__restrict__ lab1,
@@ -35,126 +35,18 @@ static __global__ __launch_bounds__(256) void k_moments_batch(
     OvfRec *__restrict__ ovf, int ovf_cap, int *__restrict__ ovf_head, int *__restrict__ dflags,
     int *__restrict__ dstate)
 {
-    __shared__ int keys[CVT_HASH_CAP];
-    __shared__ unsigned long long acc[CVT_HASH_CAP][4];
-    __shared__ int full;
+    __shared__ TileLds<CVT_HASH_CAP> L;
     const int pid = tile_patch[blockIdx.x];
     const BatchPatch P = patches[pid];
     const int T = P.tile;
     const int tile = blockIdx.x - P.tile_off;
     const int tX = tile % P.tiles_x, tY = tile / P.tiles_x;
-    const int tx0 = tX * T, ty0 = tY * T;
-    const int W = P.W, H = P.H;
     const uint32_t *lab = ((P.passes - 1) & 1 ? lab1 : lab0) + P.pix_off;
-    const float *rh = rho + P.pix_off;
-    const uint32_t base = (uint32_t)P.site_off;
-    for (int j = threadIdx.x; j < CVT_HASH_CAP; j += 256) {
-        keys[j] = -1;
-        acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0ull;
-    }
-    if (threadIdx.x == 0) full = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row16 = lane & 15;
-    const int spr = T / 64, groups = T * T / 64;
-    for (int g = wave; g < groups; g += 4) {
-        const int r = g / spr;
-        const int c0 = (g % spr) * 64;
-        const int x = tx0 + c0 + lane, y = ty0 + r;
-        int key = -1;
-        float rs = 0.0f;
-        if (x < W && y < H) {
-            const long p = (long)y * W + x;
-            const int l = (int)(lab[p] >> 3) - (int)base;
-            const float rv = rh[p];
-            if ((unsigned)l < (unsigned)P.S && rv != 0.0f) {
-                key = l;
-                rs = rv * P.rho_scale;
-            }
-        }
-        const int kprev = dpp_wave_shr1(key, -2);
-        const bool head = (lane == 0) || (kprev != key);
-        int start = head ? lane : 0;
-        start = max(start, dpp_row_shr<1>(start));
-        start = max(start, dpp_row_shr<2>(start));
-        start = max(start, dpp_row_shr<4>(start));
-        start = max(start, dpp_row_shr<8>(start));
-        start = max(start, dpp_bcast15(start));
-        start = max(start, dpp_bcast31(start));
-        const float u = (float)(lane - start);
-        float a = rs, b = rs * u, cc = rs * (u * u);
-#define CVT_SCAN_STEP(MOVE, SRC)                                                   \
-        {                                                                          \
-            const int ta = MOVE(ibits(a)), tb = MOVE(ibits(b)), tc = MOVE(ibits(cc)); \
-            a = seg_add(a, ta, SRC, start);                                        \
-            b = seg_add(b, tb, SRC, start);                                        \
-            cc = seg_add(cc, tc, SRC, start);                                      \
-        }
-        CVT_SCAN_STEP(dpp_row_shr<1>, row16 >= 1 ? lane - 1 : -1)
-        CVT_SCAN_STEP(dpp_row_shr<2>, row16 >= 2 ? lane - 2 : -1)
-        CVT_SCAN_STEP(dpp_row_shr<4>, row16 >= 4 ? lane - 4 : -1)
-        CVT_SCAN_STEP(dpp_row_shr<8>, row16 >= 8 ? lane - 8 : -1)
-        CVT_SCAN_STEP(dpp_bcast15, ((lane >> 4) & 1) ? (lane & ~15) - 1 : -1)
-        CVT_SCAN_STEP(dpp_bcast31, (lane >= 32) ? 31 : -1)
-#undef CVT_SCAN_STEP
-        const int knext = dpp_wave_shl1(key, -2);
-        const bool tail = (lane == 63) || (knext != key);
-        if (tail && key >= 0) {
-            const float2 st = sites[base + key];
-            const double A = a, B = b, Cc = cc;
-            const double ox = ((double)(x - (lane - start)) + 0.5) - (double)st.x;
-            const double oy = ((double)y + 0.5) - (double)st.y;
-            const double DX = B + A * ox;
-            const double DY = A * oy;
-            const double E = Cc + 2.0 * ox * B + A * (ox * ox + oy * oy);
-            const unsigned hsh = ((unsigned)key * 2654435761u) >> (32 - 10);
-            int slot = -1;
-            for (int probe = 0; probe < CVT_HASH_CAP; ++probe) {
-                const int h = (hsh + probe) & (CVT_HASH_CAP - 1);
-                const int old = atomicCAS(&keys[h], -1, key);
-                if (old == -1 || old == key) { slot = h; break; }
-            }
-            if (slot >= 0) {
-                atomicAdd(&acc[slot][0], (unsigned long long)cvt_fix(A, MOM_FW));
-                atomicAdd(&acc[slot][1], (unsigned long long)cvt_fix(DX, MOM_FX));
-                atomicAdd(&acc[slot][2], (unsigned long long)cvt_fix(DY, MOM_FX));
-                atomicAdd(&acc[slot][3], (unsigned long long)cvt_fix(E, MOM_FE));
-            } else {
-                full = 1;
-            }
-        }
-    }
-    __syncthreads();
-    const int epoch = dstate[ST_EPOCH];
-    for (int j = threadIdx.x; j < CVT_HASH_CAP; j += 256) {
-        const int l = keys[j];
-        if (l < 0) continue;
-        const int gl = (int)base + l;
-        const float2 s = sites[gl];
-        const long long wt = (long long)acc[j][0], dx = (long long)acc[j][1];
-        const long long dy = (long long)acc[j][2], ee = (long long)acc[j][3];
-        const int hx = min(max((int)floorf(s.x / (float)T), 0), P.tiles_x - 1);
-        const int hy = min(max((int)floorf(s.y / (float)T), 0), P.tiles_y - 1);
-        const int rx = tX - hx, ry = tY - hy;
-        if (rx >= -1 && rx <= 1 && ry >= -1 && ry <= 1) {
-            Slot sl;
-            sl.w = wt; sl.dx = dx; sl.dy = dy; sl.e = ee; sl.epoch = epoch; sl.pad = 0;
-            slots[(long)gl * 9 + (ry + 1) * 3 + (rx + 1)] = sl;
-        } else {
-            const int k = atomicAdd(&dstate[ST_OVFN], 1);
-            if (k < ovf_cap) {
-                OvfRec o;
-                o.w = wt; o.dx = dx; o.dy = dy; o.e = ee;
-                o.next = atomicExch(&ovf_head[gl], k);
-                o.pad = 0;
-                ovf[k] = o;
-            } else {
-                full = 1;
-            }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && full) dflags[FLAG_FULL] = 1;
+    tile_lds_init(L);
+    tile_moments<64>(L, lab, rho + P.pix_off, sites + P.site_off, P.site_off, P.S, P.W, P.H,
+                     P.rho_scale, tX * T, tY * T, T);
+    tile_flush(L, sites, P.site_off, T, tX, tY, P.tiles_x, P.tiles_y, dstate[ST_EPOCH], slots, ovf,
+               ovf_cap, ovf_head, dstate, dflags);
 }
 
 // site update of one block of one patch (continuous rule, as k_update)
@@ -310,6 +202,7 @@ struct Batch {
     int plan();
     int upload_patches();
     int enqueue_iteration(float omega);
+    unsigned pass_modes(int k) const;
 };
 
 static int g_cu_count()
@@ -398,11 +291,24 @@ int Batch::upload_patches()
     return CVT_OK;
 }
 
+// step classes (bit 0: >= 4, bit 1: 1, bit 2: 2) that batched pass k runs
+unsigned Batch::pass_modes(int k) const
+{
+    unsigned m = 0;
+    for (const BatchPatch &P : hp) {
+        const int local = k - (pmax - P.passes);
+        if (local < 0) continue;
+        const int step = (local < P.passes - 1) ? (P.first_step >> local) : 1;
+        m |= 1u << (step == 1 ? 1 : step == 2 ? 2 : 0);
+    }
+    return m;
+}
+
 int Batch::enqueue_iteration(float omega)
 {
     for (int k = 0; k < pmax; ++k)
         launch_jfa_batch(stream, d_patch, d_units, (int)units.size(), seed, lab0, lab1, sites, k,
-                         pmax, lds_bytes);
+                         pmax, lds_bytes, pass_modes(k));
     hipLaunchKernelGGL(k_moments_batch, dim3(ntiles), dim3(256), 0, stream, d_patch, d_tile_patch,
                        lab0, lab1, rho, sites, slots, ovf, ovf_cap, ovf_head, dflags, dstate);
     hipLaunchKernelGGL(k_update_batch, dim3(nblk), dim3(256), 0, stream, d_patch, d_blk_patch, sites,
@@ -654,7 +560,7 @@ int cvt_batch_voronoi(cvt_batch *bh)
     if (rc) return rc;
     for (int k = 0; k < b->pmax; ++k)
         launch_jfa_batch(b->stream, b->d_patch, b->d_units, (int)b->units.size(), b->seed, b->lab0,
-                         b->lab1, b->sites, k, b->pmax, b->lds_bytes);
+                         b->lab1, b->sites, k, b->pmax, b->lds_bytes, b->pass_modes(k));
     b->seeds_valid = false;
     b->labels_fresh = true;
     return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "voronoi");

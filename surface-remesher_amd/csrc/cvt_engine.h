@@ -49,8 +49,6 @@ struct Ctx {
     float *scratch = nullptr;
     double *eg = nullptr;
 
-    hipEvent_t *prof_events = nullptr;   // per-pass events while profiling
-    hipEvent_t prof_mid = nullptr;
 
     bool use_graph = true;
     bool graph_valid = false;
@@ -60,6 +58,7 @@ struct Ctx {
 
     ~Ctx();
     int enqueue_seed();
+    void clear_seed_map();         // every pixel back to none()
     int enqueue_jfa();
     int enqueue_moments();
     int enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg,

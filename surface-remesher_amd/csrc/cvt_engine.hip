@@ -87,23 +87,10 @@ static int launch_moments_on(Ctx *c, const uint32_t *lab)
                            c->ovf_head, c->dflags, c->dstate);
         break;
     }
-    if (c->prof_mid) hipEventRecord(c->prof_mid, c->stream);
     return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "moment launch failed");
 }
 
 // moments with an event between the tile kernel and the overflow kernel
-static int launch_moments_timed(Ctx *c, hipEvent_t after_tiles, hipEvent_t after_ovf)
-{
-    hipEvent_t *saved = c->prof_events;
-    c->prof_events = nullptr;
-    c->prof_mid = after_tiles;
-    int rc = launch_moments_on(c, c->lab[c->final_buf]);
-    c->prof_mid = nullptr;
-    c->prof_events = saved;
-    hipEventRecord(after_ovf, c->stream);
-    return rc;
-}
-
 static int set_tile_internal(Ctx *c, int tile)
 {
     if (tile == 0) tile = choose_tile(c->W, c->H, c->S);
@@ -135,9 +122,10 @@ int Ctx::check_flags()
         int z = 0;
         hipMemcpyAsync(dflags + FLAG_FULL, &z, sizeof z, hipMemcpyHostToDevice, stream);
         hipStreamSynchronize(stream);
+        const int cap = tile == 64 ? TileHash<64>::HC : tile == 128 ? TileHash<128>::HC : CVT_HASH_CAP;
         return fail(CVT_ERR_CAPACITY,
                     "more than %d distinct sites met one %dx%d moment tile; use a smaller tile",
-                    CVT_HASH_CAP, tile, tile);
+                    cap, tile, tile);
     }
     return CVT_OK;
 }
@@ -154,6 +142,11 @@ int Ctx::enqueue_seed()
     return CVT_OK;
 }
 
+void Ctx::clear_seed_map()
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, stream, seed, none(), (long)W * H);
+}
+
 // JFA(+1) of the current sites; the labels end in lab[final_buf]
 int Ctx::enqueue_jfa()
 {
@@ -164,7 +157,6 @@ int Ctx::enqueue_jfa()
     for (int k = 0; k < np; ++k) {
         uint32_t *out = lab[cur];
         launch_jfa_pass(stream, jplan, in, out, sites, W, H, step, k == 1 ? seed : nullptr, none());
-        if (prof_events) hipEventRecord(prof_events[k + 1], stream);
         in = out;
         cur ^= 1;
         if (step > 1) step >>= 1;
@@ -178,13 +170,17 @@ int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, doub
                         const float *domega, CompatArgs ca)
 {
     const int nb = std::max(1, (S + 255) / 256);
+    const double a = (double)W * H;
+    FinalArgs fa;
+    fa.hist = hist;
+    fa.hist_cap = hist_cap;
+    fa.record = record;
+    fa.e_scale = ldexp(1.0, rho_exp) / a;
+    fa.g_scale = 2.0 * ldexp(1.0, rho_exp) / a;
+    fa.out_eg = out_eg;
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, ovf,
                        ovf_head, dflags, dstate, omega, domega, mode, ldexp(1.0, rho_exp), mom_out,
-                       blockpart, seed, W, H, ca);
-    const double a = (double)W * H;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, blockpart, nb, dstate, dflags,
-                       hist, hist_cap, record, ldexp(1.0, rho_exp) / a, 2.0 * ldexp(1.0, rho_exp) / a,
-                       out_eg);
+                       blockpart, seed, W, H, ca, fa);
     if (mode & UPD_SEED) seeds_valid = true;
     return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "update launch failed");
 }
@@ -582,39 +578,44 @@ int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms)
     CHECK_CTX(ctx);
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c->have_density) return fail(CVT_ERR_STATE, "cvt_profile before cvt_set_density");
-    if (iterations < 1 || !ms) return fail(CVT_ERR_ARG, "iterations %d", iterations);
+    if (iterations < 1 || iterations > 1000 || !ms)
+        return fail(CVT_ERR_ARG, "iterations %d (1..1000)", iterations);
     const int np = jfa_num_passes(c->W, c->H);
-    const int nev = np + 5;
-    std::vector<hipEvent_t> ev(nev);
+    // four events per iteration bracket the phases; nothing in between, so
+    // the passes run back to back as in a graph replay
+    std::vector<hipEvent_t> ev(4 * (size_t)iterations);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e), "event");
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int rc = CVT_OK;
     for (int it = 0; it < iterations && !rc; ++it) {
         if (!c->seeds_valid) c->enqueue_seed();
-        c->prof_events = ev.data();
-        hipEventRecord(ev[0], c->stream);
-        rc = c->enqueue_jfa();                      // records ev[1..np]
-        if (!rc) rc = launch_moments_timed(c, ev[np + 1], ev[np + 2]);
+        hipEventRecord(ev[4 * it + 0], c->stream);
+        rc = c->enqueue_jfa();
+        hipEventRecord(ev[4 * it + 1], c->stream);
+        if (!rc) rc = c->enqueue_moments();
+        hipEventRecord(ev[4 * it + 2], c->stream);
         if (!rc) rc = c->enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr);
-        hipEventRecord(ev[np + 3], c->stream);
-        c->prof_events = nullptr;
-        if (hipEventSynchronize(ev[np + 3]) != hipSuccess) { rc = fail(CVT_ERR_HIP, "profile"); break; }
+        hipEventRecord(ev[4 * it + 3], c->stream);
+    }
+    double acc[4] = {0, 0, 0, 0};
+    if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(CVT_ERR_HIP, "profile");
+    for (int it = 0; it < iterations && !rc; ++it) {
         float t;
-        for (int k = 0; k < np; ++k) { hipEventElapsedTime(&t, ev[k], ev[k + 1]); acc[0] += t; }
-        hipEventElapsedTime(&t, ev[np], ev[np + 1]); acc[1] += t;      // moments
-        hipEventElapsedTime(&t, ev[np + 1], ev[np + 2]); acc[2] += t;  // overflow
-        hipEventElapsedTime(&t, ev[np + 2], ev[np + 3]); acc[3] += t;  // update + finalize
-        hipEventElapsedTime(&t, ev[0], ev[np + 3]); acc[5] += t;       // whole iteration
+        for (int k = 0; k < 3; ++k) {
+            hipEventElapsedTime(&t, ev[4 * it + k], ev[4 * it + k + 1]);
+            acc[k] += t;
+        }
+        hipEventElapsedTime(&t, ev[4 * it], ev[4 * it + 3]);
+        acc[3] += t;
     }
     for (auto &e : ev) hipEventDestroy(e);
     if (rc) return rc;
     c->labels_fresh = true;
-    ms[0] = acc[0] / (iterations * np);
+    ms[0] = acc[0] / ((double)iterations * np);
     ms[1] = acc[1] / iterations;
-    ms[2] = acc[2] / iterations;
-    ms[3] = acc[3] / iterations;
+    ms[2] = 0.0;
+    ms[3] = acc[2] / iterations;
     ms[4] = 0.0;
-    ms[5] = acc[5] / iterations;
+    ms[5] = acc[3] / iterations;
     ms[6] = np;
     ms[7] = c->tile;
     return c->check_flags();

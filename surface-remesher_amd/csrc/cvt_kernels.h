@@ -97,13 +97,6 @@ static __global__ void k_export_labels(const uint32_t *__restrict__ lab, int32_t
 // moments (reference replacement for pbaCVDComputeCentroid + the energy
 // texture, source/gcvt.cu:562-732, :788-802)
 // ---------------------------------------------------------------------------
-template <int T> struct TileCfg {
-    static constexpr int SEG = T < 64 ? T : 64;       // pixels per row segment
-    static constexpr int RPW = 64 / SEG;              // rows per wave step
-    static constexpr int SPR = T / SEG;               // segments per row
-    static constexpr int GROUPS = T * T / 64;         // wave steps per tile
-};
-
 #define CVT_HASH_CAP 1024
 
 __device__ __forceinline__ long long cvt_fix(double v, int f)
@@ -147,6 +140,280 @@ __device__ __forceinline__ float seg_add(float v, int t, int src, int start)
     return (src >= start) ? v + fbits(t) : v;
 }
 
+// ---------------------------------------------------------------------------
+// Tile moments.  A lane owns 4 consecutive pixels of a row segment of SEG
+// pixels (SEG = 64 for every tile of side >= 64, so all those tiles cut runs
+// at the same places and give bit-identical sums; SEG = 32 for 32-pixel
+// tiles).  A segment is SEG/4 lanes inside one 16-lane DPP row.  Runs of one
+// key (label with density) are summed relative to the run's first pixel
+// (u = pixel - run start) in fp32: in-lane partial sums, then a segmented
+// scan of the lanes' carries (4 DPP steps).  Each run end becomes a record in
+// the wave's LDS queue (ballot + mbcnt compaction); the wave then converts
+// the records to site-relative fp64 sums and adds them, as int64 fixed point,
+// to the tile's LDS hash with integer atomics (exact, order-free).
+// ---------------------------------------------------------------------------
+struct TailRec {
+    int key;
+    float a, b, c;
+    int xy;          // run start x | y << 16
+};
+#define CVT_TAILQ 256   // records per wave and step (4 pixels x 64 lanes)
+
+// LDS of one tile: a hash of HC labels (a power of two <= CVT_HASH_CAP) with
+// their fixed-point sums, and the waves' run-end queues
+template <int HC> struct TileLds {
+    int keys[HC];
+    unsigned long long acc[HC][4];
+    TailRec q[4][CVT_TAILQ];
+    int full;
+};
+
+// hash capacity of a T x T tile kernel: enough for one label per 16 pixels
+// at T = 64 and 32 pixels at T = 128 (denser sites take a smaller tile)
+template <int T> struct TileHash {
+    static constexpr int HC = T == 64 ? 256 : T == 128 ? 512 : CVT_HASH_CAP;
+};
+
+template <int HC>
+__device__ __forceinline__ void tile_lds_init(TileLds<HC> &L)
+{
+    for (int j = threadIdx.x; j < HC; j += blockDim.x) {
+        L.keys[j] = -1;
+        L.acc[j][0] = L.acc[j][1] = L.acc[j][2] = L.acc[j][3] = 0ull;
+    }
+    if (threadIdx.x == 0) L.full = 0;
+    __syncthreads();
+}
+
+__device__ __forceinline__ unsigned lane_rank(unsigned long long m)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Adds the runs of the tile [tx0, tx0 + T) x [ty0, ty0 + T) to the hash.
+// lab: labels (byte offsets); a label's site index is (lab >> 3) - key_base,
+// valid when below S; sites: the S sites of this map.
+template <int SEG, int HC>
+__device__ __forceinline__ void tile_moments(TileLds<HC> &L, const uint32_t *__restrict__ lab,
+                                             const float *__restrict__ rho,
+                                             const float2 *__restrict__ sites, int key_base,
+                                             int S, int W, int H, float rho_scale, int tx0,
+                                             int ty0, int T)
+{
+    constexpr int SL = SEG / 4;                    // lanes per segment
+    constexpr int SPW = 64 / SL;                   // segments per wave step
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int seg_lane = lane % SL;
+    const int spr = T / SEG;                       // segments per tile row
+    const int steps = T * spr / SPW;               // wave steps per tile
+    const uint32_t map_bytes = 4u * (uint32_t)W * (uint32_t)H;
+    const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t *>(lab), (short)0, (int)map_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(rho), (short)0, (int)map_bytes, 0x00020000);
+    TailRec *q = L.q[wave];
+
+    for (int g = wave; g < steps; g += 4) {
+        const int seg = g * SPW + lane / SL;
+        const int r = seg / spr;
+        const int x0 = tx0 + (seg - r * spr) * SEG + 4 * seg_lane;
+        const int y = ty0 + r;
+        int key[4];
+        float rs[4];
+        {
+            const bool rowok = y < H;
+            const uint32_t off = 4u * ((uint32_t)min(y, H - 1) * (uint32_t)W + (uint32_t)x0);
+            uint32_t lv[4], rv[4];
+            if ((W & 3) == 0) {
+                typedef unsigned u4 __attribute__((ext_vector_type(4)));
+                const u4 a4 = __builtin_amdgcn_raw_buffer_load_b128(rl, off, 0, 0);
+                const u4 b4 = __builtin_amdgcn_raw_buffer_load_b128(rr, off, 0, 0);
+                lv[0] = a4.x; lv[1] = a4.y; lv[2] = a4.z; lv[3] = a4.w;
+                rv[0] = b4.x; rv[1] = b4.y; rv[2] = b4.z; rv[3] = b4.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    lv[j] = __builtin_amdgcn_raw_buffer_load_b32(rl, off + 4u * j, 0, 0);
+                    rv[j] = __builtin_amdgcn_raw_buffer_load_b32(rr, off + 4u * j, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int l = (int)(lv[j] >> 3) - key_base;
+                const float d = __uint_as_float(rv[j]);
+                const bool ok = rowok && (x0 + j < W) && (unsigned)l < (unsigned)S && d != 0.0f;
+                key[j] = ok ? l : -1;
+                rs[j] = ok ? d * rho_scale : 0.0f;
+            }
+        }
+        // run heads and run starts (positions 4 * lane + j within the wave)
+        const int kprev = dpp_wave_shr1(key[3], -2);
+        bool hd[4];
+        hd[0] = (seg_lane == 0) || (kprev != key[0]);
+        hd[1] = key[1] != key[0];
+        hd[2] = key[2] != key[1];
+        hd[3] = key[3] != key[2];
+        const int p0 = 4 * lane;
+        int lasth = hd[3] ? p0 + 3 : hd[2] ? p0 + 2 : hd[1] ? p0 + 1 : hd[0] ? p0 : -1;
+        // inclusive max over the lanes of the segment (a segment's first lane
+        // always has a head, and positions grow with the lane)
+        lasth = max(lasth, dpp_row_shr<1>(lasth));
+        lasth = max(lasth, dpp_row_shr<2>(lasth));
+        if (SL > 4) lasth = max(lasth, dpp_row_shr<4>(lasth));
+        if (SL > 8) lasth = max(lasth, dpp_row_shr<8>(lasth));
+        const int prevh = dpp_row_shr<1>(lasth);          // start carried into pixel 0
+        int st[4];
+        st[0] = hd[0] ? p0 : prevh;
+        st[1] = hd[1] ? p0 + 1 : st[0];
+        st[2] = hd[2] ? p0 + 2 : st[1];
+        st[3] = hd[3] ? p0 + 3 : st[2];
+        // in-lane running sums of the current piece, reset at heads
+        float pa[4], pb[4], pc[4];
+        {
+            float ra = 0.0f, rb = 0.0f, rc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float u = (float)(p0 + j - st[j]);
+                if (hd[j]) { ra = 0.0f; rb = 0.0f; rc = 0.0f; }
+                ra += rs[j];
+                rb += rs[j] * u;
+                rc += rs[j] * (u * u);
+                pa[j] = ra; pb[j] = rb; pc[j] = rc;
+            }
+        }
+        // segmented inclusive scan of the lanes' carries (their last piece);
+        // a lane's chain starts at the lane of its last run head
+        const int hlane = lasth >> 2;
+        float ca = pa[3], cb = pb[3], cc = pc[3];
+#define CVT_CARRY_STEP(K)                                                                \
+        {                                                                                \
+            const int ta = dpp_row_shr<K>(ibits(ca)), tb = dpp_row_shr<K>(ibits(cb)),     \
+                      tc = dpp_row_shr<K>(ibits(cc));                                    \
+            const int src = ((lane & 15) >= K) ? lane - K : -1;                           \
+            ca = seg_add(ca, ta, src, hlane);                                            \
+            cb = seg_add(cb, tb, src, hlane);                                            \
+            cc = seg_add(cc, tc, src, hlane);                                            \
+        }
+        CVT_CARRY_STEP(1)
+        CVT_CARRY_STEP(2)
+        if (SL > 4) CVT_CARRY_STEP(4)
+        if (SL > 8) CVT_CARRY_STEP(8)
+#undef CVT_CARRY_STEP
+        // carry into this lane's first piece (only when pixel 0 continues a run)
+        const float ia = fbits(dpp_row_shr<1>(ibits(ca)));
+        const float ib = fbits(dpp_row_shr<1>(ibits(cb)));
+        const float ic = fbits(dpp_row_shr<1>(ibits(cc)));
+        // run ends
+        const int knext = dpp_wave_shl1(key[0], -2);
+        bool tl[4];
+        tl[0] = key[1] != key[0];
+        tl[1] = key[2] != key[1];
+        tl[2] = key[3] != key[2];
+        tl[3] = (seg_lane == SL - 1) || (knext != key[3]);
+        // queue the run ends: records ordered by (j, lane)
+        int base = 0;
+        bool first_piece = true;                 // no head yet in this lane at <= j
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (hd[j]) first_piece = false;
+            const bool m = tl[j] && key[j] >= 0;
+            const unsigned long long bal = __ballot(m);
+            if (m) {
+                const int idx = base + (int)lane_rank(bal);
+                TailRec t;
+                t.key = key[j];
+                const bool carry = first_piece;  // the run began in an earlier lane
+                t.a = carry ? pa[j] + ia : pa[j];
+                t.b = carry ? pb[j] + ib : pb[j];
+                t.c = carry ? pc[j] + ic : pc[j];
+                const int xs = x0 + (st[j] - p0);
+                t.xy = xs | (y << 16);
+                q[idx] = t;
+            }
+            base += __popcll(bal);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // one lane per record: site-relative fp64 sums into the hash
+        for (int e = lane; e < base; e += 64) {
+            const TailRec t = q[e];
+            const float2 sv = sites[t.key];
+            const double A = t.a, B = t.b, Cc = t.c;
+            const double ox = ((double)(t.xy & 0xffff) + 0.5) - (double)sv.x;
+            const double oy = ((double)(t.xy >> 16) + 0.5) - (double)sv.y;
+            const double DX = B + A * ox;
+            const double DY = A * oy;
+            const double E = Cc + 2.0 * ox * B + A * (ox * ox + oy * oy);
+            const unsigned hsh = ((unsigned)t.key * 2654435761u) >> (32 - __builtin_ctz(HC));
+            int slot = -1;
+            for (int probe = 0; probe < HC; ++probe) {
+                const int h = (hsh + probe) & (HC - 1);
+                const int old = atomicCAS(&L.keys[h], -1, t.key);
+                if (old == -1 || old == t.key) { slot = h; break; }
+            }
+            if (slot >= 0) {
+                atomicAdd(&L.acc[slot][0], (unsigned long long)cvt_fix(A, MOM_FW));
+                atomicAdd(&L.acc[slot][1], (unsigned long long)cvt_fix(DX, MOM_FX));
+                atomicAdd(&L.acc[slot][2], (unsigned long long)cvt_fix(DY, MOM_FX));
+                atomicAdd(&L.acc[slot][3], (unsigned long long)cvt_fix(E, MOM_FE));
+            } else {
+                L.full = 1;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// Writes the tile's hash entries: a site whose home tile (the tile holding
+// its position) is within one tile of this one gets a unique-writer slot of
+// its 3 x 3 window, tagged with the epoch; otherwise a record is chained to
+// the site (integer atomics only to allocate records).  site_off: index of
+// this map's first site in slots / ovf_head / sites.
+template <int HC>
+__device__ __forceinline__ void tile_flush(TileLds<HC> &L, const float2 *__restrict__ sites,
+                                           int site_off, int T, int tX, int tY, int tilesX,
+                                           int tilesY, int epoch, Slot *__restrict__ slots,
+                                           OvfRec *__restrict__ ovf, int ovf_cap,
+                                           int *__restrict__ ovf_head, int *__restrict__ dstate,
+                                           int *__restrict__ dflags)
+{
+    __syncthreads();
+    for (int j = threadIdx.x; j < HC; j += blockDim.x) {
+        const int l = L.keys[j];
+        if (l < 0) continue;
+        const int gl = site_off + l;
+        const float2 s = sites[gl];
+        const long long wt = (long long)L.acc[j][0], dx = (long long)L.acc[j][1];
+        const long long dy = (long long)L.acc[j][2], ee = (long long)L.acc[j][3];
+        const int hx = min(max((int)floorf(s.x / (float)T), 0), tilesX - 1);
+        const int hy = min(max((int)floorf(s.y / (float)T), 0), tilesY - 1);
+        const int rx = tX - hx, ry = tY - hy;
+        if (rx >= -1 && rx <= 1 && ry >= -1 && ry <= 1) {
+            Slot sl;
+            sl.w = wt; sl.dx = dx; sl.dy = dy; sl.e = ee; sl.epoch = epoch; sl.pad = 0;
+            slots[(long)gl * 9 + (ry + 1) * 3 + (rx + 1)] = sl;
+        } else {
+            const int k = atomicAdd(&dstate[ST_OVFN], 1);
+            if (k < ovf_cap) {
+                OvfRec o;
+                o.w = wt; o.dx = dx; o.dy = dy; o.e = ee;
+                o.next = atomicExch(&ovf_head[gl], k);
+                o.pad = 0;
+                ovf[k] = o;
+            } else {
+                L.full = 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && L.full) dflags[FLAG_FULL] = 1;
+}
+
+// moments (reference replacement for pbaCVDComputeCentroid + the energy
+// texture, source/gcvt.cu:562-732, :788-802): one workgroup per T x T tile
 template <int T>
 static __global__ __launch_bounds__(256) void k_moments(
     const uint32_t *__restrict__ lab, const float *__restrict__ rho,
@@ -154,130 +421,13 @@ static __global__ __launch_bounds__(256) void k_moments(
     int tilesY, Slot *__restrict__ slots, OvfRec *__restrict__ ovf, int ovf_cap,
     int *__restrict__ ovf_head, int *__restrict__ dflags, int *__restrict__ dstate)
 {
-    using C = TileCfg<T>;
-    __shared__ int keys[CVT_HASH_CAP];
-    __shared__ unsigned long long acc[CVT_HASH_CAP][4];
-    __shared__ int full;
-
+    __shared__ TileLds<TileHash<T>::HC> L;
     const int tile = blockIdx.x;
     const int tX = tile % tilesX, tY = tile / tilesX;
-    const int tx0 = tX * T, ty0 = tY * T;
-    for (int j = threadIdx.x; j < CVT_HASH_CAP; j += 256) {
-        keys[j] = -1;
-        acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0ull;
-    }
-    if (threadIdx.x == 0) full = 0;
-    __syncthreads();
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int seg_lane = lane % C::SEG;
-    const int row16 = lane & 15;
-
-    for (int g = wave; g < C::GROUPS; g += 4) {
-        const int r = (g / C::SPR) * C::RPW + lane / C::SEG;
-        const int c0 = (g % C::SPR) * C::SEG;
-        const int x = tx0 + c0 + seg_lane, y = ty0 + r;
-        int key = -1;
-        float rs = 0.0f;
-        if (x < W && y < H) {
-            const long p = (long)y * W + x;
-            const int l = (int)(lab[p] >> 3);
-            const float rv = rho[p];
-            if ((unsigned)l < (unsigned)S && rv != 0.0f) {
-                key = l;
-                rs = rv * rho_scale;
-            }
-        }
-        // runs of equal key inside the segment; start = first lane of the run
-        const int kprev = dpp_wave_shr1(key, -2);
-        const bool head = (seg_lane == 0) || (kprev != key);
-        int start = head ? lane : 0;
-        start = max(start, dpp_row_shr<1>(start));
-        start = max(start, dpp_row_shr<2>(start));
-        start = max(start, dpp_row_shr<4>(start));
-        start = max(start, dpp_row_shr<8>(start));
-        start = max(start, dpp_bcast15(start));
-        start = max(start, dpp_bcast31(start));
-        // per pixel, relative to the run's first pixel: u = lane - start
-        const float u = (float)(lane - start);
-        float a = rs, b = rs * u, cc = rs * (u * u);
-        // segmented inclusive scan of (a, b, cc)
-#define CVT_SCAN_STEP(MOVE, SRC)                                                   \
-        {                                                                          \
-            const int ta = MOVE(ibits(a)), tb = MOVE(ibits(b)), tc = MOVE(ibits(cc)); \
-            a = seg_add(a, ta, SRC, start);                                        \
-            b = seg_add(b, tb, SRC, start);                                        \
-            cc = seg_add(cc, tc, SRC, start);                                      \
-        }
-        CVT_SCAN_STEP(dpp_row_shr<1>, row16 >= 1 ? lane - 1 : -1)
-        CVT_SCAN_STEP(dpp_row_shr<2>, row16 >= 2 ? lane - 2 : -1)
-        CVT_SCAN_STEP(dpp_row_shr<4>, row16 >= 4 ? lane - 4 : -1)
-        CVT_SCAN_STEP(dpp_row_shr<8>, row16 >= 8 ? lane - 8 : -1)
-        CVT_SCAN_STEP(dpp_bcast15, ((lane >> 4) & 1) ? (lane & ~15) - 1 : -1)
-        CVT_SCAN_STEP(dpp_bcast31, (lane >= 32) ? 31 : -1)
-#undef CVT_SCAN_STEP
-        const int knext = dpp_wave_shl1(key, -2);
-        const bool tail = (seg_lane == C::SEG - 1) || (knext != key);
-        if (tail && key >= 0) {
-            // run -> site-relative sums in float64 (one site read per run)
-            const float2 st = sites[key];
-            const double A = a, B = b, Cc = cc;
-            const double ox = ((double)(x - (lane - start)) + 0.5) - (double)st.x;
-            const double oy = ((double)y + 0.5) - (double)st.y;
-            const double W_ = A;
-            const double DX = B + A * ox;
-            const double DY = A * oy;
-            const double E = Cc + 2.0 * ox * B + A * (ox * ox + oy * oy);
-            const unsigned hsh = ((unsigned)key * 2654435761u) >> (32 - 10);
-            int slot = -1;
-            for (int probe = 0; probe < CVT_HASH_CAP; ++probe) {
-                const int h = (hsh + probe) & (CVT_HASH_CAP - 1);
-                const int old = atomicCAS(&keys[h], -1, key);
-                if (old == -1 || old == key) { slot = h; break; }
-            }
-            if (slot >= 0) {
-                atomicAdd(&acc[slot][0], (unsigned long long)cvt_fix(W_, MOM_FW));
-                atomicAdd(&acc[slot][1], (unsigned long long)cvt_fix(DX, MOM_FX));
-                atomicAdd(&acc[slot][2], (unsigned long long)cvt_fix(DY, MOM_FX));
-                atomicAdd(&acc[slot][3], (unsigned long long)cvt_fix(E, MOM_FE));
-            } else {
-                full = 1;
-            }
-        }
-    }
-    __syncthreads();
-
-    const int epoch = dstate[ST_EPOCH];
-    for (int j = threadIdx.x; j < CVT_HASH_CAP; j += 256) {
-        const int l = keys[j];
-        if (l < 0) continue;
-        const float2 s = sites[l];
-        const long long wt = (long long)acc[j][0], dx = (long long)acc[j][1];
-        const long long dy = (long long)acc[j][2], ee = (long long)acc[j][3];
-        const int hx = min(max((int)floorf(s.x / (float)T), 0), tilesX - 1);
-        const int hy = min(max((int)floorf(s.y / (float)T), 0), tilesY - 1);
-        const int rx = tX - hx, ry = tY - hy;
-        if (rx >= -1 && rx <= 1 && ry >= -1 && ry <= 1) {
-            Slot sl;
-            sl.w = wt; sl.dx = dx; sl.dy = dy; sl.e = ee; sl.epoch = epoch; sl.pad = 0;
-            slots[(long)l * 9 + (ry + 1) * 3 + (rx + 1)] = sl;
-        } else {
-            // a cell reaching beyond its home tile's 3x3 window: chain a record
-            // to the site (integer sums: the chain order cannot change them)
-            const int k = atomicAdd(&dstate[ST_OVFN], 1);
-            if (k < ovf_cap) {
-                OvfRec o;
-                o.w = wt; o.dx = dx; o.dy = dy; o.e = ee;
-                o.next = atomicExch(&ovf_head[l], k);
-                o.pad = 0;
-                ovf[k] = o;
-            } else {
-                full = 1;
-            }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && full) dflags[FLAG_FULL] = 1;
+    tile_lds_init(L);
+    tile_moments<(T < 64 ? T : 64)>(L, lab, rho, sites, 0, S, W, H, rho_scale, tX * T, tY * T, T);
+    tile_flush(L, sites, 0, T, tX, tY, tilesX, tilesY, dstate[ST_EPOCH], slots, ovf, ovf_cap,
+               ovf_head, dstate, dflags);
 }
 
 // exact 128-bit fixed point -> double (value * 2^-f)
@@ -331,9 +481,11 @@ static __global__ __launch_bounds__(256) void k_update(
     const Slot *__restrict__ slots, const OvfRec *__restrict__ ovf, int *__restrict__ ovf_head,
     const int *__restrict__ dflags, int *__restrict__ dstate, float omega_arg,
     const float *__restrict__ domega, int mode, double rho_unscale, double *__restrict__ mom_out,
-    double *__restrict__ blockpart, uint32_t *__restrict__ seed, int W, int H, CompatArgs ca)
+    double *__restrict__ blockpart, uint32_t *__restrict__ seed, int W, int H, CompatArgs ca,
+    FinalArgs fa)
 {
     __shared__ double red_e[256], red_g[256];
+    __shared__ int last;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int live = (mode & UPD_DEV_S) ? dstate[ST_NSITES] : S;
     const float omega = domega ? *domega : omega_arg;
@@ -389,12 +541,49 @@ static __global__ __launch_bounds__(256) void k_update(
         }
         __syncthreads();
     }
+    // every chain was consumed above: the record pool restarts empty
+    if (blockIdx.x == 0 && threadIdx.x == 0) dstate[ST_OVFN] = 0;
+    // publish this block's partials; the last block to finish adds all of
+    // them in block order (the release / acquire pair at agent scope makes
+    // every block's partials visible to it, whichever XCD wrote them)
     if (threadIdx.x == 0) {
         blockpart[2 * blockIdx.x] = red_e[0];
         blockpart[2 * blockIdx.x + 1] = red_g[0];
+        const int done = __hip_atomic_fetch_add(&dstate[ST_DONE], 1, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        last = (done == (int)gridDim.x - 1);
     }
-    // every chain was consumed above: the record pool restarts empty
-    if (blockIdx.x == 0 && threadIdx.x == 0) dstate[ST_OVFN] = 0;
+    __syncthreads();
+    if (!last) return;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    double te = 0.0, tg = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) {
+        te += __hip_atomic_load(&blockpart[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tg += __hip_atomic_load(&blockpart[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    red_e[threadIdx.x] = te;
+    red_g[threadIdx.x] = tg;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red_e[threadIdx.x] += red_e[threadIdx.x + o];
+            red_g[threadIdx.x] += red_g[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double E = red_e[0] * fa.e_scale;
+        const double G = sqrt(red_g[0]) * fa.g_scale;
+        if (fa.record) {
+            const int h = dstate[ST_HIST];
+            if (h < fa.hist_cap) { fa.hist[2 * h] = E; fa.hist[2 * h + 1] = G; }
+            dstate[ST_HIST] = h + 1;
+        }
+        if (fa.out_eg) { fa.out_eg[0] = E; fa.out_eg[1] = G; }
+        dstate[ST_EPOCH] = dstate[ST_EPOCH] + 1;
+        dstate[ST_DONE] = 0;
+    }
 }
 
 // Integer-site merge (one workgroup): sites whose target pixel they own

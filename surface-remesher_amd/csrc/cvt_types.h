@@ -32,7 +32,18 @@ struct __align__(8) OvfRec {
 // device flags
 enum { FLAG_FULL = 1, FLAG_BAD = 2, NUM_FLAGS = 4 };
 // device state words
-enum { ST_EPOCH = 0, ST_HIST = 1, ST_OVFN = 2, ST_NSITES = 3, NUM_STATE = 4 };
+// (ST_DONE counts the finished blocks of a site-update launch; the last block
+// to finish resets it)
+enum { ST_EPOCH = 0, ST_HIST = 1, ST_OVFN = 2, ST_NSITES = 3, ST_DONE = 4, NUM_STATE = 5 };
+
+// what the last block of a site update does with the energy partials
+struct FinalArgs {
+    double *hist;          // history (E, |grad E|) [hist_cap][2]
+    int hist_cap;
+    int record;            // append to the history
+    double e_scale, g_scale;
+    double *out_eg;        // optional (E, |grad E|) copy
+};
 
 
 struct CompatArgs {

@@ -7,11 +7,12 @@
 namespace cvt {
 
 // Launch shape of a JFA pass for one grid: the quad kernel (4 pixels per
-// lane, persistent row bands) when W % 4 == 0, with the site table in LDS
-// when it fits; otherwise one pixel per lane.
+// lane, persistent row bands) when W % 4 == 0, with the site table in a
+// static LDS array of lds_entries float2 (16, 40 or 160 KB) when it fits;
+// otherwise one pixel per lane.
 struct JfaPlan {
     bool quad = false, lds = false;
-    int grid = 0, threads = 0, table_f4 = 0;
+    int grid = 0, threads = 0, table_f4 = 0, lds_entries = 0;
     size_t lds_bytes = 0;
 };
 JfaPlan plan_jfa(int W, int H, int capS);
@@ -44,9 +45,11 @@ struct BatchUnit {
 
 // Pass `pass` (0 .. pmax-1) of the batched JFA over all units; lds_bytes = 0
 // gathers sites from global memory, otherwise every patch's table (at most
-// lds_bytes) is staged in LDS.
+// lds_bytes) is staged in LDS.  One launch per step class in `modes` (bit 0:
+// step >= 4, bit 1: step 1, bit 2: step 2); a unit runs in the launch of its
+// patch's class.
 void launch_jfa_batch(hipStream_t st, const BatchPatch *patches, const BatchUnit *units, int nunits,
                       uint32_t *seed, uint32_t *lab0, uint32_t *lab1, const float2 *sites, int pass,
-                      int pmax, size_t lds_bytes);
+                      int pmax, size_t lds_bytes, unsigned modes);
 
 }  // namespace cvt

@@ -123,158 +123,315 @@ __global__ __launch_bounds__(256) void k_jfa_pass(const uint32_t *__restrict__ i
 }
 
 // ---------------------------------------------------------------------------
-// Quad pass: each lane owns 4 consecutive pixels (one dwordx4 per tap row and
-// column), workgroups are persistent and own a band of rows.  With
-// SITES_IN_LDS the whole site table is first copied to LDS (<= 160 KB) and
-// every candidate lookup is an LDS read at the label's byte offset; otherwise
-// the lookups gather from the global table.
+// Quad pass.  Each lane owns 4 consecutive pixels of a row, a wave takes one
+// 256-pixel row segment per item, and persistent workgroups own bands of
+// rows.  The code is specialised on the step:
+//   STEP_ANY (>= 4): nine tap quads; every (pixel, tap) pair is distinct.
+//   STEP_1 / STEP_2: the three tap columns of a row overlap, so a tap row
+//                    holds 6 (8) distinct positions; their site lookups and
+//                    dy^2 are shared by the 4 pixels.
+// The next item's label loads are issued before the current item is
+// evaluated (two register sets alternate), so each wave keeps one item's
+// loads in flight while it computes.
 // ---------------------------------------------------------------------------
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(3))) char lds_cchar;
+typedef const __attribute__((address_space(3))) f32x2 lds_cf32x2;
+typedef __attribute__((address_space(3))) f32x4 lds_f32x4;
+
+enum { STEP_ANY = 0, STEP_1 = 1, STEP_2 = 2 };
+enum { TAB_LDS0 = 0, TAB_LDSB = 1, TAB_GLOBAL = 2 };
+
+// Site lookup by label (a byte offset into the site table).
+//   TAB_LDS0:   the table is staged at LDS byte 0, so the label is the LDS
+//               address and a lookup is one ds_read_b64;
+//   TAB_LDSB:   one patch's table is staged in LDS; `base` is the label of
+//               its first entry;
+//   TAB_GLOBAL: a buffer load from the global table.
+template <int TAB> struct SiteTab {
+    lds_cchar *lds;
+    __amdgpu_buffer_rsrc_t g;
+    uint32_t base;
+    const f32x4 *src;     // staging source (LDS kinds)
+    int n4;               // float4 count to stage
+
+    __device__ __forceinline__ float2 operator()(uint32_t l) const
+    {
+        if constexpr (TAB == TAB_LDS0) {
+            const f32x2 v = *(lds_cf32x2 *)(lds + l);
+            return make_float2(v.x, v.y);
+        } else if constexpr (TAB == TAB_LDSB) {
+            const f32x2 v = *(lds_cf32x2 *)(lds + (l - base));
+            return make_float2(v.x, v.y);
+        } else {
+            const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(g, l, 0, 0);
+            return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+        }
+    }
+    // copy the table into LDS (all threads of the block take part)
+    __device__ __forceinline__ void stage() const
+    {
+        if constexpr (TAB != TAB_GLOBAL) {
+            lds_f32x4 *dst = (lds_f32x4 *)lds;
+            for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
+            __syncthreads();
+        }
+    }
+};
+
+// label loads of one item
+template <int MODE> struct Item;
+template <> struct Item<STEP_ANY> { u32x4 q[9]; };
+template <> struct Item<STEP_1> { u32x4 c[3]; uint32_t l[3], r[3]; };
+template <> struct Item<STEP_2> { u32x4 c[3]; u32x2 l[3], r[3]; };
+
+// per-pass constants of a map
+struct MapGeo {
+    __amdgpu_buffer_rsrc_t rin, rout, rclr;
+    uint32_t row_bytes;
+    int W, H, step;
+    bool clear;
+    uint32_t none;
+};
+
+__device__ __forceinline__ uint32_t row_off(const MapGeo &m, int y)
+{
+    return __builtin_amdgcn_readfirstlane((uint32_t)y * m.row_bytes);
+}
+
+template <int MODE>
+__device__ __forceinline__ void load_item(Item<MODE> &it, const MapGeo &m, int y, int seg, int lane4)
+{
+    const int xc = min(seg * 256 + lane4, m.W - 4);
+    const uint32_t o1 = 4u * (uint32_t)xc;
+    const int st = m.step;
+    const uint32_t s0 = row_off(m, (y - st >= 0) ? y - st : y);
+    const uint32_t s1 = row_off(m, y);
+    const uint32_t s2 = row_off(m, (y + st < m.H) ? y + st : y);
+    if constexpr (MODE == STEP_ANY) {
+        // a tap column outside the grid is redirected to the centre column
+        // (for step >= 4 the whole quad is in or out together)
+        const uint32_t o0 = (xc - st >= 0) ? o1 - 4u * (uint32_t)st : o1;
+        const uint32_t o2 = (xc + 3 + st < m.W) ? o1 + 4u * (uint32_t)st : o1;
+        it.q[0] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o0, s0, 0);
+        it.q[1] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, s0, 0);
+        it.q[2] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o2, s0, 0);
+        it.q[3] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o0, s1, 0);
+        it.q[4] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, s1, 0);
+        it.q[5] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o2, s1, 0);
+        it.q[6] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o0, s2, 0);
+        it.q[7] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, s2, 0);
+        it.q[8] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o2, s2, 0);
+    } else {
+        // side words: the one (two) labels left of the quad and right of it;
+        // unused (and replaced by the centre labels) at the grid's edges
+        const uint32_t oL = (xc > 0) ? o1 - 4u * MODE : o1;
+        const uint32_t oR = (xc + 4 < m.W) ? o1 + 16u : o1;
+        const uint32_t so[3] = {s0, s1, s2};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            it.c[r] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, so[r], 0);
+            if constexpr (MODE == STEP_1) {
+                it.l[r] = __builtin_amdgcn_raw_buffer_load_b32(m.rin, oL, so[r], 0);
+                it.r[r] = __builtin_amdgcn_raw_buffer_load_b32(m.rin, oR, so[r], 0);
+            } else {
+                it.l[r] = __builtin_amdgcn_raw_buffer_load_b64(m.rin, oL, so[r], 0);
+                it.r[r] = __builtin_amdgcn_raw_buffer_load_b64(m.rin, oR, so[r], 0);
+            }
+        }
+    }
+}
 
 __device__ __forceinline__ uint32_t q_el(const u32x4 &q, int j)
 {
     return j == 0 ? q.x : j == 1 ? q.y : j == 2 ? q.z : q.w;
 }
 
-// labels of the left (or right) tap of pixels 0..3 of a quad at step < 4:
-// elements of the neighbouring quad nb and the centre quad c, i.e. the labels
-// at x + j - s (left) or x + j + s (right); an out-of-grid neighbour quad
-// yields the pixel's own centre label (a duplicate candidate)
-__device__ __forceinline__ uint32_t small_left(const u32x4 &nb, const u32x4 &c, int j, int s,
-                                               bool nb_ok)
+__device__ __forceinline__ double cand(float2 s, uint32_t l, float px, float py)
 {
-    const int k = j - s;   // -3 .. 2
-    if (k >= 0) return q_el(c, k);
-    return nb_ok ? q_el(nb, 4 + k) : q_el(c, j);
-}
-__device__ __forceinline__ uint32_t small_right(const u32x4 &nb, const u32x4 &c, int j, int s,
-                                                bool nb_ok)
-{
-    const int k = j + s;   // 1 .. 6
-    if (k < 4) return q_el(c, k);
-    return nb_ok ? q_el(nb, k - 4) : q_el(c, j);
+    const float dx = s.x - px;
+    const float dy = s.y - py;
+    return jfa_key(__builtin_fmaf(dx, dx, dy * dy), l);
 }
 
-// Rows [y0, y1) of one W x H map, all waves of the block.  tab is the site
-// table (LDS or global) and tab_base the byte offset its first entry has in
-// label space (labels are byte offsets into the full table).
-__device__ __forceinline__ void jfa_quad_rows(const uint32_t *__restrict__ in,
-                                              uint32_t *__restrict__ out,
-                                              uint32_t *__restrict__ clear, const char *tab,
-                                              uint32_t tab_base, int W, int H, int y0, int y1,
-                                              int step, uint32_t none)
+template <int MODE, int TAB>
+__device__ __forceinline__ void eval_item(const Item<MODE> &it, const MapGeo &m,
+                                          const SiteTab<TAB> &tab, int y, int seg, int lane4)
 {
-    tab -= tab_base;
-    const int nseg = (W + 255) >> 8;              // 256-pixel segments per row
-    const int nitems = (y1 - y0) * nseg;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nwaves = blockDim.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const uint32_t map_bytes = 4u * (uint32_t)W * (uint32_t)H;
-    const __amdgpu_buffer_rsrc_t rin = buf_rsrc(in, map_bytes);
-    const __amdgpu_buffer_rsrc_t rout = buf_rsrc(out, map_bytes);
-    const uint32_t row_bytes = 4u * (uint32_t)W;
-    const bool small = step < 4;
-    const int cstep = small ? 4 : step;           // column offset of the side quads
-
-    for (int it = wave; it < nitems; it += nwaves) {
-        const int row = it / nseg;
-        const int y = __builtin_amdgcn_readfirstlane(y0 + row);
-        const int seg = __builtin_amdgcn_readfirstlane(it - row * nseg);
-        const int xb = seg * 256 + lane * 4;
-        const bool live = xb < W;
-        const int xc = min(xb, W - 4);
-        const uint32_t o1 = 4u * (uint32_t)xc;
-        const bool lok = xc - cstep >= 0, rok = xc + 3 + cstep < W;
-        const uint32_t o0 = lok ? o1 - 4u * (uint32_t)cstep : o1;
-        const uint32_t o2 = rok ? o1 + 4u * (uint32_t)cstep : o1;
-        const int r0 = (y - step >= 0) ? y - step : y;
-        const int r2 = (y + step < H) ? y + step : y;
-        const uint32_t s0 = __builtin_amdgcn_readfirstlane((uint32_t)r0 * row_bytes);
-        const uint32_t s1 = __builtin_amdgcn_readfirstlane((uint32_t)y * row_bytes);
-        const uint32_t s2 = __builtin_amdgcn_readfirstlane((uint32_t)r2 * row_bytes);
-        u32x4 q[9];
-        q[0] = __builtin_amdgcn_raw_buffer_load_b128(rin, o0, s0, 0);
-        q[1] = __builtin_amdgcn_raw_buffer_load_b128(rin, o1, s0, 0);
-        q[2] = __builtin_amdgcn_raw_buffer_load_b128(rin, o2, s0, 0);
-        q[3] = __builtin_amdgcn_raw_buffer_load_b128(rin, o0, s1, 0);
-        q[4] = __builtin_amdgcn_raw_buffer_load_b128(rin, o1, s1, 0);
-        q[5] = __builtin_amdgcn_raw_buffer_load_b128(rin, o2, s1, 0);
-        q[6] = __builtin_amdgcn_raw_buffer_load_b128(rin, o0, s2, 0);
-        q[7] = __builtin_amdgcn_raw_buffer_load_b128(rin, o1, s2, 0);
-        q[8] = __builtin_amdgcn_raw_buffer_load_b128(rin, o2, s2, 0);
-        const float py = (float)y + 0.5f;
-        uint32_t res[4];
+    const int xb = seg * 256 + lane4;
+    const int xc = min(xb, m.W - 4);
+    const float px0 = (float)xc + 0.5f;
+    const float py = (float)y + 0.5f;
+    uint32_t res[4];
+    if constexpr (MODE == STEP_ANY) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float px = (float)(xc + j) + 0.5f;
-            uint32_t l[9];
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-                const u32x4 &L = q[3 * rr], &Cq = q[3 * rr + 1], &R = q[3 * rr + 2];
-                l[3 * rr + 1] = q_el(Cq, j);
-                if (small) {
-                    l[3 * rr] = small_left(L, Cq, j, step, lok);
-                    l[3 * rr + 2] = small_right(R, Cq, j, step, rok);
-                } else {
-                    l[3 * rr] = q_el(L, j);
-                    l[3 * rr + 2] = q_el(R, j);
-                }
-            }
-            double best;
+            const float px = px0 + (float)j;
+            double best = cand(tab(q_el(it.q[4], j)), q_el(it.q[4], j), px, py);
 #pragma unroll
             for (int t = 0; t < 9; ++t) {
-                const float2 sv = *reinterpret_cast<const float2 *>(tab + l[t]);
-                const float dx = sv.x - px;
-                const float dy = sv.y - py;
-                const float d = __builtin_fmaf(dx, dx, dy * dy);
-                const double key = jfa_key(d, l[t]);
-                best = (t == 0) ? key : __builtin_fmin(best, key);
+                if (t == 4) continue;
+                const uint32_t l = q_el(it.q[t], j);
+                best = __builtin_fmin(best, cand(tab(l), l, px, py));
             }
             res[j] = (uint32_t)__double2loint(best);
         }
-        if (live) {
-            const u32x4 r4 = {res[0], res[1], res[2], res[3]};
-            __builtin_amdgcn_raw_buffer_store_b128(r4, rout, o1, s1, 0);
-            if (clear) {
-                const u32x4 n4 = {none, none, none, none};
-                __builtin_amdgcn_raw_buffer_store_b128(n4, buf_rsrc(clear, map_bytes), o1, s1, 0);
+    } else {
+        constexpr int NP = (MODE == STEP_1) ? 6 : 8;
+        const bool lok = xc > 0, rok = xc + 4 < m.W;
+        double best[4];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const u32x4 &c = it.c[r];
+            uint32_t P[NP];
+            if constexpr (MODE == STEP_1) {
+                P[0] = lok ? it.l[r] : c.x;
+                P[1] = c.x; P[2] = c.y; P[3] = c.z; P[4] = c.w;
+                P[5] = rok ? it.r[r] : c.w;
+            } else {
+                P[0] = lok ? it.l[r].x : c.x;
+                P[1] = lok ? it.l[r].y : c.y;
+                P[2] = c.x; P[3] = c.y; P[4] = c.z; P[5] = c.w;
+                P[6] = rok ? it.r[r].x : c.z;
+                P[7] = rok ? it.r[r].y : c.w;
             }
+            float sx[NP], dyy[NP];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const float2 s = tab(P[k]);
+                const float dy = s.y - py;
+                sx[k] = s.x;
+                dyy[k] = dy * dy;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float px = px0 + (float)j;
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const int k = j + t * MODE;      // left, centre, right
+                    const float dx = sx[k] - px;
+                    const double key = jfa_key(__builtin_fmaf(dx, dx, dyy[k]), P[k]);
+                    best[j] = (r == 0 && t == 0) ? key : __builtin_fmin(best[j], key);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) res[j] = (uint32_t)__double2loint(best[j]);
+    }
+    if (xb < m.W) {
+        const uint32_t o1 = 4u * (uint32_t)xc;
+        const uint32_t s1 = row_off(m, y);
+        const u32x4 r4 = {res[0], res[1], res[2], res[3]};
+        __builtin_amdgcn_raw_buffer_store_b128(r4, m.rout, o1, s1, 0);
+        if (m.clear) {
+            const u32x4 n4 = {m.none, m.none, m.none, m.none};
+            __builtin_amdgcn_raw_buffer_store_b128(n4, m.rclr, o1, s1, 0);
         }
     }
 }
 
-template <bool SITES_IN_LDS>
-__global__ __launch_bounds__(1024) void k_jfa_quad(const uint32_t *__restrict__ in,
-                                                   uint32_t *__restrict__ out,
-                                                   const float2 *__restrict__ sites,
-                                                   int table_f4, int W, int H, int step,
-                                                   uint32_t *__restrict__ clear, uint32_t none)
+// Rows [y0, y1) of one map by all waves of the block.  Items (row, segment)
+// are dealt to the waves round robin; a wave's first load is issued before
+// the site table is staged, so the two latencies overlap.
+template <int MODE, int TAB>
+__device__ __forceinline__ void quad_rows(const MapGeo &m, const SiteTab<TAB> &tab, int y0, int y1)
 {
-    extern __shared__ float4 lds_sites[];
-    if (SITES_IN_LDS) {
-        const float4 *g4 = reinterpret_cast<const float4 *>(sites);
-        for (int i = threadIdx.x; i < table_f4; i += blockDim.x) lds_sites[i] = g4[i];
-        __syncthreads();
+    const int nseg = (m.W + 255) >> 8;
+    const int nwaves = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane4 = 4 * (threadIdx.x & 63);
+    const int last = max(y1 - 1, 0);
+    int row = y0 + wave / nseg, seg = wave % nseg;
+    Item<MODE> A, B;
+    load_item<MODE>(A, m, min(row, last), seg, lane4);
+    tab.stage();
+    while (row < y1) {
+        int r2 = row, g2 = seg + nwaves;
+        while (g2 >= nseg) { g2 -= nseg; ++r2; }
+        // loads past the band are clamped to its last row (never used), so
+        // every iteration has the same number of loads in flight
+        load_item<MODE>(B, m, min(r2, last), g2, lane4);
+        eval_item<MODE, TAB>(A, m, tab, row, seg, lane4);
+        if (r2 >= y1) break;
+        int r3 = r2, g3 = g2 + nwaves;
+        while (g3 >= nseg) { g3 -= nseg; ++r3; }
+        load_item<MODE>(A, m, min(r3, last), g3, lane4);
+        eval_item<MODE, TAB>(B, m, tab, r2, g2, lane4);
+        row = r3;
+        seg = g3;
     }
-    const char *tab = SITES_IN_LDS ? reinterpret_cast<const char *>(lds_sites)
-                                   : reinterpret_cast<const char *>(sites);
-    // band of rows of this workgroup; bands are dealt so that each XCD (blocks
-    // go to the 8 XCDs round robin) owns a contiguous stretch of rows
-    const int G = gridDim.x, g = blockIdx.x;
+}
+
+__device__ __forceinline__ MapGeo map_geo(const uint32_t *in, uint32_t *out, uint32_t *clear,
+                                          int W, int H, int step, uint32_t none)
+{
+    MapGeo m;
+    const uint32_t bytes = 4u * (uint32_t)W * (uint32_t)H;
+    m.rin = buf_rsrc(in, bytes);
+    m.rout = buf_rsrc(out, bytes);
+    m.rclr = buf_rsrc(clear ? clear : out, bytes);
+    m.row_bytes = 4u * (uint32_t)W;
+    m.W = W;
+    m.H = H;
+    m.step = step;
+    m.clear = clear != nullptr;
+    m.none = none;
+    return m;
+}
+
+// band of rows of workgroup g of G: bands are dealt so that each XCD (blocks
+// go to the 8 XCDs round robin) owns a contiguous stretch of rows, keeping
+// the tap rows of small steps inside one XCD's L2
+__device__ __forceinline__ void band_of(int g, int G, int H, int &y0, int &y1)
+{
     const int q8 = G >> 3, r8 = G & 7, xcd = g & 7;
     const int band = xcd * q8 + min(xcd, r8) + (g >> 3);
-    const int y0 = (int)((long)H * band / G), y1 = (int)((long)H * (band + 1) / G);
-    jfa_quad_rows(in, out, clear, tab, 0u, W, H, y0, y1, step, none);
+    y0 = (int)((long)H * band / G);
+    y1 = (int)((long)H * (band + 1) / G);
+}
+
+// Single map, site table staged at LDS byte 0 in a static array of ENTRIES
+// float2 (the kernel's only LDS), or gathered from global memory when
+// ENTRIES == 0.
+#define JFA_THREADS(E) ((E) > 5120 ? 1024 : 256)
+#define JFA_MINB(E) ((E) > 5120 ? 1 : 4)
+
+template <int MODE, int ENTRIES>
+__global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa_quad(
+    const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+    const float2 *__restrict__ sites, int table_f4, int W, int H, int step,
+    uint32_t *__restrict__ clear, uint32_t none)
+{
+    const MapGeo m = map_geo(in, out, clear, W, H, step, none);
+    int y0, y1;
+    band_of(blockIdx.x, gridDim.x, H, y0, y1);
+    if constexpr (ENTRIES > 0) {
+        __shared__ float4 lds_tab[ENTRIES / 2];
+        SiteTab<TAB_LDS0> tab;
+        tab.lds = (lds_cchar *)lds_tab;
+        tab.src = reinterpret_cast<const f32x4 *>(sites);
+        tab.n4 = table_f4;
+        tab.base = 0;
+        quad_rows<MODE, TAB_LDS0>(m, tab, y0, y1);
+    } else {
+        SiteTab<TAB_GLOBAL> tab;
+        tab.g = buf_rsrc(sites, 16u * (uint32_t)table_f4);
+        tab.base = 0;
+        tab.n4 = 0;
+        quad_rows<MODE, TAB_GLOBAL>(m, tab, y0, y1);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// batched pass: one launch over many patches.  Each workgroup takes one work
+// Batched pass: one launch over many patches.  Each workgroup takes one work
 // unit (patch, band of rows); patches run their passes right-aligned, so pass
 // k of the batch is pass k - (Pmax - P) of a patch with P passes (skipped
 // when negative) and every patch ends its JFA in the same launch.
 // ---------------------------------------------------------------------------
-template <bool SITES_IN_LDS>
+template <bool SITES_IN_LDS, int MODE>
 __global__ __launch_bounds__(1024) void k_jfa_batch(const BatchPatch *__restrict__ patches,
                                                     const BatchUnit *__restrict__ units,
                                                     uint32_t *__restrict__ seed,
@@ -283,50 +440,69 @@ __global__ __launch_bounds__(1024) void k_jfa_batch(const BatchPatch *__restrict
                                                     const float2 *__restrict__ sites, int pass,
                                                     int pmax)
 {
-    extern __shared__ float4 lds_sites[];
+    extern __shared__ float4 lds_dyn[];
     const BatchUnit u = units[blockIdx.x];
     const BatchPatch P = patches[u.patch];
     const int local = pass - (pmax - P.passes);
     if (local < 0) return;                                   // uniform per block
     const int step = (local < P.passes - 1) ? (P.first_step >> local) : 1;
-    const char *tab;
-    uint32_t base;
-    if (SITES_IN_LDS) {
-        const float4 *g4 = reinterpret_cast<const float4 *>(sites + P.site_off);
-        const int nf4 = (P.cap + 2) / 2;
-        for (int i = threadIdx.x; i < nf4; i += blockDim.x) lds_sites[i] = g4[i];
-        __syncthreads();
-        tab = reinterpret_cast<const char *>(lds_sites);
-        base = 8u * (uint32_t)P.site_off;
-    } else {
-        tab = reinterpret_cast<const char *>(sites);
-        base = 0u;
-    }
+    if ((step == 1 ? STEP_1 : step == 2 ? STEP_2 : STEP_ANY) != MODE) return;
     uint32_t *bufs[2] = {lab0 + P.pix_off, lab1 + P.pix_off};
     const uint32_t *in = (local == 0) ? seed + P.pix_off : bufs[(local - 1) & 1];
     uint32_t *out = bufs[local & 1];
     uint32_t *clear = (local == 1) ? seed + P.pix_off : nullptr;
     const uint32_t none = 8u * (uint32_t)(P.site_off + P.cap);
-    jfa_quad_rows(in, out, clear, tab, base, P.W, P.H, u.y0, u.y1, step, none);
+    const MapGeo m = map_geo(in, out, clear, P.W, P.H, step, none);
+    if (SITES_IN_LDS) {
+        SiteTab<TAB_LDSB> tab;
+        tab.lds = (lds_cchar *)lds_dyn;
+        tab.src = reinterpret_cast<const f32x4 *>(sites + P.site_off);
+        tab.n4 = (P.cap + 2) / 2;
+        tab.base = 8u * (uint32_t)P.site_off;
+        quad_rows<MODE, TAB_LDSB>(m, tab, u.y0, u.y1);
+    } else {
+        SiteTab<TAB_GLOBAL> tab;
+        tab.g = buf_rsrc(sites, 0xffffffffu);
+        tab.base = 0;
+        tab.n4 = 0;
+        quad_rows<MODE, TAB_GLOBAL>(m, tab, u.y0, u.y1);
+    }
+}
+
+template <bool LDS, int MODE>
+static void launch_batch_mode(hipStream_t st, const BatchPatch *patches, const BatchUnit *units,
+                              int nunits, uint32_t *seed, uint32_t *lab0, uint32_t *lab1,
+                              const float2 *sites, int pass, int pmax, size_t lds_bytes)
+{
+    if (LDS) {
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void *)k_jfa_batch<LDS, MODE>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr = true;
+        }
+    }
+    hipLaunchKernelGGL((k_jfa_batch<LDS, MODE>), dim3(nunits), dim3(1024), LDS ? lds_bytes : 0, st,
+                       patches, units, seed, lab0, lab1, sites, pass, pmax);
 }
 
 void launch_jfa_batch(hipStream_t st, const BatchPatch *patches, const BatchUnit *units, int nunits,
                       uint32_t *seed, uint32_t *lab0, uint32_t *lab1, const float2 *sites, int pass,
-                      int pmax, size_t lds_bytes)
+                      int pmax, size_t lds_bytes, unsigned modes)
 {
-    if (lds_bytes) {
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute((const void *)k_jfa_batch<true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            attr = true;
-        }
-        hipLaunchKernelGGL(k_jfa_batch<true>, dim3(nunits), dim3(1024), lds_bytes, st, patches,
-                           units, seed, lab0, lab1, sites, pass, pmax);
-    } else {
-        hipLaunchKernelGGL(k_jfa_batch<false>, dim3(nunits), dim3(1024), 0, st, patches, units,
-                           seed, lab0, lab1, sites, pass, pmax);
+#define CVT_BATCH_MODE(M)                                                                      \
+    if (modes & (1u << (M))) {                                                                 \
+        if (lds_bytes)                                                                         \
+            launch_batch_mode<true, M>(st, patches, units, nunits, seed, lab0, lab1, sites, pass, \
+                                       pmax, lds_bytes);                                       \
+        else                                                                                   \
+            launch_batch_mode<false, M>(st, patches, units, nunits, seed, lab0, lab1, sites,    \
+                                        pass, pmax, 0);                                        \
     }
+    CVT_BATCH_MODE(STEP_ANY)
+    CVT_BATCH_MODE(STEP_1)
+    CVT_BATCH_MODE(STEP_2)
+#undef CVT_BATCH_MODE
 }
 
 #ifndef CVT_JFA_ROWS
@@ -334,7 +510,9 @@ void launch_jfa_batch(hipStream_t st, const BatchPatch *patches, const BatchUnit
 #endif
 
 static int g_num_cu = 0;
-static size_t g_lds_max = 0;
+
+// static LDS table sizes of the single-map kernel, in float2 entries
+static const int kLdsEntries[3] = {2048, 5120, 20480};
 
 JfaPlan plan_jfa(int W, int H, int capS)
 {
@@ -342,32 +520,28 @@ JfaPlan plan_jfa(int W, int H, int capS)
         int dev = 0;
         (void)hipGetDevice(&dev);
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
-            g_num_cu = prop.multiProcessorCount;
-            g_lds_max = prop.sharedMemPerBlock;
-            if (prop.maxSharedMemoryPerMultiProcessor > g_lds_max)
-                g_lds_max = prop.maxSharedMemoryPerMultiProcessor;
-        } else {
-            g_num_cu = 256;
-            g_lds_max = 65536;
-        }
-        (void)hipFuncSetAttribute((const void *)k_jfa_quad<true>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_max);
+        g_num_cu = (hipGetDeviceProperties(&prop, dev) == hipSuccess) ? prop.multiProcessorCount
+                                                                      : 256;
     }
     JfaPlan p;
     p.quad = (W % 4 == 0) && W >= 8;
-    const size_t table = (size_t)(capS + 2) / 2 * 16;   // sites incl. sentinel, float4 units
-    p.table_f4 = (int)(table / 16);
-    p.lds = p.quad && table <= g_lds_max;
-    p.lds_bytes = p.lds ? table : 0;
+    // sites and the sentinel (index capS), as whole float4 pairs; the site
+    // buffer holds capS + 2 entries, so the last pair is always inside it
+    p.table_f4 = (capS + 2) / 2;
+    const int entries = 2 * p.table_f4;
+    p.lds_entries = 0;
+    if (p.quad)
+        for (int e : kLdsEntries)
+            if (entries <= e) { p.lds_entries = e; break; }
+    p.lds = p.lds_entries > 0;
+    p.lds_bytes = p.lds ? (size_t)p.lds_entries * 8 : 0;
     const long items = (long)H * ((W + 255) / 256);
-    if (p.lds && table > 40 * 1024) {
+    if (p.lds_entries > 5120) {
         p.threads = 1024;                               // one workgroup per CU
         p.grid = g_num_cu;
     } else {
         p.threads = 256;
-        int per_cu = p.lds ? (int)std::min<size_t>(8, g_lds_max / std::max<size_t>(table, 1)) : 8;
-        p.grid = g_num_cu * std::max(per_cu, 1);
+        p.grid = g_num_cu * 4;
     }
     // at least one item per wave
     const long waves = (long)p.grid * (p.threads / 64);
@@ -376,16 +550,38 @@ JfaPlan plan_jfa(int W, int H, int capS)
     return p;
 }
 
+template <int MODE>
+static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
+                        const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none)
+{
+    const dim3 g(p.grid), b(p.threads);
+    switch (p.lds_entries) {
+    case 2048:
+        hipLaunchKernelGGL((k_jfa_quad<MODE, 2048>), g, b, 0, st, in, out, sites, p.table_f4, W, H,
+                           step, clear, none);
+        break;
+    case 5120:
+        hipLaunchKernelGGL((k_jfa_quad<MODE, 5120>), g, b, 0, st, in, out, sites, p.table_f4, W, H,
+                           step, clear, none);
+        break;
+    case 20480:
+        hipLaunchKernelGGL((k_jfa_quad<MODE, 20480>), g, b, 0, st, in, out, sites, p.table_f4, W,
+                           H, step, clear, none);
+        break;
+    default:
+        hipLaunchKernelGGL((k_jfa_quad<MODE, 0>), g, b, 0, st, in, out, sites, p.table_f4, W, H,
+                           step, clear, none);
+        break;
+    }
+}
+
 void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none)
 {
     if (p.quad) {
-        if (p.lds)
-            hipLaunchKernelGGL(k_jfa_quad<true>, dim3(p.grid), dim3(p.threads), p.lds_bytes, st, in,
-                               out, sites, p.table_f4, W, H, step, clear, none);
-        else
-            hipLaunchKernelGGL(k_jfa_quad<false>, dim3(p.grid), dim3(p.threads), 0, st, in, out,
-                               sites, p.table_f4, W, H, step, clear, none);
+        if (step == 1) launch_quad<STEP_1>(st, p, in, out, sites, W, H, step, clear, none);
+        else if (step == 2) launch_quad<STEP_2>(st, p, in, out, sites, W, H, step, clear, none);
+        else launch_quad<STEP_ANY>(st, p, in, out, sites, W, H, step, clear, none);
         return;
     }
     constexpr int R = CVT_JFA_ROWS;

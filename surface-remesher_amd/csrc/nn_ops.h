@@ -41,9 +41,16 @@ void nn_sample_loss(hipStream_t s, const float *field, int R, const float *uv, c
                     int S, int nb, float scale, float *pred, float *gfield, float *loss_part);
 void nn_adam(hipStream_t s, float *w, const float *g, float *m1, float *m2, long n, float lr,
              float b1, float b2, float eps, int t);
-// network input (R, R, C) from an n x n density and S sites (pixel units);
-// count is an R*R float scratch
-void nn_prepare(hipStream_t s, const float *rho, int n, float inv_mean, const float *xy, int S,
-                int R, int C, uint16_t *out, float *count);
+#define NN_SUM_BLOCKS 256
+// 1 / mean of an n x n density into inv_mean[0] (device), 0 if the sum is 0;
+// part is NN_SUM_BLOCKS doubles of scratch
+void nn_density_inv_mean(hipStream_t s, const float *rho, int n, double *part, float *inv_mean);
+// network input (R, R, C) from an n x n density, its 1 / mean (device) and S
+// sites (pixel units); count is an R*R float scratch
+void nn_prepare(hipStream_t s, const float *rho, int n, const float *inv_mean, const float *xy,
+                int S, int R, int C, uint16_t *out, float *count);
+// sites += field(site / n) * n for free sites whose target pixel has density
+void nn_apply_disp(hipStream_t s, float2 *sites, const uint8_t *fixed, int S, const float *field,
+                   int R, const float *rho, int n);
 
 }  // namespace cvt

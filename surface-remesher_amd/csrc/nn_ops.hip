@@ -389,6 +389,34 @@ __global__ void k_sample_loss(const float *__restrict__ field, int R, const floa
     if (threadIdx.x == 0 && loss_part) loss_part[blockIdx.x] = red[0];
 }
 
+// Moves each free site of an n x n engine grid by the field sampled at its
+// position (displacements in units of the grid side).  A site keeps its
+// position when it is fixed, when the target lies off the grid, or when the
+// target pixel has zero density (a site there would own no mass).
+__global__ void k_apply_disp(float2 *__restrict__ sites, const uint8_t *__restrict__ fixed, int S,
+                             const float *__restrict__ field, int R, const float *__restrict__ rho,
+                             int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S || fixed[i]) return;
+    const float2 s = sites[i];
+    const float inv = 1.0f / (float)n;
+    int x0, x1, y0, y1;
+    float tx, ty;
+    bil_coords(s.x * inv, R, x0, x1, tx);
+    bil_coords(s.y * inv, R, y0, y1, ty);
+    const float w00 = (1 - tx) * (1 - ty), w10 = tx * (1 - ty), w01 = (1 - tx) * ty, w11 = tx * ty;
+    const long p00 = (long)y0 * R + x0, p10 = (long)y0 * R + x1, p01 = (long)y1 * R + x0,
+               p11 = (long)y1 * R + x1;
+    float d[2];
+    for (int j = 0; j < 2; ++j)
+        d[j] = w00 * field[2 * p00 + j] + w10 * field[2 * p10 + j] + w01 * field[2 * p01 + j] +
+               w11 * field[2 * p11 + j];
+    const float x = s.x + d[0] * (float)n, y = s.y + d[1] * (float)n;
+    if (!(x >= 0.0f && x < (float)n && y >= 0.0f && y < (float)n)) return;   // NaN too
+    if (rho[(long)(int)y * n + (int)x] > 0.0f) sites[i] = make_float2(x, y);
+}
+
 // Adam (bias-corrected) on fp32 parameters
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m1,
                        float *__restrict__ m2, long n, float lr, float b1, float b2, float eps,
@@ -404,11 +432,17 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
     w[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
 }
 
+__global__ void k_fill_f32(float *__restrict__ p, float v, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // network input at R x R: channel 0 = box-mean density / its mean, channel 1 =
 // site count per cell * R^2 / S, channel 2 = 1 where the cell has density;
 // channels 3 .. C-1 zero.  The density is n x n with n a multiple of R.
 __global__ void k_prepare_density(const float *__restrict__ rho, int n, int R, int C,
-                                  float inv_mean, uint16_t *__restrict__ out)
+                                  const float *__restrict__ inv_mean, uint16_t *__restrict__ out)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)R * R) return;
@@ -419,10 +453,41 @@ __global__ void k_prepare_density(const float *__restrict__ rho, int n, int R, i
         for (int xx = 0; xx < f; ++xx) s += rho[(long)(y * f + yy) * n + (x * f + xx)];
     s /= (float)(f * f);
     uint16_t *o = out + i * C;
-    o[0] = f2bf(s * inv_mean);
+    o[0] = f2bf(s * inv_mean[0]);
     o[1] = 0;
     o[2] = f2bf(s > 0.0f ? 1.0f : 0.0f);
     for (int c = 3; c < C; ++c) o[c] = 0;
+}
+
+// float64 sum of the density: per-block partial sums over a fixed partition,
+// then one block adds the partials in index order (deterministic)
+__global__ __launch_bounds__(256) void k_rho_partial(const float *__restrict__ rho, long n,
+                                                     double *__restrict__ part)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += 256L * NN_SUM_BLOCKS)
+        acc += (double)rho[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_rho_inv_mean(const double *__restrict__ part, long n,
+                                                      float *__restrict__ inv_mean)
+{
+    __shared__ double red[256];
+    red[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) inv_mean[0] = red[0] > 0.0 ? (float)((double)n / red[0]) : 0.0f;
 }
 
 __global__ void k_prepare_sites(const float *__restrict__ xy, int S, int n, int R, int C,
@@ -550,16 +615,31 @@ void nn_adam(hipStream_t s, float *w, const float *g, float *m1, float *m2, long
     hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, s, w, g, m1, m2, n, lr, b1, b2, eps, c1, c2);
 }
 
-void nn_prepare(hipStream_t s, const float *rho, int n, float inv_mean, const float *xy, int S,
-                int R, int C, uint16_t *out, float *count)
+void nn_density_inv_mean(hipStream_t s, const float *rho, int n, double *part, float *inv_mean)
+{
+    hipLaunchKernelGGL(k_rho_partial, dim3(NN_SUM_BLOCKS), dim3(256), 0, s, rho, (long)n * n, part);
+    hipLaunchKernelGGL(k_rho_inv_mean, dim3(1), dim3(256), 0, s, part, (long)n * n, inv_mean);
+}
+
+void nn_prepare(hipStream_t s, const float *rho, int n, const float *inv_mean, const float *xy,
+                int S, int R, int C, uint16_t *out, float *count)
 {
     hipLaunchKernelGGL(k_prepare_density, dim3(nblk((long)R * R)), dim3(256), 0, s, rho, n, R, C,
                        inv_mean, out);
-    (void)hipMemsetAsync(count, 0, sizeof(float) * R * R, s);
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblk((long)R * R)), dim3(256), 0, s, count, 0.0f,
+                       (long)R * R);
     if (S > 0)
         hipLaunchKernelGGL(k_prepare_sites, dim3(nblk(S)), dim3(256), 0, s, xy, S, n, R, C,
                            (float)R * R / (float)S, count);
     hipLaunchKernelGGL(k_prepare_finish, dim3(nblk((long)R * R)), dim3(256), 0, s, count, R, C, out);
+}
+
+void nn_apply_disp(hipStream_t s, float2 *sites, const uint8_t *fixed, int S, const float *field,
+                   int R, const float *rho, int n)
+{
+    if (S > 0)
+        hipLaunchKernelGGL(k_apply_disp, dim3(nblk(S)), dim3(256), 0, s, sites, fixed, S, field, R,
+                           rho, n);
 }
 
 }  // namespace cvt

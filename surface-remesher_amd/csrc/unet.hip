@@ -65,6 +65,10 @@ struct UNet {
     float *uv_buf = nullptr, *tgt_buf = nullptr, *count = nullptr, *xy_buf = nullptr;
     int site_cap = 0;
     long site_cap_loss = 0;
+    double *mean_part = nullptr;            // NN_SUM_BLOCKS partial sums
+    float *inv_mean = nullptr;              // 1 / mean density of the last prepare
+    float *rho_buf = nullptr;               // device copy of a host density
+    long rho_cap = 0;
 
     ~UNet();
     int res(int level) const { return R >> level; }
@@ -77,7 +81,7 @@ struct UNet {
 UNet::~UNet()
 {
     std::vector<void *> bufs = {p, g, m1, m2, x0, field, gfield, work, loss_part, uv_buf, tgt_buf,
-                                count, xy_buf};
+                                count, xy_buf, mean_part, inv_mean, rho_buf};
     for (int l = 0; l < 14; ++l) {
         bufs.push_back(wf[l]); bufs.push_back(wb[l]); bufs.push_back(a[l]); bufs.push_back(ga[l]);
     }
@@ -228,6 +232,8 @@ int cvt_unet_create(cvt_unet **out, int R, int max_batch, unsigned long long see
     u->work_n = wn;
     alloc((void **)&u->work, sizeof(float) * wn);
     alloc((void **)&u->count, sizeof(float) * R * R);
+    alloc((void **)&u->mean_part, sizeof(double) * NN_SUM_BLOCKS);
+    alloc((void **)&u->inv_mean, sizeof(float) * 4);
     if (!ok) {
         delete u;
         return fail(CVT_ERR_NOMEM, "U-Net buffers (R %d, batch %d)", R, max_batch);
@@ -335,6 +341,21 @@ int cvt_unet_set_input(cvt_unet *net, const uint16_t *x, int nb)
     return CVT_OK;
 }
 
+static bool is_device_ptr(const void *p)
+{
+    hipPointerAttribute_t at;
+    return hipPointerGetAttributes(&at, p) == hipSuccess && at.type == hipMemoryTypeDevice;
+}
+
+// input of `slot` (or into dst) from a device density and device sites, on
+// stream st; nothing synchronises
+static void prepare_device(UNet *u, hipStream_t st, const float *rho, int n, const float *xy, int S,
+                           uint16_t *out)
+{
+    nn_density_inv_mean(st, rho, n, u->mean_part, u->inv_mean);
+    nn_prepare(st, rho, n, u->inv_mean, xy, S, u->R, IN_C, out, u->count);
+}
+
 int cvt_unet_prepare(cvt_unet *net, int slot, const float *density, int n, const float *xy, int S,
                      uint16_t *dst)
 {
@@ -343,48 +364,77 @@ int cvt_unet_prepare(cvt_unet *net, int slot, const float *density, int n, const
     if (!density || n < u->R || (n % u->R) != 0 || S < 0 || (S > 0 && !xy))
         return fail(CVT_ERR_ARG, "density %d (a multiple of %d), %d sites", n, u->R, S);
     if (!dst && (slot < 0 || slot >= u->maxb)) return fail(CVT_ERR_ARG, "slot %d", slot);
-    // the mean density, for the normalisation of channel 0
-    std::vector<float> h((size_t)n * n);
-    if (hipMemcpy(h.data(), density, sizeof(float) * h.size(), hipMemcpyDefault) != hipSuccess)
-        return fail(CVT_ERR_HIP, "density read");
-    double sum = 0.0;
-    for (float v : h) sum += v;
-    const float inv_mean = sum > 0 ? (float)((double)h.size() / sum) : 0.0f;
-    float *drho = nullptr;
-    const bool dev_rho = [&] {
-        hipPointerAttribute_t at;
-        return hipPointerGetAttributes(&at, density) == hipSuccess && at.type == hipMemoryTypeDevice;
-    }();
-    if (!dev_rho) {
-        if (hipMalloc(&drho, sizeof(float) * h.size()) != hipSuccess) return fail(CVT_ERR_NOMEM, "density");
-        (void)hipMemcpyAsync(drho, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, u->stream);
+    const long nn = (long)n * n;
+    const float *rho = density;
+    if (!is_device_ptr(density)) {
+        if (nn > u->rho_cap) {
+            if (u->rho_buf) (void)hipFree(u->rho_buf);
+            u->rho_buf = nullptr;
+            u->rho_cap = 0;
+            if (hipMalloc(&u->rho_buf, sizeof(float) * nn) != hipSuccess)
+                return fail(CVT_ERR_NOMEM, "density");
+            u->rho_cap = nn;
+        }
+        if (hipMemcpyAsync(u->rho_buf, density, sizeof(float) * nn, hipMemcpyDefault, u->stream) !=
+            hipSuccess)
+            return fail(CVT_ERR_HIP, "density upload");
+        rho = u->rho_buf;
     }
     if (S > u->site_cap) {
         if (u->xy_buf) (void)hipFree(u->xy_buf);
         u->xy_buf = nullptr;
-        if (hipMalloc(&u->xy_buf, sizeof(float) * 2 * S) != hipSuccess) {
-            if (drho) (void)hipFree(drho);
+        u->site_cap = 0;
+        if (hipMalloc(&u->xy_buf, sizeof(float) * 2 * S) != hipSuccess)
             return fail(CVT_ERR_NOMEM, "sites");
-        }
         u->site_cap = S;
     }
-    if (S > 0) (void)hipMemcpyAsync(u->xy_buf, xy, sizeof(float) * 2 * S, hipMemcpyDefault, u->stream);
+    if (S > 0 &&
+        hipMemcpyAsync(u->xy_buf, xy, sizeof(float) * 2 * S, hipMemcpyDefault, u->stream) != hipSuccess)
+        return fail(CVT_ERR_HIP, "site upload");
     // kernels write device memory only: a caller's dst (host or device) is
     // filled by a copy from a device staging buffer
     const size_t bytes = 2 * (size_t)u->R * u->R * IN_C;
     uint16_t *stage = nullptr;
-    if (dst && hipMalloc(&stage, bytes) != hipSuccess) {
-        if (drho) (void)hipFree(drho);
-        return fail(CVT_ERR_NOMEM, "staging");
-    }
+    if (dst && hipMalloc(&stage, bytes) != hipSuccess) return fail(CVT_ERR_NOMEM, "staging");
     uint16_t *out = dst ? stage : u->x0 + (size_t)slot * u->R * u->R * IN_C;
-    nn_prepare(u->stream, dev_rho ? density : drho, n, inv_mean, u->xy_buf, S, u->R, IN_C, out,
-               u->count);
-    if (dst) (void)hipMemcpyAsync(dst, stage, bytes, hipMemcpyDefault, u->stream);
-    const hipError_t e = hipStreamSynchronize(u->stream);
-    if (drho) (void)hipFree(drho);
-    if (stage) (void)hipFree(stage);
+    prepare_device(u, u->stream, rho, n, u->xy_buf, S, out);
+    hipError_t e = hipSuccess;
+    if (dst) {
+        (void)hipMemcpyAsync(dst, stage, bytes, hipMemcpyDefault, u->stream);
+        e = hipStreamSynchronize(u->stream);
+        (void)hipFree(stage);
+    }
+    // a host density or site array is read by the copies above before the
+    // call returns only after they complete
+    if (!dst && (rho != density || (S > 0 && !is_device_ptr(xy))))
+        e = hipStreamSynchronize(u->stream);
     return e == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "prepare");
+}
+
+int cvt_unet_warm_start(cvt_unet *net, cvt_ctx *ctx)
+{
+    UCHK(net);
+    if (!ctx) return fail(CVT_ERR_ARG, "null context");
+    UNet *u = reinterpret_cast<UNet *>(net);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->have_density) return fail(CVT_ERR_STATE, "warm start before cvt_set_density");
+    if (c->W != c->H || c->W < u->R || c->W % u->R != 0)
+        return fail(CVT_ERR_ARG, "warm start needs a square grid, a multiple of %d (got %d x %d)",
+                    u->R, c->W, c->H);
+    if (c->S == 0) return CVT_OK;
+    const hipStream_t st = c->stream;
+    // the seed map holds the old sites' seeds: clear them before moving
+    if (c->seeds_valid) c->clear_seed_map();
+    prepare_device(u, st, c->rho, c->W, reinterpret_cast<const float *>(c->sites), c->S, u->x0);
+    const hipStream_t saved = u->stream;
+    u->stream = st;                      // the forward runs on the engine's stream
+    u->forward(1);
+    u->stream = saved;
+    nn_apply_disp(st, c->sites, c->fixed, c->S, u->field, u->R, c->rho, c->W);
+    c->seeds_valid = false;
+    c->labels_fresh = false;
+    c->enqueue_seed();
+    return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "warm start launch");
 }
 
 int cvt_unet_forward(cvt_unet *net, int nb)

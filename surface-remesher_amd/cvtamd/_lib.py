@@ -87,6 +87,7 @@ SIGNATURES = {
     "cvt_unet_loss_backward": (_i, [_vp, _i, _vp, _vp, _i, C.POINTER(_f)]),
     "cvt_unet_adam": (_i, [_vp, _f, _f, _f, _f]),
     "cvt_unet_sample": (_i, [_vp, _i, _vp, _i, _vp]),
+    "cvt_unet_warm_start": (_i, [_vp, _vp]),
     "cvt_unet_save": (_i, [_vp, C.c_char_p]),
     "cvt_unet_load": (_i, [_vp, C.c_char_p]),
     "cvt_unet_synchronize": (_i, [_vp]),

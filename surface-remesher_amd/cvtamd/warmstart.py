@@ -98,6 +98,11 @@ class UNet:
              out.ctypes.data_as(C.c_void_p))
         return out
 
+    def warm_start(self, engine):
+        """Moves the engine's sites by the network's displacement field, on
+        the engine's stream (cvt_unet_warm_start)."""
+        call("cvt_unet_warm_start", self._h, engine._h)
+
     def save(self, path):
         call("cvt_unet_save", self._h, str(path).encode())
 

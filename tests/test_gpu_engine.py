@@ -52,6 +52,9 @@ def check_moments(m, ref, W_tol=1e-6):
     (98, 77, 150, 7),         # width not a multiple of 4: one pixel per lane
     (1024, 1024, 30000, 8),   # site table too large for LDS: global gathers
     (12, 9, 5, 9),            # tiny grid
+    (8, 8, 3, 11),            # narrowest grid of the 4-pixel kernel
+    (260, 130, 5000, 12),     # 40 KB site table, width not a multiple of 256
+    (512, 384, 8000, 10),     # 160 KB site table on a small grid
 ])
 def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
     xy = rand_sites(seed, W, H, S)
@@ -146,7 +149,7 @@ def test_moments_lattice_exact(gpu, orc):
 
 def test_capacity_error(gpu):
     import cvtamd
-    # 2000 sites inside one 64x64 tile exceed its 1024-entry table
+    # 2000 sites inside one 64x64 tile exceed its 256-entry table
     n = 256
     g = np.random.default_rng(3)
     xy = (g.random((2000, 2)) * 64).astype(np.float32)

@@ -103,3 +103,42 @@ def test_prepare_matches_reference(gpu):
     got = (dst.astype(np.uint32) << 16).view(np.float32)
     exp = ref.bf16(ref.prepare(w.density, w.sites, 256))
     assert np.allclose(got, exp, rtol=1e-2, atol=1e-6)
+
+
+def test_warm_start_moves_sites_by_the_field(gpu, orc):
+    """cvt_unet_warm_start equals prepare + forward + sample through the
+    separate entry points, bit for bit; fixed sites and moves onto zero
+    density are refused; the engine's next Voronoi map is that of the moved
+    sites."""
+    import cvtamd
+    from cvtamd import workloads as wl
+    from cvtamd.warmstart import UNet
+    n, S = 512, 2000
+    w = wl.config2(5, n=n, num_sites=S)
+    net = UNet(256, 1, 3)
+    p = net.params()
+    p[-66:-2] = np.random.default_rng(8).normal(0, 0.002, 64).astype(np.float32)
+    net.set_params(p)
+    fixed = np.zeros(S, np.uint8)
+    fixed[:50] = 1
+    with cvtamd.Engine(n, n, S) as e:
+        e.set_density(w.density)
+        e.set_sites(w.sites, fixed)
+        e.lloyd(1)                       # leaves seeds of the moved sites pending
+        start = e.sites()
+        net.warm_start(e)
+        got = e.sites()
+        net.prepare(0, w.density, start)
+        net.forward(1)
+        d = net.sample(0, start / np.float32(n))
+        tgt = start + d * np.float32(n)
+        inside = (tgt[:, 0] >= 0) & (tgt[:, 0] < n) & (tgt[:, 1] >= 0) & (tgt[:, 1] < n)
+        ix = np.clip(tgt[:, 0].astype(int), 0, n - 1)
+        iy = np.clip(tgt[:, 1].astype(int), 0, n - 1)
+        ok = inside & (w.density[iy, ix] > 0) & (fixed == 0)
+        exp = np.where(ok[:, None], tgt, start)
+        assert np.array_equal(got, exp)
+        assert ok.sum() > S // 2 and np.abs(d).max() * n > 0.01
+        assert np.array_equal(got[:50], start[:50])
+        e.voronoi()
+        assert np.array_equal(e.labels(), orc.jfa(n, n, got))
