@@ -184,7 +184,8 @@ int cvt_batch_synchronize(cvt_batch *batch);
  * U-Net, 32 / 64 / 96 / 128 channels, BF16 weights and activations with FP32
  * accumulation on MFMA, FP32 master weights, Adam.  Input R x R (R = 256 for
  * the warm start): density box mean / mean, initial-site count per cell *
- * R^2 / S, and the domain indicator; output a 2-channel displacement field
+ * R^2 / S averaged over a (2h+1)^2 window with h = round(R / sqrt(S)) - 1,
+ * and the domain indicator; output a 2-channel displacement field
  * (units of the domain side) sampled bilinearly at the sites.  All buffers
  * may be host or device pointers.
  * ------------------------------------------------------------------------- */

@@ -217,6 +217,13 @@ def prepare(density, xy, R):
         cx = np.clip((xy[:, 0] * R / n).astype(int), 0, R - 1)
         cy = np.clip((xy[:, 1] * R / n).astype(int), 0, R - 1)
         np.add.at(cnt, (cy, cx), R * R / len(xy))
-    x[..., 1] = cnt
+    # site density averaged over a (2h+1)^2 window, zero outside the grid
+    h = max(0, int(round(R / np.sqrt(len(xy)))) - 1) if len(xy) else 0
+    pad = np.pad(cnt, h)
+    win = np.zeros((R, R))
+    for dy in range(2 * h + 1):
+        for dx in range(2 * h + 1):
+            win += pad[dy:dy + R, dx:dx + R]
+    x[..., 1] = win / (2 * h + 1) ** 2
     x[..., 2] = (box > 0).astype(np.float64)
     return x

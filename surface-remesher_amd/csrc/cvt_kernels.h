@@ -458,11 +458,17 @@ __device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__
 {
     __int128 sw = 0, sx = 0, sy = 0, se = 0;
     const Slot *sl = slots + (long)i * 9;
+    // all nine slots are loaded before any is tested, so the loads overlap
+    Slot v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = sl[k];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        if (sl[k].epoch == epoch) {
-            sw += sl[k].w; sx += sl[k].dx; sy += sl[k].dy; se += sl[k].e;
-        }
+        const bool on = v[k].epoch == epoch;
+        sw += on ? v[k].w : 0;
+        sx += on ? v[k].dx : 0;
+        sy += on ? v[k].dy : 0;
+        se += on ? v[k].e : 0;
     }
     for (int k = ovf_head[i]; k >= 0; k = ovf[k].next) {
         sw += ovf[k].w; sx += ovf[k].dx; sy += ovf[k].dy; se += ovf[k].e;

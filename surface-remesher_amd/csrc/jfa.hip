@@ -171,12 +171,31 @@ template <int TAB> struct SiteTab {
             return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
         }
     }
-    // copy the table into LDS (all threads of the block take part)
+    // copy the table into LDS (all threads of the block take part); each
+    // thread issues all of its loads (8 per round, range-checked buffer loads)
+    // before its first LDS store, so the round trips overlap
     __device__ __forceinline__ void stage() const
     {
         if constexpr (TAB != TAB_GLOBAL) {
+            typedef unsigned u4 __attribute__((ext_vector_type(4)));
             lds_f32x4 *dst = (lds_f32x4 *)lds;
-            for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
+            const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<f32x4 *>(src), (short)0, 16 * n4, 0x00020000);
+            const int stride = blockDim.x;
+            for (int i0 = threadIdx.x; i0 < n4; i0 += 8 * stride) {
+                u4 v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    v[k] = __builtin_amdgcn_raw_buffer_load_b128(r, 16u * (uint32_t)(i0 + k * stride),
+                                                                 0, 0);
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (i0 + k * stride < n4) {
+                        const f32x4 f = {__uint_as_float(v[k].x), __uint_as_float(v[k].y),
+                                         __uint_as_float(v[k].z), __uint_as_float(v[k].w)};
+                        dst[i0 + k * stride] = f;
+                    }
+            }
             __syncthreads();
         }
     }

@@ -45,6 +45,9 @@ void nn_adam(hipStream_t s, float *w, const float *g, float *m1, float *m2, long
 // 1 / mean of an n x n density into inv_mean[0] (device), 0 if the sum is 0;
 // part is NN_SUM_BLOCKS doubles of scratch
 void nn_density_inv_mean(hipStream_t s, const float *rho, int n, double *part, float *inv_mean);
+// half-width h of the window that averages the site splat: round(R / sqrt(S))
+// - 1, so a window holds about three sites on average
+int nn_splat_halfwidth(int R, int S);
 // network input (R, R, C) from an n x n density, its 1 / mean (device) and S
 // sites (pixel units); count is an R*R float scratch
 void nn_prepare(hipStream_t s, const float *rho, int n, const float *inv_mean, const float *xy,

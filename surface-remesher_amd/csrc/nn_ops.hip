@@ -500,12 +500,19 @@ __global__ void k_prepare_sites(const float *__restrict__ xy, int S, int n, int 
     atomicAdd(&count[(long)y * R + x], weight);
 }
 
-__global__ void k_prepare_finish(const float *__restrict__ count, int R, int C,
+// channel 1: the site counts averaged over a (2h+1)^2 window (zero outside
+// the grid, divided by the full window area), so the channel holds a local
+// site density of about the same noise level at any site count
+__global__ void k_prepare_finish(const float *__restrict__ count, int R, int C, int h,
                                  uint16_t *__restrict__ out)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)R * R) return;
-    out[i * C + 1] = f2bf(count[i]);
+    const int x = (int)(i % R), y = (int)(i / R);
+    float acc = 0.0f;
+    for (int yy = max(y - h, 0); yy <= min(y + h, R - 1); ++yy)
+        for (int xx = max(x - h, 0); xx <= min(x + h, R - 1); ++xx) acc += count[(long)yy * R + xx];
+    out[i * C + 1] = f2bf(acc / (float)((2 * h + 1) * (2 * h + 1)));
 }
 
 // ---------------------------------------------------------------------------
@@ -615,6 +622,12 @@ void nn_adam(hipStream_t s, float *w, const float *g, float *m1, float *m2, long
     hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, s, w, g, m1, m2, n, lr, b1, b2, eps, c1, c2);
 }
 
+int nn_splat_halfwidth(int R, int S)
+{
+    if (S <= 0) return 0;
+    return std::max(0, (int)lround((double)R / sqrt((double)S)) - 1);
+}
+
 void nn_density_inv_mean(hipStream_t s, const float *rho, int n, double *part, float *inv_mean)
 {
     hipLaunchKernelGGL(k_rho_partial, dim3(NN_SUM_BLOCKS), dim3(256), 0, s, rho, (long)n * n, part);
@@ -631,7 +644,8 @@ void nn_prepare(hipStream_t s, const float *rho, int n, const float *inv_mean, c
     if (S > 0)
         hipLaunchKernelGGL(k_prepare_sites, dim3(nblk(S)), dim3(256), 0, s, xy, S, n, R, C,
                            (float)R * R / (float)S, count);
-    hipLaunchKernelGGL(k_prepare_finish, dim3(nblk((long)R * R)), dim3(256), 0, s, count, R, C, out);
+    hipLaunchKernelGGL(k_prepare_finish, dim3(nblk((long)R * R)), dim3(256), 0, s, count, R, C,
+                       nn_splat_halfwidth(R, S), out);
 }
 
 void nn_apply_disp(hipStream_t s, float2 *sites, const uint8_t *fixed, int S, const float *field,

@@ -4,7 +4,9 @@
 // displacement field), BF16 weights and activations with FP32 accumulation on
 // MFMA, FP32 master weights and Adam.  Input: an R x R grid (R = 256 for the
 // warm start) of channel 0 = density box mean / mean, channel 1 = initial
-// site count per cell * R^2 / S, channel 2 = 1 where the cell has density.
+// site count per cell * R^2 / S averaged over a window of about three sites
+// (half-width round(R / sqrt(S)) - 1 cells), channel 2 = 1 where the cell
+// has density.
 // Output: the displacement of a site (in units of the domain side) sampled
 // bilinearly from the field at the site's position.
 //
