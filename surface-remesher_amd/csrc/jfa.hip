@@ -286,17 +286,44 @@ __device__ __forceinline__ void eval_item(const Item<MODE> &it, const MapGeo &m,
     const float py = (float)y + 0.5f;
     uint32_t res[4];
     if constexpr (MODE == STEP_ANY) {
+        // a tap row or column that lies off the grid for every lane of the
+        // wave was redirected to the centre: its candidates duplicate the
+        // centre's and cannot change a minimum, so they are not evaluated
+        // (the test is per item; items with every side on the grid, almost
+        // all of them below step 256, take the branch-free path)
+        const bool up = y - m.step >= 0, down = y + m.step < m.H;
+        const bool left = __ballot(xc - m.step >= 0) != 0ull;
+        const bool right = __ballot(xc + 3 + m.step < m.W) != 0ull;
+        if (up && down && left && right) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float px = px0 + (float)j;
-            double best = cand(tab(q_el(it.q[4], j)), q_el(it.q[4], j), px, py);
+            for (int j = 0; j < 4; ++j) {
+                const float px = px0 + (float)j;
+                double best = cand(tab(q_el(it.q[4], j)), q_el(it.q[4], j), px, py);
 #pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                if (t == 4) continue;
-                const uint32_t l = q_el(it.q[t], j);
-                best = __builtin_fmin(best, cand(tab(l), l, px, py));
+                for (int t = 0; t < 9; ++t) {
+                    if (t == 4) continue;
+                    const uint32_t l = q_el(it.q[t], j);
+                    best = __builtin_fmin(best, cand(tab(l), l, px, py));
+                }
+                res[j] = (uint32_t)__double2loint(best);
             }
-            res[j] = (uint32_t)__double2loint(best);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float px = px0 + (float)j;
+                double best = cand(tab(q_el(it.q[4], j)), q_el(it.q[4], j), px, py);
+#pragma unroll
+                for (int t = 0; t < 9; ++t) {
+                    if (t == 4) continue;
+                    const int tr = t / 3, tc = t % 3;
+                    if ((tr == 0 && !up) || (tr == 2 && !down) || (tc == 0 && !left) ||
+                        (tc == 2 && !right))
+                        continue;
+                    const uint32_t l = q_el(it.q[t], j);
+                    best = __builtin_fmin(best, cand(tab(l), l, px, py));
+                }
+                res[j] = (uint32_t)__double2loint(best);
+            }
         }
     } else {
         constexpr int NP = (MODE == STEP_1) ? 6 : 8;

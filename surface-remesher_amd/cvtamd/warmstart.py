@@ -1,4 +1,5 @@
-"""Host API of the learned warm start (cvt_unet_* of include/cvt.h)."""
+"""Host API of the learned warm start (cvt_unet_* of include/cvt.h) and the
+judged warm start: plain Lloyd decides whether the network's sites are used."""
 from __future__ import annotations
 
 import ctypes as C
@@ -111,3 +112,41 @@ class UNet:
 
     def synchronize(self):
         call("cvt_unet_synchronize", self._h)
+
+
+TRIAL = 10
+
+
+def warm_start_judged(net: UNet, engine, xy0, trial: int = TRIAL) -> dict:
+    """Warm start with plain Lloyd as the judge and the fallback.
+
+    From the initial sites xy0 (device or host, pixel units), run `trial`
+    plain Lloyd iterations and keep their sites; then, from xy0 again, move
+    the sites by the network (cvt_unet_warm_start) and run `trial`
+    iterations.  The warm-started run continues if its energy is finite and
+    not above the plain one's; otherwise the engine goes back to the plain
+    run's sites.  Either way the engine is left after `trial` iterations of
+    the chosen run, with its history cleared.  Returns {"warm": bool,
+    "e_plain", "e_warm": the energies at the last trial iteration,
+    "trial_energies": the chosen run's per-iteration energies}.
+    """
+    import torch
+    engine.set_sites(xy0)
+    engine.clear_history()
+    engine.lloyd(trial)
+    h_plain = engine.history()[:, 0].copy()
+    dev = xy0.device if hasattr(xy0, "device") else torch.device("cuda")
+    kept = torch.empty((engine.num_sites, 2), dtype=torch.float32, device=dev)
+    engine.sites(out=kept)
+    engine.set_sites(xy0)
+    net.warm_start(engine)
+    engine.clear_history()
+    engine.lloyd(trial)
+    h_warm = engine.history()[:, 0].copy()
+    engine.clear_history()
+    e_plain, e_warm = float(h_plain[-1]), float(h_warm[-1])
+    warm = bool(np.isfinite(e_warm) and e_warm <= e_plain)
+    if not warm:
+        engine.set_sites(kept)
+    return {"warm": warm, "e_plain": e_plain, "e_warm": e_warm,
+            "trial_energies": h_warm if warm else h_plain}

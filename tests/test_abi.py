@@ -103,3 +103,22 @@ def test_batch_patch_rules():
         assert p.n % 64 == 0 and 256 <= p.n <= 2048
         assert 256 <= p.num_sites <= 16384
         assert 4 <= p.meta["components"] <= 12
+
+
+def test_c_host_compiles_links_and_runs(tmp_path):
+    """include/cvt.h is plain C: the documented call sequence compiles with
+    gcc, links to the library and runs (an error report without a GPU)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    from conftest import ROOT
+    lib = ROOT / "surface-remesher_amd" / "lib"
+    exe = tmp_path / "abi_example"
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-I", str(ROOT / "include"),
+                    str(ROOT / "tests" / "c" / "abi_example.c"), "-L", str(lib), "-lcvt_hip",
+                    f"-Wl,-rpath,{lib}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("version ")
