@@ -29,6 +29,22 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "surface-remesher_amd"))
 sys.path.insert(0, str(ROOT))
 
+PMC_SUMMARY = ROOT / "profiles" / "k_jfa_quad_c2_pmc.txt"
+
+
+def pmc_traffic(workload: str):
+    """HBM bytes per JFA launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size), for the
+    workload it was measured on; None otherwise."""
+    try:
+        for line in PMC_SUMMARY.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch") and workload.startswith("c2"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 METRIC = "Lloyd iter/s (2048², 16K sites); time-to-tol: learned init vs plain Lloyd"
 HBM_PEAK_GBS = 8000.0
 
@@ -113,6 +129,63 @@ def time_to_tol_sample(k: int) -> dict:
     return out
 
 
+# U-Net layers (Cin, Cout, level) as built by csrc/unet.hip
+UNET_SPEC = [(32, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 96, 2), (96, 96, 2),
+             (96, 128, 3), (128, 128, 3), (224, 96, 2), (96, 96, 2), (160, 64, 1), (64, 64, 1),
+             (96, 32, 0), (32, 32, 0)]
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def unet_mfma(batch: int = 16, steps: int = 10) -> dict:
+    """The warm-start network's convolutions on MFMA: forward and training
+    step throughput at 256^2 against the dense bf16 peak.  FLOPs count the
+    3x3 convolutions as executed (input channels padded to 32): forward
+    2*H*W*9*Cin*Cout per layer; a training step adds the data gradient
+    (all layers but the first) and the weight gradient."""
+    import numpy as np
+    import torch
+    from cvtamd.warmstart import UNet
+    R = 256
+    fwd = sum(2.0 * (R >> l) ** 2 * 9 * ci * co for ci, co, l in UNET_SPEC)
+    dgrad = fwd - 2.0 * R * R * 9 * 32 * 32
+    train = fwd + dgrad + fwd
+    net = UNet(R, batch, 1)
+    stream = torch.cuda.Stream()
+    net.set_stream(stream.cuda_stream)
+    g = np.random.default_rng(0)
+    x = np.zeros((batch, R, R, 32), np.float32)
+    x[..., :3] = g.random((batch, R, R, 3))
+    net.set_input(np.ascontiguousarray((x.view(np.uint32) >> 16).astype(np.uint16)), batch)
+    S = 1024
+    uv = torch.from_numpy(g.random((batch, S, 2)).astype(np.float32)).cuda()
+    tg = torch.zeros_like(uv)
+    net.forward(batch)
+    net.loss_backward(batch, uv, tg, S)
+    net.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(stream)
+    for _ in range(steps):
+        net.forward(batch)
+    e1.record(stream)
+    for _ in range(steps):
+        net.forward(batch)
+        net.loss_backward(batch, uv, tg, S, sync=False)
+        net.adam(1e-4)
+    e2.record(stream)
+    torch.cuda.synchronize()
+    f_ms = e0.elapsed_time(e1) / steps
+    t_ms = e1.elapsed_time(e2) / steps
+    net.close()
+    f_tf = fwd * batch / (f_ms * 1e-3) / 1e12
+    t_tf = train * batch / (t_ms * 1e-3) / 1e12
+    return {"batch": batch, "grid": R, "forward_gflop_per_sample": round(fwd / 1e9, 3),
+            "forward_ms": round(f_ms, 4), "forward_tflops": round(f_tf, 2),
+            "train_step_ms": round(t_ms, 4), "train_tflops": round(t_tf, 2),
+            "samples_per_s_train": round(batch / (t_ms * 1e-3), 1),
+            "peak_tflops": BF16_PEAK_TFLOPS, "forward_frac": round(f_tf / BF16_PEAK_TFLOPS, 5),
+            "train_frac": round(t_tf / BF16_PEAK_TFLOPS, 5)}
+
+
 def bench_batch(args):
     """The 64-patch batch: one step = one batched Lloyd iteration over every
     patch (one graph launch); value = batched iterations/s, with the
@@ -120,6 +193,7 @@ def bench_batch(args):
     import numpy as np
     import torch
     import cvtamd
+    from cvtamd import autotune
     from cvtamd import workloads as wl
     ws = wl.config4(args.seed)
     stream = torch.cuda.Stream()
@@ -128,6 +202,7 @@ def bench_batch(args):
     for k, w in enumerate(ws):
         b.set_density(k, w.density)
         b.set_sites(k, w.sites)
+    tiles = autotune.apply_batch(b, [(w.n, w.n) for w in ws], [w.num_sites for w in ws])
     b.lloyd(args.warmup, 1.0)
     b.synchronize()
     torch.cuda.synchronize()
@@ -149,7 +224,8 @@ def bench_batch(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "c4: 64 patches, sides 256..2048, batched", "patches": len(ws),
-                   "pixels": pix, "sites": sum(w.num_sites for w in ws), "seed": args.seed},
+                   "pixels": pix, "sites": sum(w.num_sites for w in ws), "seed": args.seed,
+                   "moment_tiles": {str(t): tiles.count(t) for t in sorted(set(tiles))}},
         "pixels_per_s": round(pix / (ms * 1e-3), 1),
         "iteration_achieved_GBps": round(abytes / (ms * 1e-3) / 1e9, 1),
     }), flush=True)
@@ -170,6 +246,10 @@ def main():
     ap.add_argument("--ttt", type=int, default=4, metavar="K",
                     help="time-to-tolerance of the learned warm start vs plain Lloyd on K "
                          "held-out 2048^2 / 16K patches (0 = skip; needs trained weights)")
+    ap.add_argument("--unet", type=int, default=16, metavar="B",
+                    help="MFMA throughput of the warm-start network at batch B (0 = skip)")
+    ap.add_argument("--unet-only", action="store_true",
+                    help="only the network measurement (for profiling its kernels)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the CVT step, time a plain device copy of MIB MiB "
                          "(calibrates the HBM byte counters against a known size)")
@@ -185,11 +265,15 @@ def main():
     import torch
 
     import cvtamd
+    from cvtamd import autotune
     from cvtamd import workloads as wl
 
     torch.cuda.set_device(0)
     if args.copy_probe:
         return copy_probe(args)
+    if args.unet_only:
+        print(json.dumps({"mfma": unet_mfma(max(args.unet, 1))}), flush=True)
+        return
     if args.workload == "c4":
         return bench_batch(args)
     w = {"c1": wl.config1, "c2": wl.config2, "c3": wl.config3}[args.workload](args.seed)
@@ -200,6 +284,7 @@ def main():
     eng = cvtamd.Engine(n, n, S)
     eng.set_stream(stream.cuda_stream)
     eng.set_density(rho)
+    autotune.apply(eng, n, n, S)          # the tuned moment tile (built-in rule if none)
     eng.set_sites(xy)
     torch.cuda.synchronize()
 
@@ -252,7 +337,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(jfa_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": pmc_traffic(w.name),
             "algorithmic_bytes_per_launch": jfa_bytes,
             "launch_ms": round(prof["jfa_pass"], 6),
             "iteration_achieved": round(iter_gbs, 1),
@@ -263,6 +348,8 @@ def main():
     }
     if args.ttt > 0 and args.workload == "c2":
         out["time_to_tol"] = time_to_tol_sample(args.ttt)
+    if args.unet > 0 and args.workload == "c2":
+        out["mfma"] = unet_mfma(args.unet)
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_seconds)
         out["cpu_baseline"] = cb

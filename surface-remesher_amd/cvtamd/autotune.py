@@ -54,12 +54,14 @@ def apply(engine, W: int, H: int, S: int, table: dict | None = None) -> int:
 
 
 def apply_batch(batch, sizes, sites, table: dict | None = None):
+    """Sets each patch's recorded tile (batched tiles are at least 64: the
+    batched moment pass uses 64-pixel segments); returns the tiles in use."""
     table = load() if table is None else table
     out = []
     for p, ((W, H), S) in enumerate(zip(sizes, sites)):
         t = tile_for(W, H, S, table)
         if t:
-            batch.set_tile(p, t)
+            batch.set_tile(p, max(t, 64))
         out.append(batch.tile(p))
     return out
 
