@@ -12,11 +12,13 @@ namespace cvt {
 void nn_conv3x3(hipStream_t s, const uint16_t *in, const uint16_t *w, const float *bias,
                 uint16_t *out, int N, int H, int W, int Cin, int Cout, int relu);
 // weight gradient dw[9][Cout][Cin] (+= when accumulate) of out-gradient g;
-// work holds nn_dw_splits(M) * 9 * Cout * Cin floats
+// work holds nn_dw_splits(M, Cin, Cout) * 9 * Cout * Cin floats
 void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *dw, float *work,
                    int N, int H, int W, int Cin, int Cout, int accumulate);
-int nn_dw_splits(long M);
-// db[c] = sum_m g[m][c]; work holds nn_dw_splits(M) * C floats
+int nn_dw_splits(long M, int Cin, int Cout);
+// db[c] = sum_m g[m][c]; work holds nn_colsum_splits(M) * C floats
+// splits of the column sum over M rows (its workspace holds splits * C floats)
+int nn_colsum_splits(long M);
 void nn_colsum(hipStream_t s, const uint16_t *g, long M, int C, float *db, float *work);
 // fp32 weights -> bf16 forward copy and flipped/transposed data-gradient copy
 void nn_weights_bf16(hipStream_t s, const float *w, int Cin, int Cout, uint16_t *wf, uint16_t *wb);

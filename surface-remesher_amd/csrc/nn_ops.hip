@@ -30,25 +30,29 @@ __device__ __forceinline__ uint16_t f2bf(float f)
     return (uint16_t)(r >> 16);
 }
 
-#define LDA 40   // LDS row pitch in bf16 (32 + 8 padding)
-
+// Implicit-GEMM 3x3 convolution, NHWC bf16, fp32 accumulation on MFMA:
 // out[m][co] = act(bias[co] + sum_tap sum_ci in[m + off(tap)][ci] * w[tap][co][ci])
-// grid (ceil(M / 64), Cout / (16 NB)), block 256; Cin % 32 == 0, Cout % (16 NB) == 0
-template <int NB>
-__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in,
-                                                 const uint16_t *__restrict__ w,
-                                                 const float *__restrict__ bias,
-                                                 uint16_t *__restrict__ out, int N, int H, int W,
-                                                 int Cin, int Cout, int relu)
+// A block computes 128 pixels x BN output channels (4 waves x 32 pixels;
+// each wave 2 x BN/16 MFMA tiles of 16 x 16); the reduction runs over
+// (tap, 32-channel slice) stages, double-buffered in LDS with the next
+// stage's global loads issued before the current stage's MFMAs.
+// grid (ceil(M / 128), Cout / BN), block 256; Cin % 32 == 0, Cout % BN == 0.
+template <int BN>
+__global__ __launch_bounds__(256) void k_conv3x3_t(const uint16_t *__restrict__ in,
+                                                   const uint16_t *__restrict__ w,
+                                                   const float *__restrict__ bias,
+                                                   uint16_t *__restrict__ out, int N, int H, int W,
+                                                   int Cin, int Cout, int relu)
 {
-    __shared__ uint16_t sA[64 * LDA];
-    __shared__ uint16_t sB[16 * NB * LDA];
+    constexpr int BM = 128, LK = 40;               // pixels per block, LDS row pitch (bf16)
+    __shared__ uint16_t sA[2][BM * LK];
+    __shared__ uint16_t sB[2][BN * LK];
     const long M = (long)N * H * W;
-    const long m0 = (long)blockIdx.x * 64;
-    const int co0 = blockIdx.y * 16 * NB;
+    const long m0 = (long)blockIdx.x * BM;
+    const int co0 = blockIdx.y * BN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    // the pixel this thread stages for A: m_local = t / 4, 8 channels at part * 8
-    const int am = t >> 2, apart = t & 3;
+    // A staging: pixel am = t / 2, channels [16 * (t % 2), +16): two 16-byte words
+    const int am = t >> 1, ahalf = t & 1;
     const long m = m0 + am;
     int an = 0, ay = 0, ax = 0;
     if (m < M) {
@@ -57,107 +61,207 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in
         ay = r / W;
         ax = r - ay * W;
     }
-    f32x4 acc[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < 9; ++tap) {
+    // B staging: output channel bco = t / 4, 8 channels at 8 * (t % 4)
+    const bool bstage = t < BN * 4;
+    const int bco = t >> 2, bpart = t & 3;
+    const int nci = Cin / 32, nstage = 9 * nci;
+    uint4 va0, va1, vb;
+    auto load = [&](int st) {
+        const int tap = st / nci, ci0 = (st - tap * nci) * 32;
         const int yy = ay + tap / 3 - 1, xx = ax + tap % 3 - 1;
-        const bool inside = (m < M) && yy >= 0 && yy < H && xx >= 0 && xx < W;
-        const long src = inside ? (((long)an * H + yy) * W + xx) * Cin : 0;
-        for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
-            __syncthreads();
-            uint4 va = make_uint4(0, 0, 0, 0);
-            if (inside) va = *reinterpret_cast<const uint4 *>(in + src + ci0 + apart * 8);
-            *reinterpret_cast<uint4 *>(&sA[am * LDA + apart * 8]) = va;
-            if (t < 16 * NB * 4) {
-                const int bco = t >> 2, bpart = t & 3;
-                const uint4 vb = *reinterpret_cast<const uint4 *>(
-                    w + ((long)tap * Cout + co0 + bco) * Cin + ci0 + bpart * 8);
-                *reinterpret_cast<uint4 *>(&sB[bco * LDA + bpart * 8]) = vb;
-            }
-            __syncthreads();
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
-                &sA[(wave * 16 + (lane & 15)) * LDA + 8 * (lane >> 4)]);
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const bf16x8 bb = *reinterpret_cast<const bf16x8 *>(
-                    &sB[(b * 16 + (lane & 15)) * LDA + 8 * (lane >> 4)]);
-                acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[b], 0, 0, 0);
-            }
+        va0 = va1 = make_uint4(0, 0, 0, 0);
+        if (m < M && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(
+                in + (((long)an * H + yy) * W + xx) * Cin + ci0 + 16 * ahalf);
+            va0 = src[0];
+            va1 = src[1];
         }
+        if (bstage)
+            vb = *reinterpret_cast<const uint4 *>(w + ((long)tap * Cout + co0 + bco) * Cin + ci0 +
+                                                  8 * bpart);
+    };
+    auto store = [&](int buf) {
+        uint4 *da = reinterpret_cast<uint4 *>(&sA[buf][am * LK + 16 * ahalf]);
+        da[0] = va0;
+        da[1] = va1;
+        if (bstage) *reinterpret_cast<uint4 *>(&sB[buf][bco * LK + 8 * bpart]) = vb;
+    };
+    f32x4 acc[2][BN / 16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int b = 0; b < BN / 16; ++b) acc[i][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int st = 0; st < nstage; ++st) {
+        const bool more = st + 1 < nstage;
+        if (more) load(st + 1);
+        bf16x8 a[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            a[i] = *reinterpret_cast<const bf16x8 *>(
+                &sA[buf][(wave * 32 + i * 16 + (lane & 15)) * LK + 8 * (lane >> 4)]);
+#pragma unroll
+        for (int b = 0; b < BN / 16; ++b) {
+            const bf16x8 bb = *reinterpret_cast<const bf16x8 *>(
+                &sB[buf][(b * 16 + (lane & 15)) * LK + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                acc[i][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb, acc[i][b], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
+    for (int b = 0; b < BN / 16; ++b) {
         const int co = co0 + b * 16 + (lane & 15);
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const long mm = m0 + wave * 16 + 4 * (lane >> 4) + e;
-            if (mm < M) {
-                float v = acc[b][e] + bv;
-                if (relu) v = fmaxf(v, 0.0f);
-                out[mm * Cout + co] = f2bf(v);
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const long mm = m0 + wave * 32 + i * 16 + 4 * (lane >> 4) + e;
+                if (mm < M) {
+                    float v = acc[i][b][e] + bv;
+                    if (relu) v = fmaxf(v, 0.0f);
+                    out[mm * Cout + co] = f2bf(v);
+                }
             }
-        }
     }
 }
 
-// partial weight gradient of one tap, 32 co x 32 ci, over pixels
-// [split * chunk, (split + 1) * chunk): part[split][tap][co][ci]
-// grid (9 * Cout/32 * Cin/32, splits), block 256 (4 waves = 2 x 2 blocks of 16x16)
-__global__ __launch_bounds__(256) void k_conv3x3_dw(const uint16_t *__restrict__ in,
-                                                    const uint16_t *__restrict__ g,
-                                                    float *__restrict__ part, int N, int H, int W,
-                                                    int Cin, int Cout, long chunk)
+// Weight-gradient partials of one tap on a BCO x BCI tile of (co, ci):
+// part[split][tap][co][ci] = sum over the split's pixels m of
+// g[m][co] * in[m + off(tap)][ci].  grid (9 * Cout/BCO * Cin/BCI, splits),
+// block 256.  The reduction runs over pixels in stages of 32; each stage's
+// g and input tiles are staged in LDS transposed to [channel][pixel] (the
+// MFMA operands take 8 consecutive pixels per lane): a thread loads 4
+// pixels x 4 channels and writes 4 eight-byte rows.  The next stage's loads
+// are issued before this stage's MFMAs (two LDS buffers, one barrier per
+// stage).  Requires W % 4 == 0 and chunk % 32 == 0.
+template <int BCO, int BCI>
+__global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict__ in,
+                                                     const uint16_t *__restrict__ g,
+                                                     float *__restrict__ part, int N, int H, int W,
+                                                     int Cin, int Cout, long chunk)
 {
-    __shared__ uint16_t sG[32 * LDA];   // [co][m]
-    __shared__ uint16_t sX[32 * LDA];   // [ci][m]
-    const int nco = Cout / 32, nci = Cin / 32;
+    constexpr int KM = 32, LP = KM + 8;            // pixels per stage, LDS row pitch
+    constexpr int NT = (BCO / 16) * (BCI / 16);    // 16x16 output tiles
+    constexpr int TPW = NT / 4;                    // tiles per wave
+    constexpr int NG = 8 * (BCO / 4), NX = 8 * (BCI / 4);   // staging threads
+    __shared__ uint16_t sG[2][BCO * LP];
+    __shared__ uint16_t sX[2][BCI * LP];
+    const int nco = Cout / BCO, nci = Cin / BCI;
     const int bx = blockIdx.x;
     const int tap = bx / (nco * nci);
-    const int co0 = ((bx / nci) % nco) * 32, ci0 = (bx % nci) * 32;
+    const int co0 = ((bx / nci) % nco) * BCO, ci0 = (bx % nci) * BCI;
     const long M = (long)N * H * W;
     const long mb = (long)blockIdx.y * chunk, me = min(M, mb + chunk);
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int bi = wave >> 1, bj = wave & 1;      // this wave's 16x16 block
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const int ky = tap / 3 - 1, kx = tap % 3 - 1;
-    // staging: thread t handles pixel row mr = t / 8 (32 rows), 4 channels at (t % 8) * 4
-    const int mr = t >> 3, cpart = (t & 7) * 4;
-    for (long mc = mb; mc < me; mc += 32) {
-        __syncthreads();
-        const long mm = mc + mr;
-        uint2 vg = make_uint2(0, 0), vx = make_uint2(0, 0);
-        if (mm < me) {
-            vg = *reinterpret_cast<const uint2 *>(g + mm * Cout + co0 + cpart);
-            const int n = (int)(mm / ((long)H * W));
-            const int r = (int)(mm - (long)n * H * W);
-            const int y = r / W, x = r - (r / W) * W;
-            const int yy = y + ky, xx = x + kx;
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                vx = *reinterpret_cast<const uint2 *>(in + (((long)n * H + yy) * W + xx) * Cin + ci0 + cpart);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    // staging role: G rows (co) or X rows (ci); 4 pixels x 4 channels
+    const bool isG = t < NG, isX = !isG && t < NG + NX;
+    const int tt = isG ? t : t - NG;
+    const int cw = isG ? BCO / 4 : BCI / 4;
+    const int cgrp = (isG || isX) ? tt % cw : 0, pg = (isG || isX) ? tt / cw : 0;
+    const int ch = 4 * cgrp;
+    uint2 v[4];
+    auto load = [&](long mc) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) v[p] = make_uint2(0, 0);
+        const long m = mc + 4 * pg;                 // 4 pixels of one row (W % 4 == 0)
+        if (!(isG || isX) || m >= me) return;
+        const int n = (int)(m / ((long)H * W));
+        const int r = (int)(m - (long)n * H * W);
+        const int y = r / W, x = r - (r / W) * W;
+        if (isG) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                if (m + p < me) v[p] = *reinterpret_cast<const uint2 *>(g + (m + p) * Cout + co0 + ch);
+        } else {
+            const int yy = y + ky;
+            if (yy < 0 || yy >= H) return;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const int xx = x + p + kx;
+                if (m + p < me && xx >= 0 && xx < W)
+                    v[p] = *reinterpret_cast<const uint2 *>(
+                        in + (((long)n * H + yy) * W + xx) * Cin + ci0 + ch);
+            }
         }
-        // transpose into [channel][m]
-        const uint16_t *pg = reinterpret_cast<const uint16_t *>(&vg);
-        const uint16_t *px = reinterpret_cast<const uint16_t *>(&vx);
+    };
+    auto store = [&](int buf) {
+        if (!(isG || isX)) return;
+        uint16_t *dst = isG ? sG[buf] : sX[buf];
+        // channel q of the 4: its values at the 4 pixels, one 8-byte row piece
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            sG[(cpart + q) * LDA + mr] = pg[q];
-            sX[(cpart + q) * LDA + mr] = px[q];
+            const uint32_t e0 = ((q < 2 ? v[0].x : v[0].y) >> (16 * (q & 1))) & 0xffffu;
+            const uint32_t e1 = ((q < 2 ? v[1].x : v[1].y) >> (16 * (q & 1))) & 0xffffu;
+            const uint32_t e2 = ((q < 2 ? v[2].x : v[2].y) >> (16 * (q & 1))) & 0xffffu;
+            const uint32_t e3 = ((q < 2 ? v[3].x : v[3].y) >> (16 * (q & 1))) & 0xffffu;
+            *reinterpret_cast<uint2 *>(&dst[(ch + q) * LP + 4 * pg]) =
+                make_uint2(e0 | (e1 << 16), e2 | (e3 << 16));
         }
+    };
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int k = 0; k < TPW; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int buf = 0;
+    load(mb);
+    store(0);
+    __syncthreads();
+    for (long mc = mb; mc < me; mc += KM) {
+        const bool more = mc + KM < me;
+        if (more) load(mc + KM);
+#pragma unroll
+        for (int k = 0; k < TPW; ++k) {
+            const int tile = wave + 4 * k;
+            const int tco = tile / (BCI / 16), tci = tile % (BCI / 16);
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
+                &sG[buf][(tco * 16 + (lane & 15)) * LP + 8 * (lane >> 4)]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(
+                &sX[buf][(tci * 16 + (lane & 15)) * LP + 8 * (lane >> 4)]);
+            acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[k], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
         __syncthreads();
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sG[(bi * 16 + (lane & 15)) * LDA + 8 * (lane >> 4)]);
-        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sX[(bj * 16 + (lane & 15)) * LDA + 8 * (lane >> 4)]);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        buf ^= 1;
     }
     const long tap_sz = (long)Cout * Cin;
     float *dst = part + (long)blockIdx.y * 9 * tap_sz + (long)tap * tap_sz;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int co = co0 + bi * 16 + 4 * (lane >> 4) + e;
-        const int ci = ci0 + bj * 16 + (lane & 15);
-        dst[(long)co * Cin + ci] = acc[e];
+    for (int k = 0; k < TPW; ++k) {
+        const int tile = wave + 4 * k;
+        const int tco = tile / (BCI / 16), tci = tile % (BCI / 16);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int co = co0 + tco * 16 + 4 * (lane >> 4) + e;
+            const int ci = ci0 + tci * 16 + (lane & 15);
+            dst[(long)co * Cin + ci] = acc[k][e];
+        }
     }
+}
+
+// out[c] = sum_s part[s][c] for few columns and many splits: one block per
+// column, threads take every 256th split, then a fixed-order tree
+__global__ __launch_bounds__(256) void k_sum_cols(const float *__restrict__ part, int splits, int C,
+                                                  float *__restrict__ out)
+{
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float acc = 0.0f;
+    for (int k = threadIdx.x; k < splits; k += 256) acc += part[(long)k * C + c];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
 }
 
 // out[i] = sum_s part[s][i] in split order (deterministic)
@@ -171,15 +275,39 @@ __global__ void k_sum_splits(const float *__restrict__ part, int splits, long n,
     out[i] = accumulate ? out[i] + s : s;
 }
 
-// per-split column sums of g [M][C] -> part[split][C]
+// per-split column sums of g [M][C] (bf16, C % 8 == 0, C <= 256) ->
+// part[split][C]: a thread owns 8 consecutive channels (one 16-byte load per
+// row) and every (256 / (C / 8))-th row of the split; the threads of one
+// channel group are then added in thread order (deterministic)
 __global__ __launch_bounds__(256) void k_colsum(const uint16_t *__restrict__ g, long M, int C,
                                                 long chunk, float *__restrict__ part)
 {
+    __shared__ float red[256][9];
     const long mb = (long)blockIdx.x * chunk, me = min(M, mb + chunk);
-    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    const int cg = C >> 3;                    // channel groups of 8
+    const int rows = 256 / cg;                // rows per block step
+    const int t = threadIdx.x;
+    const int grp = t % cg, r0 = t / cg;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (r0 < rows) {
+        for (long m = mb + r0; m < me; m += rows) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(g + m * C + grp * 8);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                acc[2 * k] += __uint_as_float(w[k] << 16);
+                acc[2 * k + 1] += __uint_as_float(w[k] & 0xffff0000u);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[t][k] = acc[k];
+    __syncthreads();
+    if (t < C) {
+        const int cgp = t >> 3, k = t & 7;
         float s = 0.0f;
-        for (long m = mb; m < me; ++m) s += bf2f(g[m * C + c]);
-        part[(long)blockIdx.x * C + c] = s;
+        for (int r = 0; r < rows; ++r) s += red[r * cg + cgp][k];
+        part[(long)blockIdx.x * C + t] = s;
     }
 }
 
@@ -200,41 +328,88 @@ __global__ void k_weights_bf16(const float *__restrict__ w, int Cin, int Cout,
 }
 
 // g *= (y > 0)
-__global__ void k_relu_bwd(uint16_t *__restrict__ g, const uint16_t *__restrict__ y, long n)
+// Elementwise layers work on 8 channels (one 16-byte word) per thread with
+// 32-bit indices: channel counts are multiples of 8 and every tensor of the
+// network has fewer than 2^31 elements.
+__device__ __forceinline__ void unpack8(const uint4 &v, float f[8])
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && !(bf2f(y[i]) > 0.0f)) g[i] = 0;
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f[2 * k] = __uint_as_float(w[k] << 16);
+        f[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8])
+{
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// g = 0 where the ReLU output y is not positive (n % 8 == 0)
+__global__ void k_relu_bwd(uint16_t *__restrict__ g, const uint16_t *__restrict__ y, int n8)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const uint4 yv = reinterpret_cast<const uint4 *>(y)[i];
+    uint4 gv = reinterpret_cast<const uint4 *>(g)[i];
+    const uint32_t yw[4] = {yv.x, yv.y, yv.z, yv.w};
+    uint32_t gw[4] = {gv.x, gv.y, gv.z, gv.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!(__uint_as_float(yw[k] << 16) > 0.0f)) gw[k] &= 0xffff0000u;
+        if (!(__uint_as_float(yw[k] & 0xffff0000u) > 0.0f)) gw[k] &= 0x0000ffffu;
+    }
+    reinterpret_cast<uint4 *>(g)[i] = make_uint4(gw[0], gw[1], gw[2], gw[3]);
 }
 
 // 2x2 average pool, NHWC (H, W even)
 __global__ void k_avgpool2(const uint16_t *__restrict__ in, uint16_t *__restrict__ out, int N,
                            int H, int W, int C)
 {
-    const int Ho = H / 2, Wo = W / 2;
-    const long n = (long)N * Ho * Wo * C;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = (int)(i % C);
-    const long p = i / C;
-    const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), b = (int)(p / ((long)Wo * Ho));
-    const long base = (((long)b * H + 2 * y) * W + 2 * x) * C + c;
-    const float s = bf2f(in[base]) + bf2f(in[base + C]) + bf2f(in[base + (long)W * C]) +
-                    bf2f(in[base + (long)W * C + C]);
-    out[i] = f2bf(0.25f * s);
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    const int n8 = N * Ho * Wo * C8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const int c8 = i % C8;
+    const int p = i / C8;
+    const int x = p % Wo, y = (p / Wo) % Ho, b = p / (Wo * Ho);
+    const uint4 *src = reinterpret_cast<const uint4 *>(in);
+    const int base = ((b * H + 2 * y) * W + 2 * x) * C8 + c8;
+    float f0[8], f1[8], f2[8], f3[8], o[8];
+    unpack8(src[base], f0);
+    unpack8(src[base + C8], f1);
+    unpack8(src[base + W * C8], f2);
+    unpack8(src[base + W * C8 + C8], f3);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = 0.25f * (f0[k] + f1[k] + f2[k] + f3[k]);
+    reinterpret_cast<uint4 *>(out)[i] = pack8(o);
 }
 
-// gradient of the 2x2 average pool: each input pixel gets a quarter
 __global__ void k_avgpool2_bwd(const uint16_t *__restrict__ gout, uint16_t *__restrict__ gin,
                                int N, int H, int W, int C, int accumulate)
 {
-    const long n = (long)N * H * W * C;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = (int)(i % C);
-    const long p = i / C;
-    const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((long)W * H));
-    const float v = 0.25f * bf2f(gout[(((long)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]);
-    gin[i] = f2bf(accumulate ? bf2f(gin[i]) + v : v);
+    const int C8 = C / 8;
+    const int n8 = N * H * W * C8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const int c8 = i % C8;
+    const int p = i / C8;
+    const int x = p % W, y = (p / W) % H, b = p / (W * H);
+    float v[8], o[8];
+    unpack8(reinterpret_cast<const uint4 *>(gout)[((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C8 + c8], v);
+    if (accumulate) {
+        float g0[8];
+        unpack8(reinterpret_cast<const uint4 *>(gin)[i], g0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[k] = g0[k] + 0.25f * v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[k] = 0.25f * v[k];
+    }
+    reinterpret_cast<uint4 *>(gin)[i] = pack8(o);
 }
 
 // nearest 2x upsample of a (N, H, W, C1) tensor into channels [0, C1) of a
@@ -242,15 +417,16 @@ __global__ void k_avgpool2_bwd(const uint16_t *__restrict__ gout, uint16_t *__re
 __global__ void k_up_concat(const uint16_t *__restrict__ low, const uint16_t *__restrict__ skip,
                             uint16_t *__restrict__ out, int N, int H, int W, int C1, int C2)
 {
-    const int Ho = 2 * H, Wo = 2 * W, C = C1 + C2;
-    const long n = (long)N * Ho * Wo * C;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = (int)(i % C);
-    const long p = i / C;
-    const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), b = (int)(p / ((long)Wo * Ho));
-    out[i] = (c < C1) ? low[(((long)b * H + y / 2) * W + x / 2) * C1 + c]
-                      : skip[p * C2 + (c - C1)];
+    const int Ho = 2 * H, Wo = 2 * W, C8 = (C1 + C2) / 8, L8 = C1 / 8, S8 = C2 / 8;
+    const int n8 = N * Ho * Wo * C8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const int c8 = i % C8;
+    const int p = i / C8;
+    const int x = p % Wo, y = (p / Wo) % Ho, b = p / (Wo * Ho);
+    reinterpret_cast<uint4 *>(out)[i] =
+        (c8 < L8) ? reinterpret_cast<const uint4 *>(low)[((b * H + y / 2) * W + x / 2) * L8 + c8]
+                  : reinterpret_cast<const uint4 *>(skip)[p * S8 + (c8 - L8)];
 }
 
 // gradient of k_up_concat: 2x2 sums into the low tensor, and the skip part
@@ -259,24 +435,37 @@ __global__ void k_up_concat_bwd(const uint16_t *__restrict__ gout, uint16_t *__r
                                 uint16_t *__restrict__ gskip, int N, int H, int W, int C1, int C2,
                                 int acc_skip)
 {
-    const int Ho = 2 * H, Wo = 2 * W, C = C1 + C2;
-    const long nl = (long)N * H * W * C1;
-    const long ns = (long)N * Ho * Wo * C2;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = 2 * H, Wo = 2 * W, C8 = (C1 + C2) / 8, L8 = C1 / 8, S8 = C2 / 8;
+    const int nl = N * H * W * L8;
+    const int ns = N * Ho * Wo * S8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint4 *go = reinterpret_cast<const uint4 *>(gout);
     if (i < nl) {
-        const int c = (int)(i % C1);
-        const long p = i / C1;
-        const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((long)W * H));
-        const long o = (((long)b * Ho + 2 * y) * Wo + 2 * x) * C + c;
-        const float s = bf2f(gout[o]) + bf2f(gout[o + C]) + bf2f(gout[o + (long)Wo * C]) +
-                        bf2f(gout[o + (long)Wo * C + C]);
-        glow[i] = f2bf(s);
+        const int c8 = i % L8;
+        const int p = i / L8;
+        const int x = p % W, y = (p / W) % H, b = p / (W * H);
+        const int o = ((b * Ho + 2 * y) * Wo + 2 * x) * C8 + c8;
+        float f0[8], f1[8], f2[8], f3[8], r[8];
+        unpack8(go[o], f0);
+        unpack8(go[o + C8], f1);
+        unpack8(go[o + Wo * C8], f2);
+        unpack8(go[o + Wo * C8 + C8], f3);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] = f0[k] + f1[k] + f2[k] + f3[k];
+        reinterpret_cast<uint4 *>(glow)[i] = pack8(r);
     }
     if (i < ns) {
-        const int c = (int)(i % C2);
-        const long p = i / C2;
-        const float v = bf2f(gout[p * C + C1 + c]);
-        gskip[i] = f2bf(acc_skip ? bf2f(gskip[i]) + v : v);
+        const int c8 = i % S8;
+        const int p = i / S8;
+        float v[8];
+        unpack8(go[p * C8 + L8 + c8], v);
+        if (acc_skip) {
+            float g0[8];
+            unpack8(reinterpret_cast<const uint4 *>(gskip)[i], g0);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = g0[k] + v[k];
+        }
+        reinterpret_cast<uint4 *>(gskip)[i] = pack8(v);
     }
 }
 
@@ -525,33 +714,55 @@ void nn_conv3x3(hipStream_t s, const uint16_t *in, const uint16_t *w, const floa
 {
     const long M = (long)N * H * W;
     if (Cout % 64 == 0)
-        hipLaunchKernelGGL(k_conv3x3<4>, dim3(nblk(M, 64), Cout / 64), dim3(256), 0, s, in, w, bias,
-                           out, N, H, W, Cin, Cout, relu);
+        hipLaunchKernelGGL(k_conv3x3_t<64>, dim3(nblk(M, 128), Cout / 64), dim3(256), 0, s, in, w,
+                           bias, out, N, H, W, Cin, Cout, relu);
     else
-        hipLaunchKernelGGL(k_conv3x3<2>, dim3(nblk(M, 64), Cout / 32), dim3(256), 0, s, in, w, bias,
-                           out, N, H, W, Cin, Cout, relu);
+        hipLaunchKernelGGL(k_conv3x3_t<32>, dim3(nblk(M, 128), Cout / 32), dim3(256), 0, s, in, w,
+                           bias, out, N, H, W, Cin, Cout, relu);
 }
 
-int nn_dw_splits(long M) { return (int)std::min<long>(64, std::max<long>(1, M / 4096)); }
+// pixel splits of a weight gradient: about 2048 blocks per launch, at most
+// 64 splits and at least 1024 pixels per split
+int nn_dw_splits(long M, int Cin, int Cout)
+{
+    const int bco = (Cout % 64 == 0) ? 64 : 32, bci = (Cin % 64 == 0) ? 64 : 32;
+    const long per_split = 9L * (Cout / bco) * (Cin / bci);
+    const long want = std::max<long>(1, 2048 / per_split);
+    return (int)std::max<long>(1, std::min<long>({64, want, M / 1024}));
+}
 
 void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *dw, float *work,
                    int N, int H, int W, int Cin, int Cout, int accumulate)
 {
     const long M = (long)N * H * W;
-    const int splits = nn_dw_splits(M);
+    const int splits = nn_dw_splits(M, Cin, Cout);
     const long chunk = ((M + splits - 1) / splits + 31) / 32 * 32;
-    hipLaunchKernelGGL(k_conv3x3_dw, dim3(9 * (Cout / 32) * (Cin / 32), splits), dim3(256), 0, s, in,
-                       g, work, N, H, W, Cin, Cout, chunk);
+    const int bco = (Cout % 64 == 0) ? 64 : 32, bci = (Cin % 64 == 0) ? 64 : 32;
+    const dim3 grid(9 * (Cout / bco) * (Cin / bci), splits);
+    if (bco == 64 && bci == 64)
+        hipLaunchKernelGGL((k_conv3x3_dw2<64, 64>), grid, dim3(256), 0, s, in, g, work, N, H, W, Cin,
+                           Cout, chunk);
+    else if (bco == 64)
+        hipLaunchKernelGGL((k_conv3x3_dw2<64, 32>), grid, dim3(256), 0, s, in, g, work, N, H, W, Cin,
+                           Cout, chunk);
+    else if (bci == 64)
+        hipLaunchKernelGGL((k_conv3x3_dw2<32, 64>), grid, dim3(256), 0, s, in, g, work, N, H, W, Cin,
+                           Cout, chunk);
+    else
+        hipLaunchKernelGGL((k_conv3x3_dw2<32, 32>), grid, dim3(256), 0, s, in, g, work, N, H, W, Cin,
+                           Cout, chunk);
     const long n = 9L * Cout * Cin;
     hipLaunchKernelGGL(k_sum_splits, dim3(nblk(n)), dim3(256), 0, s, work, splits, n, dw, accumulate);
 }
 
+int nn_colsum_splits(long M) { return (int)std::min<long>(1024, std::max<long>(1, M / 1024)); }
+
 void nn_colsum(hipStream_t s, const uint16_t *g, long M, int C, float *db, float *work)
 {
-    const int splits = nn_dw_splits(M);
+    const int splits = nn_colsum_splits(M);
     const long chunk = (M + splits - 1) / splits;
     hipLaunchKernelGGL(k_colsum, dim3(splits), dim3(256), 0, s, g, M, C, chunk, work);
-    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(C)), dim3(256), 0, s, work, splits, (long)C, db, 0);
+    hipLaunchKernelGGL(k_sum_cols, dim3(C), dim3(256), 0, s, work, splits, C, db);
 }
 
 void nn_weights_bf16(hipStream_t s, const float *w, int Cin, int Cout, uint16_t *wf, uint16_t *wb)
@@ -561,33 +772,33 @@ void nn_weights_bf16(hipStream_t s, const float *w, int Cin, int Cout, uint16_t 
 
 void nn_relu_bwd(hipStream_t s, uint16_t *g, const uint16_t *y, long n)
 {
-    hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n)), dim3(256), 0, s, g, y, n);
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n / 8)), dim3(256), 0, s, g, y, (int)(n / 8));
 }
 
 void nn_avgpool2(hipStream_t s, const uint16_t *in, uint16_t *out, int N, int H, int W, int C)
 {
-    hipLaunchKernelGGL(k_avgpool2, dim3(nblk((long)N * (H / 2) * (W / 2) * C)), dim3(256), 0, s, in,
+    hipLaunchKernelGGL(k_avgpool2, dim3(nblk((long)N * (H / 2) * (W / 2) * C / 8)), dim3(256), 0, s, in,
                        out, N, H, W, C);
 }
 
 void nn_avgpool2_bwd(hipStream_t s, const uint16_t *gout, uint16_t *gin, int N, int H, int W, int C,
                      int accumulate)
 {
-    hipLaunchKernelGGL(k_avgpool2_bwd, dim3(nblk((long)N * H * W * C)), dim3(256), 0, s, gout, gin, N,
+    hipLaunchKernelGGL(k_avgpool2_bwd, dim3(nblk((long)N * H * W * C / 8)), dim3(256), 0, s, gout, gin, N,
                        H, W, C, accumulate);
 }
 
 void nn_up_concat(hipStream_t s, const uint16_t *low, const uint16_t *skip, uint16_t *out, int N,
                   int H, int W, int C1, int C2)
 {
-    hipLaunchKernelGGL(k_up_concat, dim3(nblk((long)N * 4 * H * W * (C1 + C2))), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_up_concat, dim3(nblk((long)N * 4 * H * W * (C1 + C2) / 8)), dim3(256), 0, s,
                        low, skip, out, N, H, W, C1, C2);
 }
 
 void nn_up_concat_bwd(hipStream_t s, const uint16_t *gout, uint16_t *glow, uint16_t *gskip, int N,
                       int H, int W, int C1, int C2, int acc_skip)
 {
-    const long n = std::max((long)N * H * W * C1, (long)N * 4 * H * W * C2);
+    const long n = std::max((long)N * H * W * C1, (long)N * 4 * H * W * C2) / 8;
     hipLaunchKernelGGL(k_up_concat_bwd, dim3(nblk(n)), dim3(256), 0, s, gout, glow, gskip, N, H, W,
                        C1, C2, acc_skip);
 }

@@ -230,7 +230,10 @@ int cvt_unet_create(cvt_unet **out, int R, int max_batch, unsigned long long see
     // split-K workspace: the largest conv weight gradient and the head partials
     long wn = 256L * (2 * HEAD_C + 2);
     for (int l = 0; l < 14; ++l)
-        wn = std::max(wn, (long)nn_dw_splits(u->npix(SPEC[l].level, B)) * 9L * SPEC[l].cin * SPEC[l].cout);
+        wn = std::max({wn,
+                       (long)nn_dw_splits(u->npix(SPEC[l].level, B), SPEC[l].cin, SPEC[l].cout) * 9L *
+                           SPEC[l].cin * SPEC[l].cout,
+                       (long)nn_colsum_splits(u->npix(SPEC[l].level, B)) * SPEC[l].cout});
     u->work_n = wn;
     alloc((void **)&u->work, sizeof(float) * wn);
     alloc((void **)&u->count, sizeof(float) * R * R);
