@@ -1,0 +1,62 @@
+"""CPU checks of the host-side logic around the kernels: the tile tuner's
+table keys and lookups, and the time-to-tolerance summary."""
+import json
+
+import numpy as np
+import pytest
+
+
+def test_autotune_key_buckets_site_density():
+    from cvtamd import autotune as at
+    # half-octave buckets of pixels per site
+    assert at.key(2048, 2048, 16384) == "2048x2048:16"
+    assert at.key(2048, 2048, 16000) == at.key(2048, 2048, 16384)
+    assert at.key(2048, 2048, 8192) == "2048x2048:18"
+    assert at.key(512, 512, 0) == at.key(512, 512, 1)
+
+
+def test_autotune_table_lookup_and_batch_clamp(tmp_path):
+    from cvtamd import autotune as at
+    table = {at.key(512, 512, 1024): {"tile": 32}, at.key(1024, 1024, 2048): {"tile": 128}}
+    p = tmp_path / "tiles.json"
+    p.write_text(json.dumps(table))
+    t = at.load(p)
+    assert at.tile_for(512, 512, 1024, t) == 32
+    assert at.tile_for(1024, 1024, 2048, t) == 128
+    assert at.tile_for(2048, 2048, 16384, t) is None
+
+    class FakeBatch:
+        def __init__(self):
+            self.t = {}
+
+        def set_tile(self, p, tile):
+            self.t[p] = tile
+
+        def tile(self, p):
+            return self.t.get(p, 64)
+
+    b = FakeBatch()
+    got = at.apply_batch(b, [(512, 512), (1024, 1024), (2048, 2048)], [1024, 2048, 16384], t)
+    # a recorded 32 becomes 64 in a batch (64-pixel segments); unknown keys keep the default
+    assert got == [64, 128, 64]
+
+
+def test_shipped_tile_table_is_consistent():
+    from cvtamd import autotune as at
+    t = at.load()
+    assert t, "the tuned table ships with the package"
+    for k, v in t.items():
+        assert v["tile"] in (32, 64, 128, 256)
+        assert min(v["ms"], key=lambda s: v["ms"][s]) == str(v["tile"])
+
+
+def test_time_to_tol_summary():
+    from cvtamd import eval_warmstart as ev
+    rows = [{"ratio": r, "fallback": f, "plain_iters": 1000, "learned_iters": 20,
+             "plain_s": 0.2, "learned_s": 0.2 / r if r else 0.3}
+            for r, f in ((50.0, False), (40.0, False), (None, True), (60.0, False))]
+    s = ev.summarize(rows)
+    assert s["patches"] == 4 and s["both_reached"] == 3
+    assert s["fallback_rate"] == pytest.approx(0.25)
+    assert s["median_ratio"] == pytest.approx(50.0)
+    assert s["p10_ratio"] == pytest.approx(np.percentile([50.0, 40.0, 60.0], 10))
