@@ -49,8 +49,21 @@ METRIC = "Lloyd iter/s (2048², 16K sites); time-to-tol: learned init vs plain L
 HBM_PEAK_GBS = 8000.0
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, seconds: float):
-    """Times the oracle's Lloyd on the host (the 'port' baseline)."""
+    """Times the oracle's Lloyd on the host (the 'port' baseline): 3 warm-up
+    and up to 20 timed iterations per variant (JFA(+1) and exact nearest
+    site), each timed alone; the median per-iteration time of the faster
+    variant gives the rate."""
     import numpy as np
     from oracle import oracle as orc
     threads = min(orc.max_threads(), int(os.environ.get("OMP_NUM_THREADS", "64")))
@@ -58,24 +71,27 @@ def cpu_baseline(w, seconds: float):
     res = {}
     for name, exact in (("jfa", False), ("exact", True)):
         xy = w.sites.copy()
-        orc.lloyd(w.density, xy, 1, exact=exact)           # warm-up
-        t0 = time.perf_counter()
-        k = 0
-        while time.perf_counter() - t0 < seconds / 2 and k < 50:
+        for _ in range(3):
             orc.lloyd(w.density, xy, 1, exact=exact)
-            k += 1
-        dt = time.perf_counter() - t0
-        res[name] = (k / dt, k)
+        times = []
+        t_start = time.perf_counter()
+        while len(times) < 20 and time.perf_counter() - t_start < seconds / 2:
+            t0 = time.perf_counter()
+            orc.lloyd(w.density, xy, 1, exact=exact)
+            times.append(time.perf_counter() - t0)
+        res[name] = (1.0 / float(np.median(times)), len(times))
     best = max(res, key=lambda n: res[n][0])
     return {
         "value": round(res[best][0], 4),
         "unit": "iter/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{w.name}: oracle Lloyd on the host, {res['jfa'][1]} iterations of the "
-                   f"JFA(+1) variant ({res['jfa'][0]:.3f} it/s) and {res['exact'][1]} of the "
-                   f"exact nearest-site variant ({res['exact'][0]:.3f} it/s), OpenMP over "
-                   f"{threads} threads; value = the faster ({best})"),
+        "sample": (f"{w.name}: oracle Lloyd (gcc -O3 -fopenmp -march=x86-64-v3) on the host, "
+                   f"3 warm-up + {res['jfa'][1]} timed iterations of the JFA(+1) variant "
+                   f"({res['jfa'][0]:.3f} it/s) and 3 + {res['exact'][1]} of the exact "
+                   f"nearest-site variant ({res['exact'][0]:.3f} it/s), median per iteration, "
+                   f"OpenMP over {threads} threads of {os.cpu_count()} ({_cpu_model()}); "
+                   f"value = the faster ({best})"),
     }
 
 

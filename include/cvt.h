@@ -219,7 +219,9 @@ int cvt_unet_sample(cvt_unet *net, int slot, const float *uv, int S, float *disp
  * runs, and each free site moves by the field sampled at its position.  A
  * site keeps its position when it is fixed or when the target pixel lies off
  * the grid or has zero density.  Uses batch slot 0.  Plain Lloyd continues
- * from the moved sites; the caller keeps the old sites to fall back to. */
+ * from the moved sites; the caller keeps the old sites to fall back to.
+ * A network object serves one call at a time (its input buffers are
+ * shared): concurrent warm starts need one network each. */
 int cvt_unet_warm_start(cvt_unet *net, cvt_ctx *ctx);
 /* Weights as a raw bf16 blob ("CVTUNET1", R, count, values). */
 int cvt_unet_save(cvt_unet *net, const char *path);

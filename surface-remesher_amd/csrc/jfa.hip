@@ -29,6 +29,14 @@
 
 namespace cvt {
 
+// Cache policy of the map stores: device scope (sc1).  The pass's output is
+// read by every XCD in the next pass; device-scope stores write the lines
+// through to the device's point of coherence while the pass runs, instead of
+// each XCD's L2 writing its dirty lines back when the kernel ends.  Measured
+// at 2048^2 / 16K sites: 14.0 us per pass against 14.9 with default stores
+// (non-temporal stores were slower, 15.5).
+#define CVT_MAP_STORE_POLICY 16
+
 __device__ __forceinline__ double jfa_key(float d, uint32_t label)
 {
     return __hiloint2double(__float_as_int(d), (int)label);
@@ -115,9 +123,11 @@ __global__ __launch_bounds__(256) void k_jfa_pass(const uint32_t *__restrict__ i
             best = (t == 0) ? key : __builtin_fmin(best, key);
         }
         if (x < W) {
-            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)__double2loint(best), rout, o1, s1, 0);
+            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)__double2loint(best), rout, o1, s1,
+                                                  CVT_MAP_STORE_POLICY);
             if (clear)
-                __builtin_amdgcn_raw_buffer_store_b32(none, buf_rsrc(clear, map_bytes), o1, s1, 0);
+                __builtin_amdgcn_raw_buffer_store_b32(none, buf_rsrc(clear, map_bytes), o1, s1,
+                                                      CVT_MAP_STORE_POLICY);
         }
     }
 }
@@ -137,6 +147,7 @@ __global__ __launch_bounds__(256) void k_jfa_pass(const uint32_t *__restrict__ i
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(3))) char lds_cchar;
 typedef const __attribute__((address_space(3))) f32x2 lds_cf32x2;
@@ -371,10 +382,10 @@ __device__ __forceinline__ void eval_item(const Item<MODE> &it, const MapGeo &m,
         const uint32_t o1 = 4u * (uint32_t)xc;
         const uint32_t s1 = row_off(m, y);
         const u32x4 r4 = {res[0], res[1], res[2], res[3]};
-        __builtin_amdgcn_raw_buffer_store_b128(r4, m.rout, o1, s1, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(r4, m.rout, o1, s1, CVT_MAP_STORE_POLICY);
         if (m.clear) {
             const u32x4 n4 = {m.none, m.none, m.none, m.none};
-            __builtin_amdgcn_raw_buffer_store_b128(n4, m.rclr, o1, s1, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(n4, m.rclr, o1, s1, CVT_MAP_STORE_POLICY);
         }
     }
 }

@@ -64,7 +64,7 @@ def run_to(e, target, cap, sync):
     return done, False
 
 
-def time_to_tol(e, net, rho, xy0, cap=CAP):
+def time_to_tol(e, net, rho, xy0, cap=CAP, trial=TRIAL):
     """One patch: plain and learned wall times to E* (1 + TOL)."""
     import torch
     e.set_density(rho)
@@ -87,13 +87,13 @@ def time_to_tol(e, net, rho, xy0, cap=CAP):
     e.clear_history()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    j = warm_start_judged(net, e, xy0, TRIAL)
+    j = warm_start_judged(net, e, xy0, trial)
     hit = np.nonzero(j["trial_energies"] <= target)[0]
     if len(hit):
         it_l, ok_l = int(hit[0]) + 1, True
     else:
-        it, ok_l = run_to(e, target, cap - TRIAL, e.synchronize)
-        it_l = TRIAL + it
+        it, ok_l = run_to(e, target, cap - trial, e.synchronize)
+        it_l = trial + it
     t_learned = time.perf_counter() - t0
     fallback = not j["warm"]
     e_init, e_warm = j["e_plain"], j["e_warm"]
@@ -137,6 +137,8 @@ def main(argv=None):
     ap.add_argument("--first", type=int, default=0)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--out", type=Path, default=None)
+    ap.add_argument("--trial", type=int, default=TRIAL,
+                    help="Lloyd iterations of each judged trial (plain and warm-started)")
     args = ap.parse_args(argv)
 
     import multiprocessing as mp
@@ -155,7 +157,7 @@ def main(argv=None):
     for idx, rho, xy in pool.imap(_make, range(args.first, args.first + args.seeds)):
         rho_d = torch.from_numpy(rho).cuda()
         xy_d = torch.from_numpy(xy).cuda()
-        res = time_to_tol(eng, net, rho_d, xy_d)
+        res = time_to_tol(eng, net, rho_d, xy_d, trial=args.trial)
         res["index"] = idx
         rows.append(res)
         print(json.dumps(res), flush=True)
