@@ -7,12 +7,14 @@
 // own sentinel at site_off + cap).  Labels are byte offsets into the whole
 // site table, so one label space serves all patches.
 //
-// One Lloyd iteration of the whole batch is Pmax JFA launches (patches run
-// their passes right-aligned), one moment launch over the tiles of every
-// patch, one update launch over the site blocks of every patch and one
-// finalize launch (one block per patch); it is captured once into a hipGraph.
-// Each patch's results equal a single-patch run on the same inputs: labels
-// bit for bit, moments bit for bit with the same moment tile (integer sums).
+// One Lloyd iteration of the whole batch is Pmax JFA passes (patches run
+// their passes right-aligned; a pass is one launch per step class present in
+// it), one moment launch over the tiles of every patch, one update launch
+// over the site blocks of every patch and one finalize launch (one block per
+// patch); it is captured once into a hipGraph.  Each patch's results equal a
+// single-patch run on the same inputs bit for bit: labels, moments (integer
+// sums over the same 64-pixel runs for every tile of side >= 64) and the
+// energy history (the same block partials added in the same order).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>

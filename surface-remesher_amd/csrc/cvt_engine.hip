@@ -3,10 +3,11 @@
 // One cvt_ctx owns every device buffer of one grid for its whole life (the
 // reference allocates and frees about eleven N^2 buffers per gCVT call,
 // source/gcvt.cu:840-903).  A Lloyd iteration is P JFA launches, one moment
-// launch, one overflow launch, one update launch and one finalize launch; it
-// is captured once into a hipGraph and replayed, so the host does no work per
-// iteration and never synchronises inside the loop (the reference copies the
-// energy back every 10th iteration, source/gcvt.cu:1080).
+// launch and one update launch (whose last block also adds the energy
+// partials and appends to the history); it is captured once into a hipGraph
+// and replayed, so the host does no work per iteration and never
+// synchronises inside the loop (the reference copies the energy back every
+// 10th iteration, source/gcvt.cu:1080).
 #include "cvt_engine.h"
 #include "cvt_kernels.h"
 
@@ -90,7 +91,7 @@ static int launch_moments_on(Ctx *c, const uint32_t *lab)
     return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "moment launch failed");
 }
 
-// moments with an event between the tile kernel and the overflow kernel
+// moment tile side (0: from the site density) and the overflow pool it needs
 static int set_tile_internal(Ctx *c, int tile)
 {
     if (tile == 0) tile = choose_tile(c->W, c->H, c->S);
