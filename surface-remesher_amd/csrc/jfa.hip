@@ -134,15 +134,16 @@ __global__ __launch_bounds__(256) void k_jfa_pass(const uint32_t *__restrict__ i
 
 // ---------------------------------------------------------------------------
 // Quad pass.  Each lane owns 4 consecutive pixels of a row, a wave takes one
-// 256-pixel row segment per item, and persistent workgroups own bands of
-// rows.  The code is specialised on the step:
+// 256-pixel row segment per item, and persistent workgroups own a share of
+// the items: a band of rows (quad_rows), or runs of four rows one step apart
+// whose items share tap rows (quad_runs, for the small steps).  The code is
+// specialised on the step:
 //   STEP_ANY (>= 4): nine tap quads; every (pixel, tap) pair is distinct.
 //   STEP_1 / STEP_2: the three tap columns of a row overlap, so a tap row
 //                    holds 6 (8) distinct positions; their site lookups and
 //                    dy^2 are shared by the 4 pixels.
 // The next item's label loads are issued before the current item is
-// evaluated (two register sets alternate), so each wave keeps one item's
-// loads in flight while it computes.
+// evaluated, so each wave keeps one item's loads in flight while it computes.
 // ---------------------------------------------------------------------------
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
@@ -232,47 +233,76 @@ __device__ __forceinline__ uint32_t row_off(const MapGeo &m, int y)
     return __builtin_amdgcn_readfirstlane((uint32_t)y * m.row_bytes);
 }
 
+// the labels one tap row contributes to a segment: the left, centre and
+// right quads (step >= 4), or the centre quad and its side words
+template <int MODE> struct RowData;
+template <> struct RowData<STEP_ANY> { u32x4 l, c, r; };
+template <> struct RowData<STEP_1> { u32x4 c; uint32_t l, r; };
+template <> struct RowData<STEP_2> { u32x4 c; u32x2 l, r; };
+
+// loads tap row y (already redirected onto the grid) for segment seg
 template <int MODE>
-__device__ __forceinline__ void load_item(Item<MODE> &it, const MapGeo &m, int y, int seg, int lane4)
+__device__ __forceinline__ void load_row(RowData<MODE> &d, const MapGeo &m, int y, int seg,
+                                         int lane4)
 {
     const int xc = min(seg * 256 + lane4, m.W - 4);
     const uint32_t o1 = 4u * (uint32_t)xc;
-    const int st = m.step;
-    const uint32_t s0 = row_off(m, (y - st >= 0) ? y - st : y);
-    const uint32_t s1 = row_off(m, y);
-    const uint32_t s2 = row_off(m, (y + st < m.H) ? y + st : y);
+    const uint32_t s = row_off(m, y);
     if constexpr (MODE == STEP_ANY) {
         // a tap column outside the grid is redirected to the centre column
         // (for step >= 4 the whole quad is in or out together)
+        const int st = m.step;
         const uint32_t o0 = (xc - st >= 0) ? o1 - 4u * (uint32_t)st : o1;
         const uint32_t o2 = (xc + 3 + st < m.W) ? o1 + 4u * (uint32_t)st : o1;
-        it.q[0] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o0, s0, 0);
-        it.q[1] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, s0, 0);
-        it.q[2] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o2, s0, 0);
-        it.q[3] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o0, s1, 0);
-        it.q[4] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, s1, 0);
-        it.q[5] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o2, s1, 0);
-        it.q[6] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o0, s2, 0);
-        it.q[7] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, s2, 0);
-        it.q[8] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o2, s2, 0);
+        d.l = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o0, s, 0);
+        d.c = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, s, 0);
+        d.r = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o2, s, 0);
     } else {
         // side words: the one (two) labels left of the quad and right of it;
         // unused (and replaced by the centre labels) at the grid's edges
         const uint32_t oL = (xc > 0) ? o1 - 4u * MODE : o1;
         const uint32_t oR = (xc + 4 < m.W) ? o1 + 16u : o1;
-        const uint32_t so[3] = {s0, s1, s2};
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            it.c[r] = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, so[r], 0);
-            if constexpr (MODE == STEP_1) {
-                it.l[r] = __builtin_amdgcn_raw_buffer_load_b32(m.rin, oL, so[r], 0);
-                it.r[r] = __builtin_amdgcn_raw_buffer_load_b32(m.rin, oR, so[r], 0);
-            } else {
-                it.l[r] = __builtin_amdgcn_raw_buffer_load_b64(m.rin, oL, so[r], 0);
-                it.r[r] = __builtin_amdgcn_raw_buffer_load_b64(m.rin, oR, so[r], 0);
-            }
+        d.c = __builtin_amdgcn_raw_buffer_load_b128(m.rin, o1, s, 0);
+        if constexpr (MODE == STEP_1) {
+            d.l = __builtin_amdgcn_raw_buffer_load_b32(m.rin, oL, s, 0);
+            d.r = __builtin_amdgcn_raw_buffer_load_b32(m.rin, oR, s, 0);
+        } else {
+            d.l = __builtin_amdgcn_raw_buffer_load_b64(m.rin, oL, s, 0);
+            d.r = __builtin_amdgcn_raw_buffer_load_b64(m.rin, oR, s, 0);
         }
     }
+}
+
+// the item of centre row y from its up, centre and down tap rows
+template <int MODE>
+__device__ __forceinline__ Item<MODE> item_of(const RowData<MODE> &u, const RowData<MODE> &c,
+                                              const RowData<MODE> &d)
+{
+    Item<MODE> it;
+    if constexpr (MODE == STEP_ANY) {
+        it.q[0] = u.l; it.q[1] = u.c; it.q[2] = u.r;
+        it.q[3] = c.l; it.q[4] = c.c; it.q[5] = c.r;
+        it.q[6] = d.l; it.q[7] = d.c; it.q[8] = d.r;
+    } else {
+        it.c[0] = u.c; it.l[0] = u.l; it.r[0] = u.r;
+        it.c[1] = c.c; it.l[1] = c.l; it.r[1] = c.r;
+        it.c[2] = d.c; it.l[2] = d.l; it.r[2] = d.r;
+    }
+    return it;
+}
+
+// tap rows of centre row y: off-grid rows are redirected to the centre row
+__device__ __forceinline__ int up_row(const MapGeo &m, int y) { return (y - m.step >= 0) ? y - m.step : y; }
+__device__ __forceinline__ int down_row(const MapGeo &m, int y) { return (y + m.step < m.H) ? y + m.step : y; }
+
+template <int MODE>
+__device__ __forceinline__ void load_item(Item<MODE> &it, const MapGeo &m, int y, int seg, int lane4)
+{
+    RowData<MODE> u, c, d;
+    load_row<MODE>(u, m, up_row(m, y), seg, lane4);
+    load_row<MODE>(c, m, y, seg, lane4);
+    load_row<MODE>(d, m, down_row(m, y), seg, lane4);
+    it = item_of<MODE>(u, c, d);
 }
 
 __device__ __forceinline__ uint32_t q_el(const u32x4 &q, int j)
@@ -439,32 +469,124 @@ __device__ __forceinline__ MapGeo map_geo(const uint32_t *in, uint32_t *out, uin
     return m;
 }
 
-// band of rows of workgroup g of G: bands are dealt so that each XCD (blocks
-// go to the 8 XCDs round robin) owns a contiguous stretch of rows, keeping
-// the tap rows of small steps inside one XCD's L2
-__device__ __forceinline__ void band_of(int g, int G, int H, int &y0, int &y1)
+// share [a, b) of N work units for workgroup g of G: shares are dealt so
+// that each XCD (blocks go to the 8 XCDs round robin) owns a contiguous
+// stretch of units
+__device__ __forceinline__ void band_of(int g, int G, int N, int &a, int &b)
 {
     const int q8 = G >> 3, r8 = G & 7, xcd = g & 7;
     const int band = xcd * q8 + min(xcd, r8) + (g >> 3);
-    y0 = (int)((long)H * band / G);
-    y1 = (int)((long)H * (band + 1) / G);
+    a = (int)((long)N * band / G);
+    b = (int)((long)N * (band + 1) / G);
+}
+
+// Runs.  The rows of a map fall into chains of rows one step apart
+// (y = c, c + k, c + 2k, ... for c < min(k, H)); consecutive rows of a chain
+// share two of their three tap rows.  A run is four consecutive rows of a
+// chain in one 256-pixel segment, so its items need 6 tap rows instead of 12.
+// Runs are numbered by position in the chain, then chain, then segment: the
+// runs that start together cover neighbouring rows (a contiguous share of
+// runs is a compact band of first rows, as in quad_rows).
+struct RunGeo {
+    int nseg, nq, k, chains;  // segments per row, runs per chain, step, chains
+    int count;                // runs in the map
+};
+
+__device__ __forceinline__ RunGeo run_geo(const MapGeo &m)
+{
+    RunGeo g;
+    g.k = m.step;
+    g.nseg = (m.W + 255) >> 8;
+    g.chains = min(m.step, m.H);
+    const int n = (m.H + m.step - 1) / m.step;     // longest chain
+    g.nq = (n + 3) >> 2;
+    g.count = g.chains * g.nq * g.nseg;
+    return g;
+}
+
+// first centre row and segment of run u (wave-uniform)
+__device__ __forceinline__ void run_of(const RunGeo &g, int u, int &y, int &seg)
+{
+    seg = u % g.nseg;
+    const int qc = u / g.nseg;
+    const int c = qc % g.chains, q = qc / g.chains;
+    y = __builtin_amdgcn_readfirstlane(c + 4 * q * g.k);
+    seg = __builtin_amdgcn_readfirstlane(seg);
+}
+
+// Runs [u0, u1) of one map by all waves of the block, dealt round robin.  Six
+// tap-row buffers rotate through a run: item j (centre y + jk) reads buffers
+// j, j+1, j+2 while the row item j+1 adds is loaded into buffer j+3, and the
+// last item's slot loads the next run's first three rows into buffers 0-2.
+// Rows past the grid end their chain: their items are skipped and their loads
+// not issued.  A wave's first loads are issued before the site table is
+// staged, so the two latencies overlap.
+template <int MODE, int TAB>
+__device__ __forceinline__ void quad_runs(const MapGeo &m, const SiteTab<TAB> &tab, int u0, int u1)
+{
+    const RunGeo g = run_geo(m);
+    const int nwaves = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane4 = 4 * (threadIdx.x & 63);
+    const int k = m.step, H = m.H;
+    RowData<MODE> B0, B1, B2, B3, B4, B5;
+    int u = u0 + wave, y = 0, seg = 0;
+    if (u < u1) {
+        run_of(g, u, y, seg);
+        load_row<MODE>(B0, m, up_row(m, y), seg, lane4);
+        load_row<MODE>(B1, m, y, seg, lane4);
+        load_row<MODE>(B2, m, down_row(m, y), seg, lane4);
+    }
+    tab.stage();
+    for (; u < u1; u += nwaves) {
+        // item 0 (a chain shorter than the longest can leave a run empty)
+        if (y + k < H) load_row<MODE>(B3, m, down_row(m, y + k), seg, lane4);
+        if (y < H) eval_item<MODE, TAB>(item_of<MODE>(B0, B1, B2), m, tab, y, seg, lane4);
+        // item 1
+        if (y + 2 * k < H) load_row<MODE>(B4, m, down_row(m, y + 2 * k), seg, lane4);
+        if (y + k < H) eval_item<MODE, TAB>(item_of<MODE>(B1, B2, B3), m, tab, y + k, seg, lane4);
+        // item 2
+        if (y + 3 * k < H) load_row<MODE>(B5, m, down_row(m, y + 3 * k), seg, lane4);
+        if (y + 2 * k < H) eval_item<MODE, TAB>(item_of<MODE>(B2, B3, B4), m, tab, y + 2 * k, seg, lane4);
+        // item 3, with the next run's first rows
+        const int yc = y + 3 * k, sc = seg;
+        if (u + nwaves < u1) {
+            run_of(g, u + nwaves, y, seg);
+            load_row<MODE>(B0, m, up_row(m, y), seg, lane4);
+            load_row<MODE>(B1, m, y, seg, lane4);
+            load_row<MODE>(B2, m, down_row(m, y), seg, lane4);
+        }
+        if (yc < H) eval_item<MODE, TAB>(item_of<MODE>(B3, B4, B5), m, tab, yc, sc, lane4);
+    }
 }
 
 // Single map, site table staged at LDS byte 0 in a static array of ENTRIES
 // float2 (the kernel's only LDS), or gathered from global memory when
-// ENTRIES == 0.
+// ENTRIES == 0.  RUNS picks the work layout: runs down chains of rows, or
+// bands of rows (quad_rows).
 #define JFA_THREADS(E) ((E) > 5120 ? 1024 : 256)
 #define JFA_MINB(E) ((E) > 5120 ? 1 : 4)
 
-template <int MODE, int ENTRIES>
+template <int MODE, int TAB>
+__device__ __forceinline__ void quad_layout(const MapGeo &m, const SiteTab<TAB> &tab, bool runs)
+{
+    int a, b;
+    if (runs) {
+        band_of(blockIdx.x, gridDim.x, run_geo(m).count, a, b);
+        quad_runs<MODE, TAB>(m, tab, a, b);
+    } else {
+        band_of(blockIdx.x, gridDim.x, m.H, a, b);
+        quad_rows<MODE, TAB>(m, tab, a, b);
+    }
+}
+
+template <int MODE, int ENTRIES, bool RUNS>
 __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa_quad(
     const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
     const float2 *__restrict__ sites, int table_f4, int W, int H, int step,
     uint32_t *__restrict__ clear, uint32_t none)
 {
     const MapGeo m = map_geo(in, out, clear, W, H, step, none);
-    int y0, y1;
-    band_of(blockIdx.x, gridDim.x, H, y0, y1);
     if constexpr (ENTRIES > 0) {
         __shared__ float4 lds_tab[ENTRIES / 2];
         SiteTab<TAB_LDS0> tab;
@@ -472,13 +594,13 @@ __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa
         tab.src = reinterpret_cast<const f32x4 *>(sites);
         tab.n4 = table_f4;
         tab.base = 0;
-        quad_rows<MODE, TAB_LDS0>(m, tab, y0, y1);
+        quad_layout<MODE, TAB_LDS0>(m, tab, RUNS);
     } else {
         SiteTab<TAB_GLOBAL> tab;
         tab.g = buf_rsrc(sites, 16u * (uint32_t)table_f4);
         tab.base = 0;
         tab.n4 = 0;
-        quad_rows<MODE, TAB_GLOBAL>(m, tab, y0, y1);
+        quad_layout<MODE, TAB_GLOBAL>(m, tab, RUNS);
     }
 }
 
@@ -607,38 +729,48 @@ JfaPlan plan_jfa(int W, int H, int capS)
     return p;
 }
 
-template <int MODE>
+template <int MODE, bool RUNS>
 static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
                         const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none)
 {
     const dim3 g(p.grid), b(p.threads);
     switch (p.lds_entries) {
     case 2048:
-        hipLaunchKernelGGL((k_jfa_quad<MODE, 2048>), g, b, 0, st, in, out, sites, p.table_f4, W, H,
-                           step, clear, none);
+        hipLaunchKernelGGL((k_jfa_quad<MODE, 2048, RUNS>), g, b, 0, st, in, out, sites, p.table_f4,
+                           W, H, step, clear, none);
         break;
     case 5120:
-        hipLaunchKernelGGL((k_jfa_quad<MODE, 5120>), g, b, 0, st, in, out, sites, p.table_f4, W, H,
-                           step, clear, none);
+        hipLaunchKernelGGL((k_jfa_quad<MODE, 5120, RUNS>), g, b, 0, st, in, out, sites, p.table_f4,
+                           W, H, step, clear, none);
         break;
     case 20480:
-        hipLaunchKernelGGL((k_jfa_quad<MODE, 20480>), g, b, 0, st, in, out, sites, p.table_f4, W,
-                           H, step, clear, none);
+        hipLaunchKernelGGL((k_jfa_quad<MODE, 20480, RUNS>), g, b, 0, st, in, out, sites,
+                           p.table_f4, W, H, step, clear, none);
         break;
     default:
-        hipLaunchKernelGGL((k_jfa_quad<MODE, 0>), g, b, 0, st, in, out, sites, p.table_f4, W, H,
-                           step, clear, none);
+        hipLaunchKernelGGL((k_jfa_quad<MODE, 0, RUNS>), g, b, 0, st, in, out, sites, p.table_f4, W,
+                           H, step, clear, none);
         break;
     }
 }
+
+// Layout by step, as measured at 2048^2 / 16K sites (us per pass, bands of
+// rows -> runs): steps 32 .. 1 gain (13.9 -> 13.3 at 32, 14.8 -> 11.3 at
+// 1), steps 64 and up do not (14.0 -> 16.2 at 64, 13.5 -> 17.3 at 1024).
+static const int kRunsMaxStep = 32;
 
 void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none)
 {
     if (p.quad) {
-        if (step == 1) launch_quad<STEP_1>(st, p, in, out, sites, W, H, step, clear, none);
-        else if (step == 2) launch_quad<STEP_2>(st, p, in, out, sites, W, H, step, clear, none);
-        else launch_quad<STEP_ANY>(st, p, in, out, sites, W, H, step, clear, none);
+        if (step == 1) launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none);
+        // (step 2 with the global table keeps bands: six buffers of it spill)
+        else if (step == 2 && p.lds_entries > 0)
+            launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none);
+        else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none);
+        else if (step <= kRunsMaxStep)
+            launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none);
+        else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none);
         return;
     }
     constexpr int R = CVT_JFA_ROWS;
