@@ -157,7 +157,10 @@ int Ctx::enqueue_jfa()
     int cur = 0;
     for (int k = 0; k < np; ++k) {
         uint32_t *out = lab[cur];
-        launch_jfa_pass(stream, jplan, in, out, sites, W, H, step, k == 1 ? seed : nullptr, none());
+        // pass 1 resets the seed map pass 0 consumed: it holds entries only at
+        // the seed pixels of sites [0, S) (compaction only shrinks the count)
+        launch_jfa_pass(stream, jplan, in, out, sites, W, H, step, k == 1 ? seed : nullptr, none(),
+                        k == 1 ? S : 0);
         in = out;
         cur ^= 1;
         if (step > 1) step >>= 1;

@@ -19,10 +19,13 @@ JfaPlan plan_jfa(int W, int H, int capS);
 
 // One JFA pass over a W x H map of byte-offset labels (8 * site index, `none`
 // for no site): out(p) = best of the 9 labels at p + step * {-1, 0, 1}^2.
-// If clear is non-null it is reset to `none` pixel by pixel.  The site table
-// must hold capS + 1 entries rounded up to an even count.
+// If clear is non-null it is reset to `none`: at the seed pixels of sites
+// [0, nclear) when nclear > 0 and the map holds nothing elsewhere (quad
+// plans), otherwise pixel by pixel.  The site table must hold capS + 1
+// entries rounded up to an even count.
 void launch_jfa_pass(hipStream_t st, const JfaPlan &plan, const uint32_t *in, uint32_t *out,
-                     const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none);
+                     const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
+                     int nclear = 0);
 
 // A patch of a batch, as the batched kernels see it.  Maps are concatenated
 // (pix_off), site tables too (site_off; entry site_off + cap is the patch's

@@ -580,13 +580,33 @@ __device__ __forceinline__ void quad_layout(const MapGeo &m, const SiteTab<TAB> 
     }
 }
 
+// the pixel a site seeds (clamped to the grid): the seed scatter's rule
+__device__ __forceinline__ int seed_pixel(float v, int n)
+{
+    float f = floorf(v);
+    f = fminf(fmaxf(f, 0.0f), (float)(n - 1));
+    return (int)f;
+}
+
+// nclear > 0: clear is a seed map whose only entries sit at the pixels of
+// sites [0, nclear); they are reset one store per site instead of one per
+// pixel (the pass does not read that map).
 template <int MODE, int ENTRIES, bool RUNS>
 __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa_quad(
     const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
     const float2 *__restrict__ sites, int table_f4, int W, int H, int step,
-    uint32_t *__restrict__ clear, uint32_t none)
+    uint32_t *__restrict__ clear, uint32_t none, int nclear)
 {
-    const MapGeo m = map_geo(in, out, clear, W, H, step, none);
+    if (clear && nclear > 0) {
+        const __amdgpu_buffer_rsrc_t rc = buf_rsrc(clear, 4u * (uint32_t)W * (uint32_t)H);
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nclear;
+             i += gridDim.x * blockDim.x) {
+            const float2 v = sites[i];
+            const int x = seed_pixel(v.x, W), y = seed_pixel(v.y, H);
+            __builtin_amdgcn_raw_buffer_store_b32(none, rc, 4u * (uint32_t)(y * W + x), 0, 0);
+        }
+    }
+    const MapGeo m = map_geo(in, out, nclear > 0 ? nullptr : clear, W, H, step, none);
     if constexpr (ENTRIES > 0) {
         __shared__ float4 lds_tab[ENTRIES / 2];
         SiteTab<TAB_LDS0> tab;
@@ -731,25 +751,26 @@ JfaPlan plan_jfa(int W, int H, int capS)
 
 template <int MODE, bool RUNS>
 static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
-                        const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none)
+                        const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
+                        int nclear)
 {
     const dim3 g(p.grid), b(p.threads);
     switch (p.lds_entries) {
     case 2048:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 2048, RUNS>), g, b, 0, st, in, out, sites, p.table_f4,
-                           W, H, step, clear, none);
+                           W, H, step, clear, none, nclear);
         break;
     case 5120:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 5120, RUNS>), g, b, 0, st, in, out, sites, p.table_f4,
-                           W, H, step, clear, none);
+                           W, H, step, clear, none, nclear);
         break;
     case 20480:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 20480, RUNS>), g, b, 0, st, in, out, sites,
-                           p.table_f4, W, H, step, clear, none);
+                           p.table_f4, W, H, step, clear, none, nclear);
         break;
     default:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 0, RUNS>), g, b, 0, st, in, out, sites, p.table_f4, W,
-                           H, step, clear, none);
+                           H, step, clear, none, nclear);
         break;
     }
 }
@@ -760,17 +781,18 @@ static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, ui
 static const int kRunsMaxStep = 32;
 
 void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
-                     const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none)
+                     const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
+                     int nclear)
 {
     if (p.quad) {
-        if (step == 1) launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none);
+        if (step == 1) launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none, nclear);
         // (step 2 with the global table keeps bands: six buffers of it spill)
         else if (step == 2 && p.lds_entries > 0)
-            launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none);
-        else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none);
+            launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none, nclear);
+        else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none, nclear);
         else if (step <= kRunsMaxStep)
-            launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none);
-        else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none);
+            launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none, nclear);
+        else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none, nclear);
         return;
     }
     constexpr int R = CVT_JFA_ROWS;
