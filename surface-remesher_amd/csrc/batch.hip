@@ -34,7 +34,7 @@ static __global__ __launch_bounds__(256) void k_moments_batch(
     const BatchPatch *__restrict__ patches, const int *__restrict__ tile_patch,
     const uint32_t *__restrict__ lab0, const uint32_t *__restrict__ lab1,
     const float *__restrict__ rho, const float2 *__restrict__ sites, Slot *__restrict__ slots,
-    OvfRec *__restrict__ ovf, int ovf_cap, int *__restrict__ ovf_head, int *__restrict__ dflags,
+    long nrows, OvfRec *__restrict__ ovf, int ovf_cap, int *__restrict__ ovf_head, int *__restrict__ dflags,
     int *__restrict__ dstate)
 {
     __shared__ TileLds<CVT_HASH_CAP> L;
@@ -47,7 +47,7 @@ static __global__ __launch_bounds__(256) void k_moments_batch(
     tile_lds_init(L);
     tile_moments<64>(L, lab, rho + P.pix_off, sites + P.site_off, P.site_off, P.S, P.W, P.H,
                      P.rho_scale, tX * T, tY * T, T);
-    tile_flush(L, sites, P.site_off, T, tX, tY, P.tiles_x, P.tiles_y, dstate[ST_EPOCH], slots, ovf,
+    tile_flush(L, sites, P.site_off, T, tX, tY, P.tiles_x, P.tiles_y, dstate[ST_EPOCH], slots, nrows, ovf,
                ovf_cap, ovf_head, dstate, dflags);
 }
 
@@ -55,7 +55,7 @@ static __global__ __launch_bounds__(256) void k_moments_batch(
 static __global__ __launch_bounds__(256) void k_update_batch(
     const BatchPatch *__restrict__ patches, const int *__restrict__ blk_patch,
     float2 *__restrict__ sites, const uint8_t *__restrict__ fixed, const Slot *__restrict__ slots,
-    const OvfRec *__restrict__ ovf, int *__restrict__ ovf_head, int *__restrict__ dstate,
+    long nrows, const OvfRec *__restrict__ ovf, int *__restrict__ ovf_head, int *__restrict__ dstate,
     float omega, int mode, double *__restrict__ blockpart, uint32_t *__restrict__ seed)
 {
     __shared__ double red_e[256], red_g[256];
@@ -66,7 +66,7 @@ static __global__ __launch_bounds__(256) void k_update_batch(
     if (i < P.S) {
         const int gi = P.site_off + i;
         double w, dx, dy, ee;
-        cvt_site_totals(gi, dstate[ST_EPOCH], slots, ovf, ovf_head, w, dx, dy, ee);
+        cvt_site_totals(gi, dstate[ST_EPOCH], slots, nrows, ovf, ovf_head, w, dx, dy, ee);
         e = ee;
         g = dx * dx + dy * dy;
         float2 s = sites[gi];
@@ -312,9 +312,11 @@ int Batch::enqueue_iteration(float omega)
         launch_jfa_batch(stream, d_patch, d_units, (int)units.size(), seed, lab0, lab1, sites, k,
                          pmax, lds_bytes, pass_modes(k));
     hipLaunchKernelGGL(k_moments_batch, dim3(ntiles), dim3(256), 0, stream, d_patch, d_tile_patch,
-                       lab0, lab1, rho, sites, slots, ovf, ovf_cap, ovf_head, dflags, dstate);
+                       lab0, lab1, rho, sites, slots, (long)total_sites, ovf, ovf_cap, ovf_head,
+                       dflags, dstate);
     hipLaunchKernelGGL(k_update_batch, dim3(nblk), dim3(256), 0, stream, d_patch, d_blk_patch, sites,
-                       fixed, slots, ovf, ovf_head, dstate, omega, UPD_MOVE | UPD_SEED, blockpart,
+                       fixed, slots, (long)total_sites, ovf, ovf_head, dstate, omega,
+                       UPD_MOVE | UPD_SEED, blockpart,
                        seed);
     hipLaunchKernelGGL(k_finalize_batch, dim3(np), dim3(256), 0, stream, d_patch, blockpart,
                        d_escale, hist_len, hist, hist_cap, dstate);

@@ -27,6 +27,8 @@ struct Ctx {
     uint32_t *seed = nullptr;      // seed map [H][W]
     bool seeds_valid = false;      // seed map holds the current sites
     uint32_t *lab[2] = {nullptr, nullptr};
+    uint32_t *first = nullptr;     // first JFA pass [H][W]: none except at the
+                                   // receivers of the current seeds while a JFA runs
     int final_buf = 0;             // lab[final_buf] = labels of the last JFA
     bool labels_fresh = false;
     float2 *sites = nullptr;       // [capS + 1], sentinel at capS
@@ -37,7 +39,7 @@ struct Ctx {
 
     int tile = 0, tilesX = 0, tilesY = 0;
     bool tile_auto = true;
-    Slot *slots = nullptr;         // [maxS][9]
+    Slot *slots = nullptr;         // [9][capS]: slot k of site i at k * capS + i
     OvfRec *ovf = nullptr;         // [ovf_cap] record pool
     int ovf_cap = 0;
     int *ovf_head = nullptr;       // [maxS] first record of each site, -1 none

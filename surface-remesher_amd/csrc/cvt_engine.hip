@@ -69,22 +69,22 @@ static int launch_moments_on(Ctx *c, const uint32_t *lab)
     switch (c->tile) {
     case 32:
         hipLaunchKernelGGL(k_moments<32>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
-                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
+                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, (long)c->capS, c->ovf, c->ovf_cap,
                            c->ovf_head, c->dflags, c->dstate);
         break;
     case 64:
         hipLaunchKernelGGL(k_moments<64>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
-                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
+                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, (long)c->capS, c->ovf, c->ovf_cap,
                            c->ovf_head, c->dflags, c->dstate);
         break;
     case 128:
         hipLaunchKernelGGL(k_moments<128>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
-                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
+                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, (long)c->capS, c->ovf, c->ovf_cap,
                            c->ovf_head, c->dflags, c->dstate);
         break;
     default:
         hipLaunchKernelGGL(k_moments<256>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,
-                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, c->ovf, c->ovf_cap,
+                           c->W, c->H, rs, c->tilesX, c->tilesY, c->slots, (long)c->capS, c->ovf, c->ovf_cap,
                            c->ovf_head, c->dflags, c->dstate);
         break;
     }
@@ -155,12 +155,24 @@ int Ctx::enqueue_jfa()
     int step = jfa_first_step(W, H);
     const uint32_t *in = seed;
     int cur = 0;
+    // Pass 0 reads only the seed map: with three passes or more it scatters
+    // the seeds into `first`, which holds none except at their receivers.
+    // Pass 1 resets the seed map it no longer needs, pass 2 resets `first`:
+    // both hold entries only around the seed pixels of sites [0, S) (the
+    // sites do not move during a JFA; compaction only shrinks the count).
+    const bool scatter = np >= 3;
+    const int step0 = step;
     for (int k = 0; k < np; ++k) {
+        if (k == 0 && scatter) {
+            launch_jfa_seed_pass(stream, seed, first, sites, S, W, H, step);
+            in = first;
+            if (step > 1) step >>= 1;
+            continue;
+        }
         uint32_t *out = lab[cur];
-        // pass 1 resets the seed map pass 0 consumed: it holds entries only at
-        // the seed pixels of sites [0, S) (compaction only shrinks the count)
-        launch_jfa_pass(stream, jplan, in, out, sites, W, H, step, k == 1 ? seed : nullptr, none(),
-                        k == 1 ? S : 0);
+        uint32_t *clear = (k == 1) ? seed : (k == 2 && scatter) ? first : nullptr;
+        launch_jfa_pass(stream, jplan, in, out, sites, W, H, step, clear, none(),
+                        clear ? S : 0, k == 2 ? step0 : 0);
         in = out;
         cur ^= 1;
         if (step > 1) step >>= 1;
@@ -182,7 +194,7 @@ int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, doub
     fa.e_scale = ldexp(1.0, rho_exp) / a;
     fa.g_scale = 2.0 * ldexp(1.0, rho_exp) / a;
     fa.out_eg = out_eg;
-    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, ovf,
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, (long)capS, ovf,
                        ovf_head, dflags, dstate, omega, domega, mode, ldexp(1.0, rho_exp), mom_out,
                        blockpart, seed, W, H, ca, fa);
     if (mode & UPD_SEED) seeds_valid = true;
@@ -246,7 +258,7 @@ Ctx::~Ctx()
 {
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
-    void *bufs[] = {rho, seed, lab[0], lab[1], sites, fixed, slots, ovf, ovf_head,
+    void *bufs[] = {rho, seed, lab[0], lab[1], first, sites, fixed, slots, ovf, ovf_head,
                     blockpart, hist, dstate, dflags, scratch, eg};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -298,6 +310,7 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
               hipMalloc(&c->seed, sizeof(uint32_t) * n) == hipSuccess &&
               hipMalloc(&c->lab[0], sizeof(uint32_t) * n) == hipSuccess &&
               hipMalloc(&c->lab[1], sizeof(uint32_t) * n) == hipSuccess &&
+              hipMalloc(&c->first, sizeof(uint32_t) * n) == hipSuccess &&
               hipMalloc(&c->sites, sizeof(float2) * (c->capS + 2)) == hipSuccess &&
               hipMalloc(&c->fixed, ms) == hipSuccess &&
               hipMalloc(&c->slots, sizeof(Slot) * 9 * ms) == hipSuccess &&
@@ -316,6 +329,7 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->seed, c->none(), n);
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[0], c->none(), n);
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[1], c->none(), n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->first, c->none(), n);
     hipMemsetAsync(c->rho, 0, sizeof(float) * n, s);
     hipMemsetAsync(c->fixed, 0, ms, s);
     hipMemsetAsync(c->slots, 0, sizeof(Slot) * 9 * ms, s);

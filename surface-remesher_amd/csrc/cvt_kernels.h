@@ -375,7 +375,7 @@ __device__ __forceinline__ void tile_moments(TileLds<HC> &L, const uint32_t *__r
 template <int HC>
 __device__ __forceinline__ void tile_flush(TileLds<HC> &L, const float2 *__restrict__ sites,
                                            int site_off, int T, int tX, int tY, int tilesX,
-                                           int tilesY, int epoch, Slot *__restrict__ slots,
+                                           int tilesY, int epoch, Slot *__restrict__ slots, long nrows,
                                            OvfRec *__restrict__ ovf, int ovf_cap,
                                            int *__restrict__ ovf_head, int *__restrict__ dstate,
                                            int *__restrict__ dflags)
@@ -394,7 +394,7 @@ __device__ __forceinline__ void tile_flush(TileLds<HC> &L, const float2 *__restr
         if (rx >= -1 && rx <= 1 && ry >= -1 && ry <= 1) {
             Slot sl;
             sl.w = wt; sl.dx = dx; sl.dy = dy; sl.e = ee; sl.epoch = epoch; sl.pad = 0;
-            slots[(long)gl * 9 + (ry + 1) * 3 + (rx + 1)] = sl;
+            slots[(long)((ry + 1) * 3 + (rx + 1)) * nrows + gl] = sl;
         } else {
             const int k = atomicAdd(&dstate[ST_OVFN], 1);
             if (k < ovf_cap) {
@@ -418,7 +418,7 @@ template <int T>
 static __global__ __launch_bounds__(256) void k_moments(
     const uint32_t *__restrict__ lab, const float *__restrict__ rho,
     const float2 *__restrict__ sites, int S, int W, int H, float rho_scale, int tilesX,
-    int tilesY, Slot *__restrict__ slots, OvfRec *__restrict__ ovf, int ovf_cap,
+    int tilesY, Slot *__restrict__ slots, long nrows, OvfRec *__restrict__ ovf, int ovf_cap,
     int *__restrict__ ovf_head, int *__restrict__ dflags, int *__restrict__ dstate)
 {
     __shared__ TileLds<TileHash<T>::HC> L;
@@ -426,7 +426,7 @@ static __global__ __launch_bounds__(256) void k_moments(
     const int tX = tile % tilesX, tY = tile / tilesX;
     tile_lds_init(L);
     tile_moments<(T < 64 ? T : 64)>(L, lab, rho, sites, 0, S, W, H, rho_scale, tX * T, tY * T, T);
-    tile_flush(L, sites, 0, T, tX, tY, tilesX, tilesY, dstate[ST_EPOCH], slots, ovf, ovf_cap,
+    tile_flush(L, sites, 0, T, tX, tY, tilesX, tilesY, dstate[ST_EPOCH], slots, nrows, ovf, ovf_cap,
                ovf_head, dstate, dflags);
 }
 
@@ -452,16 +452,17 @@ enum {
 // Moment totals of site i: the 9 window slots of this epoch plus its overflow
 // chain, summed exactly in 128 bits.  Consumes the chain.
 __device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__restrict__ slots,
+                                                long nrows,
                                                 const OvfRec *__restrict__ ovf,
                                                 int *__restrict__ ovf_head, double &w, double &dx,
                                                 double &dy, double &ee)
 {
     __int128 sw = 0, sx = 0, sy = 0, se = 0;
-    const Slot *sl = slots + (long)i * 9;
     // all nine slots are loaded before any is tested, so the loads overlap
+    // (slot k of consecutive sites is contiguous: a wave's loads coalesce)
     Slot v[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) v[k] = sl[k];
+    for (int k = 0; k < 9; ++k) v[k] = slots[(long)k * nrows + i];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const bool on = v[k].epoch == epoch;
@@ -484,8 +485,8 @@ __device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__
 
 static __global__ __launch_bounds__(256) void k_update(
     float2 *__restrict__ sites, const uint8_t *__restrict__ fixed, int S,
-    const Slot *__restrict__ slots, const OvfRec *__restrict__ ovf, int *__restrict__ ovf_head,
-    const int *__restrict__ dflags, int *__restrict__ dstate, float omega_arg,
+    const Slot *__restrict__ slots, long nrows, const OvfRec *__restrict__ ovf,
+    int *__restrict__ ovf_head, const int *__restrict__ dflags, int *__restrict__ dstate, float omega_arg,
     const float *__restrict__ domega, int mode, double rho_unscale, double *__restrict__ mom_out,
     double *__restrict__ blockpart, uint32_t *__restrict__ seed, int W, int H, CompatArgs ca,
     FinalArgs fa)
@@ -498,7 +499,7 @@ static __global__ __launch_bounds__(256) void k_update(
     double e = 0.0, g = 0.0;
     if (i < live) {
         double w, dx, dy, ee;
-        cvt_site_totals(i, dstate[ST_EPOCH], slots, ovf, ovf_head, w, dx, dy, ee);
+        cvt_site_totals(i, dstate[ST_EPOCH], slots, nrows, ovf, ovf_head, w, dx, dy, ee);
         e = ee;
         g = dx * dx + dy * dy;
         if (mom_out) {

@@ -19,13 +19,22 @@ JfaPlan plan_jfa(int W, int H, int capS);
 
 // One JFA pass over a W x H map of byte-offset labels (8 * site index, `none`
 // for no site): out(p) = best of the 9 labels at p + step * {-1, 0, 1}^2.
-// If clear is non-null it is reset to `none`: at the seed pixels of sites
-// [0, nclear) when nclear > 0 and the map holds nothing elsewhere (quad
-// plans), otherwise pixel by pixel.  The site table must hold capS + 1
-// entries rounded up to an even count.
+// If clear is non-null it is reset to `none`: with nclear > 0 on quad plans
+// only where it can hold entries, the seed pixels q of sites [0, nclear)
+// (rstep == 0) or q + rstep * {-1, 0, 1}^2 (rstep > 0); otherwise pixel by
+// pixel.  The site table must hold capS + 1 entries rounded up to an even
+// count.
 void launch_jfa_pass(hipStream_t st, const JfaPlan &plan, const uint32_t *in, uint32_t *out,
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
-                     int nclear = 0);
+                     int nclear = 0, int rstep = 0);
+
+// The first pass from a seed map (labels only at the seed pixels of sites
+// [0, S), none elsewhere), computed from the seeds into a map `out` that
+// holds none everywhere: the same output as launch_jfa_pass with in = seed.
+// Only the receivers q + step * {-1, 0, 1}^2 of the seed pixels q are
+// written.
+void launch_jfa_seed_pass(hipStream_t st, const uint32_t *seed, uint32_t *out,
+                          const float2 *sites, int S, int W, int H, int step);
 
 // A patch of a batch, as the batched kernels see it.  Maps are concatenated
 // (pix_off), site tables too (site_off; entry site_off + cap is the patch's

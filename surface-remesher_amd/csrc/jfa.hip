@@ -588,22 +588,30 @@ __device__ __forceinline__ int seed_pixel(float v, int n)
     return (int)f;
 }
 
-// nclear > 0: clear is a seed map whose only entries sit at the pixels of
-// sites [0, nclear); they are reset one store per site instead of one per
-// pixel (the pass does not read that map).
+// nclear > 0: clear is a map whose only entries sit at the seed pixels q of
+// sites [0, nclear) (rstep == 0), or at q + rstep * {-1, 0, 1}^2 (rstep > 0,
+// the receivers of a seed pass); they are reset one store per pixel of that
+// set instead of W * H stores (the pass does not read that map).
 template <int MODE, int ENTRIES, bool RUNS>
 __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa_quad(
     const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
     const float2 *__restrict__ sites, int table_f4, int W, int H, int step,
-    uint32_t *__restrict__ clear, uint32_t none, int nclear)
+    uint32_t *__restrict__ clear, uint32_t none, int nclear, int rstep)
 {
     if (clear && nclear > 0) {
         const __amdgpu_buffer_rsrc_t rc = buf_rsrc(clear, 4u * (uint32_t)W * (uint32_t)H);
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nclear;
-             i += gridDim.x * blockDim.x) {
+        const int R = rstep > 0 ? 9 : 1;
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < R * nclear;
+             t += gridDim.x * blockDim.x) {
+            const int i = t / R, r = t - R * (t / R);
             const float2 v = sites[i];
-            const int x = seed_pixel(v.x, W), y = seed_pixel(v.y, H);
-            __builtin_amdgcn_raw_buffer_store_b32(none, rc, 4u * (uint32_t)(y * W + x), 0, 0);
+            int x = seed_pixel(v.x, W), y = seed_pixel(v.y, H);
+            if (R == 9) {
+                x += (r % 3 - 1) * rstep;
+                y += (r / 3 - 1) * rstep;
+            }
+            if (x >= 0 && x < W && y >= 0 && y < H)
+                __builtin_amdgcn_raw_buffer_store_b32(none, rc, 4u * (uint32_t)(y * W + x), 0, 0);
         }
     }
     const MapGeo m = map_geo(in, out, nclear > 0 ? nullptr : clear, W, H, step, none);
@@ -752,25 +760,25 @@ JfaPlan plan_jfa(int W, int H, int capS)
 template <int MODE, bool RUNS>
 static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
                         const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
-                        int nclear)
+                        int nclear, int rstep)
 {
     const dim3 g(p.grid), b(p.threads);
     switch (p.lds_entries) {
     case 2048:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 2048, RUNS>), g, b, 0, st, in, out, sites, p.table_f4,
-                           W, H, step, clear, none, nclear);
+                           W, H, step, clear, none, nclear, rstep);
         break;
     case 5120:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 5120, RUNS>), g, b, 0, st, in, out, sites, p.table_f4,
-                           W, H, step, clear, none, nclear);
+                           W, H, step, clear, none, nclear, rstep);
         break;
     case 20480:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 20480, RUNS>), g, b, 0, st, in, out, sites,
-                           p.table_f4, W, H, step, clear, none, nclear);
+                           p.table_f4, W, H, step, clear, none, nclear, rstep);
         break;
     default:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 0, RUNS>), g, b, 0, st, in, out, sites, p.table_f4, W,
-                           H, step, clear, none, nclear);
+                           H, step, clear, none, nclear, rstep);
         break;
     }
 }
@@ -782,17 +790,17 @@ static const int kRunsMaxStep = 32;
 
 void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
-                     int nclear)
+                     int nclear, int rstep)
 {
     if (p.quad) {
-        if (step == 1) launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none, nclear);
+        if (step == 1) launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         // (step 2 with the global table keeps bands: six buffers of it spill)
         else if (step == 2 && p.lds_entries > 0)
-            launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none, nclear);
-        else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none, nclear);
+            launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
+        else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         else if (step <= kRunsMaxStep)
-            launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none, nclear);
-        else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none, nclear);
+            launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
+        else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         return;
     }
     constexpr int R = CVT_JFA_ROWS;
@@ -800,6 +808,55 @@ void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint3
     const int ntiles = tilesX * tilesY;
     hipLaunchKernelGGL(k_jfa_pass<R>, dim3(ntiles), dim3(256), 0, st, in, out, sites, W, H, tilesX,
                        ntiles, step, clear, none);
+}
+
+
+// ---------------------------------------------------------------------------
+// First pass from a seed map, as a scatter of the seeds.  The seed map holds
+// one label at the seed pixel of each winning site and none elsewhere, so the
+// pass's output is none except at the pixels whose taps hit a seed: the
+// receivers p = q - t of each seed pixel q (t in step * {-1, 0, 1}^2; the
+// off-grid tap redirection only ever adds the centre, which is already in the
+// set).  The output map must hold none everywhere; one thread per (site,
+// offset) evaluates its receiver exactly as the dense pass would (all nine
+// taps, same candidate order) and writes it.  A receiver of several seeds
+// (or of a pixel several sites share) is written with the same value by
+// each.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_jfa_seeds(const uint32_t *__restrict__ seed,
+                                                   uint32_t *__restrict__ out,
+                                                   const float2 *__restrict__ sites, int S, int W,
+                                                   int H, int step)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / 9, r = t - 9 * (t / 9);
+    if (i >= S) return;
+    const float2 s = sites[i];
+    const int qx = seed_pixel(s.x, W), qy = seed_pixel(s.y, H);
+    const int px = qx - (r % 3 - 1) * step, py = qy - (r / 3 - 1) * step;
+    if (px < 0 || px >= W || py < 0 || py >= H) return;
+    const char *sb = reinterpret_cast<const char *>(sites);
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    double best = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int ox = (k % 3 - 1) * step, oy = (k / 3 - 1) * step;
+        const int tx = (px + ox >= 0 && px + ox < W) ? px + ox : px;
+        const int ty = (py + oy >= 0 && py + oy < H) ? py + oy : py;
+        const uint32_t l = seed[(long)ty * W + tx];
+        const double key = cand(*reinterpret_cast<const float2 *>(sb + l), l, fx, fy);
+        best = (k == 0) ? key : __builtin_fmin(best, key);
+    }
+    out[(long)py * W + px] = (uint32_t)__double2loint(best);
+}
+
+void launch_jfa_seed_pass(hipStream_t st, const uint32_t *seed, uint32_t *out,
+                          const float2 *sites, int S, int W, int H, int step)
+{
+    if (S > 0)
+        hipLaunchKernelGGL(k_jfa_seeds, dim3((9 * S + 255) / 256), dim3(256), 0, st, seed, out,
+                           sites, S, W, H, step);
 }
 
 }  // namespace cvt

@@ -65,6 +65,32 @@ def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
     assert np.array_equal(lab, orc.jfa(W, H, xy))
 
 
+@pytest.mark.parametrize("W,H,S,seed", [
+    (512, 512, 1024, 31),     # 4-pixel kernel, table in LDS
+    (98, 77, 150, 32),        # one pixel per lane
+    (1024, 1024, 30000, 33),  # global gathers
+])
+def test_jfa_after_lloyd_and_reseeding(gpu, orc, W, H, S, seed):
+    """The seed map and the first-pass map are reset sparsely (around the seed
+    pixels of the sites): labels of a fresh JFA after Lloyd iterations, after
+    moving the sites, and of a repeated JFA must still be bit-exact."""
+    xy = rand_sites(seed, W, H, S)
+    rho = np.ones((H, W), np.float32)
+    with make_engine(W, xy, rho, H) as e:
+        e.lloyd(3, 1.0)
+        e.voronoi()
+        cur = e.sites()
+        assert np.array_equal(e.labels(), orc.jfa(W, H, cur))
+        moved = rand_sites(seed + 100, W, H, S)
+        e.set_sites(moved)
+        e.voronoi()
+        e.voronoi()
+        assert np.array_equal(e.labels(), orc.jfa(W, H, moved))
+        e.lloyd(2, 1.0)
+        e.voronoi()
+        assert np.array_equal(e.labels(), orc.jfa(W, H, e.sites()))
+
+
 def test_jfa_awkward_sites(gpu, orc):
     n = 200
     xy = rand_sites(7, n, n, 300)
