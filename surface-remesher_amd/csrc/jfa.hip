@@ -784,9 +784,15 @@ static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, ui
 }
 
 // Layout by step, as measured at 2048^2 / 16K sites (us per pass, bands of
-// rows -> runs): steps 32 .. 1 gain (13.9 -> 13.3 at 32, 14.8 -> 11.3 at
-// 1), steps 64 and up do not (14.0 -> 16.2 at 64, 13.5 -> 17.3 at 1024).
+// rows -> runs): steps 32 .. 1 gain (13.9 -> 13.3 at 32, 14.8 -> 11.8 at 1),
+// and so do the two largest dense steps, max(W, H) / 4 and / 8 (16.4 -> 14.4
+// at 512, 17.3 -> 14.1 at 256); steps 128 and 64 do not (13.7 -> 14.8,
+// 13.6 -> 16.2).
 static const int kRunsMaxStep = 32;
+static bool use_runs(int W, int H, int step)
+{
+    return step <= kRunsMaxStep || 8L * step >= (long)std::max(W, H);
+}
 
 void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
@@ -798,7 +804,7 @@ void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint3
         else if (step == 2 && p.lds_entries > 0)
             launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
-        else if (step <= kRunsMaxStep)
+        else if (use_runs(W, H, step))
             launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         return;
