@@ -8,10 +8,11 @@ metric is quoted on).  One step = one Lloyd iteration = JFA(+1) (12 passes)
 Inputs are resident in HBM before the timed region.
 
 The JSON line carries:
-  roofline      the dominant kernel (the JFA pass): algorithmic bytes per
-                launch (read + write of the int32 label map, 8 B/pixel)
-                over its mean duration measured with HIP events on the
-                engine's stream, against the 8 TB/s HBM peak;
+  roofline      the dominant kernel (the dense JFA pass k_jfa_quad, passes
+                1 .. P-1; pass 0 is a scatter of the seeds): algorithmic
+                bytes per launch (read + write of the int32 label map,
+                8 B/pixel) over its mean duration measured with HIP events
+                on the engine's stream, against the 8 TB/s HBM peak;
   cpu_baseline  the oracle's OpenMP Lloyd (JFA(+1) and exact nearest-site
                 variants; the faster is reported) on a bounded sample of the
                 same workload, on this host's cores.
@@ -348,7 +349,7 @@ def main():
                    "moment_tile": prof["tile"], "seed": args.seed},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_jfa",
+            "kernel": "k_jfa_quad",
             "achieved": round(jfa_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

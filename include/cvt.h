@@ -109,10 +109,11 @@ int cvt_clear_history(cvt_ctx *ctx);
 int cvt_energy(cvt_ctx *ctx, double *energy, double *gradnorm);
 int cvt_synchronize(cvt_ctx *ctx);
 /* Runs `iterations` (1..1000) Lloyd steps eagerly with HIP events around
- * each phase and returns mean durations in ms: ms[0] one JFA pass (the
- * back-to-back pass sequence over the pass count), ms[1] the moment kernel,
- * ms[2] 0, ms[3] update + finalize, ms[4] 0, ms[5] the whole step, ms[6] the
- * JFA pass count, ms[7] the moment tile.
+ * each phase and returns mean durations in ms: ms[0] one dense JFA pass (the
+ * back-to-back sequence of passes 1 .. P-1 over P - 1), ms[1] the moment
+ * kernel, ms[2] the first JFA pass (from the seed map), ms[3] update +
+ * finalize, ms[4] 0, ms[5] the whole step, ms[6] the JFA pass count P, ms[7]
+ * the moment tile.
  * (No reference counterpart: the reference times stages with clock(),
  * source/main.cpp:181-242.) */
 int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms);

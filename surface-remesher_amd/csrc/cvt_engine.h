@@ -61,7 +61,7 @@ struct Ctx {
     ~Ctx();
     int enqueue_seed();
     void clear_seed_map();         // every pixel back to none()
-    int enqueue_jfa();
+    int enqueue_jfa(hipEvent_t after_first = nullptr);   // event recorded after pass 0
     int enqueue_moments();
     int enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg,
                        const float *domega = nullptr, CompatArgs ca = CompatArgs{});

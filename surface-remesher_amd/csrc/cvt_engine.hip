@@ -149,7 +149,7 @@ void Ctx::clear_seed_map()
 }
 
 // JFA(+1) of the current sites; the labels end in lab[final_buf]
-int Ctx::enqueue_jfa()
+int Ctx::enqueue_jfa(hipEvent_t after_first)
 {
     const int np = jfa_num_passes(W, H);
     int step = jfa_first_step(W, H);
@@ -165,6 +165,7 @@ int Ctx::enqueue_jfa()
     for (int k = 0; k < np; ++k) {
         if (k == 0 && scatter) {
             launch_jfa_seed_pass(stream, seed, first, sites, S, W, H, step);
+            if (after_first) hipEventRecord(after_first, stream);
             in = first;
             if (step > 1) step >>= 1;
             continue;
@@ -173,6 +174,7 @@ int Ctx::enqueue_jfa()
         uint32_t *clear = (k == 1) ? seed : (k == 2 && scatter) ? first : nullptr;
         launch_jfa_pass(stream, jplan, in, out, sites, W, H, step, clear, none(),
                         clear ? S : 0, k == 2 ? step0 : 0);
+        if (k == 0 && after_first) hipEventRecord(after_first, stream);
         in = out;
         cur ^= 1;
         if (step > 1) step >>= 1;
@@ -599,41 +601,43 @@ int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms)
     if (iterations < 1 || iterations > 1000 || !ms)
         return fail(CVT_ERR_ARG, "iterations %d (1..1000)", iterations);
     const int np = jfa_num_passes(c->W, c->H);
-    // four events per iteration bracket the phases; nothing in between, so
-    // the passes run back to back as in a graph replay
-    std::vector<hipEvent_t> ev(4 * (size_t)iterations);
+    // five events per iteration bracket the phases (the first JFA pass, the
+    // other passes, moments, update); nothing in between, so the passes run
+    // back to back as in a graph replay
+    constexpr int NE = 5;
+    std::vector<hipEvent_t> ev(NE * (size_t)iterations);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e), "event");
     int rc = CVT_OK;
     for (int it = 0; it < iterations && !rc; ++it) {
         if (!c->seeds_valid) c->enqueue_seed();
-        hipEventRecord(ev[4 * it + 0], c->stream);
-        rc = c->enqueue_jfa();
-        hipEventRecord(ev[4 * it + 1], c->stream);
+        hipEventRecord(ev[NE * it + 0], c->stream);
+        rc = c->enqueue_jfa(ev[NE * it + 1]);
+        hipEventRecord(ev[NE * it + 2], c->stream);
         if (!rc) rc = c->enqueue_moments();
-        hipEventRecord(ev[4 * it + 2], c->stream);
+        hipEventRecord(ev[NE * it + 3], c->stream);
         if (!rc) rc = c->enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr);
-        hipEventRecord(ev[4 * it + 3], c->stream);
+        hipEventRecord(ev[NE * it + 4], c->stream);
     }
-    double acc[4] = {0, 0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0};
     if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(CVT_ERR_HIP, "profile");
     for (int it = 0; it < iterations && !rc; ++it) {
         float t;
-        for (int k = 0; k < 3; ++k) {
-            hipEventElapsedTime(&t, ev[4 * it + k], ev[4 * it + k + 1]);
+        for (int k = 0; k < NE - 1; ++k) {
+            hipEventElapsedTime(&t, ev[NE * it + k], ev[NE * it + k + 1]);
             acc[k] += t;
         }
-        hipEventElapsedTime(&t, ev[4 * it], ev[4 * it + 3]);
-        acc[3] += t;
+        hipEventElapsedTime(&t, ev[NE * it], ev[NE * it + NE - 1]);
+        acc[4] += t;
     }
     for (auto &e : ev) hipEventDestroy(e);
     if (rc) return rc;
     c->labels_fresh = true;
-    ms[0] = acc[0] / ((double)iterations * np);
-    ms[1] = acc[1] / iterations;
-    ms[2] = 0.0;
-    ms[3] = acc[2] / iterations;
+    ms[0] = np > 1 ? acc[1] / ((double)iterations * (np - 1)) : acc[0] / iterations;
+    ms[1] = acc[2] / iterations;
+    ms[2] = np > 1 ? acc[0] / iterations : 0.0;
+    ms[3] = acc[3] / iterations;
     ms[4] = 0.0;
-    ms[5] = acc[3] / iterations;
+    ms[5] = acc[4] / iterations;
     ms[6] = np;
     ms[7] = c->tile;
     return c->check_flags();

@@ -128,11 +128,12 @@ class Engine:
 
     def profile(self, iterations: int = 5, omega: float = 1.0) -> dict:
         """Eager Lloyd steps timed per phase with HIP events (ms); jfa_pass is
-        the pass sequence's duration over the pass count."""
+        one dense JFA pass (passes 1 .. P-1, mean), jfa_first the first pass
+        (from the seed map)."""
         ms = np.zeros(8, np.float64)
         p, keep = _ptr(ms)
         call("cvt_profile", self._h, int(iterations), float(omega), p)
-        return {"jfa_pass": ms[0], "moments": ms[1], "update": ms[3],
+        return {"jfa_pass": ms[0], "jfa_first": ms[2], "moments": ms[1], "update": ms[3],
                 "step": ms[5], "jfa_passes": int(ms[6]), "tile": int(ms[7])}
 
     def synchronize(self):
