@@ -156,7 +156,8 @@ int Ctx::enqueue_jfa(hipEvent_t after_first)
     const uint32_t *in = seed;
     int cur = 0;
     // Pass 0 reads only the seed map: with three passes or more it scatters
-    // the seeds into `first`, which holds none except at their receivers.
+    // the seeds into `first`, which holds none except at their receivers
+    // (measured: 7.6 against 13.5 us at 2048^2, 6.9 against 7.8 at 512^2).
     // Pass 1 resets the seed map it no longer needs, pass 2 resets `first`:
     // both hold entries only around the seed pixels of sites [0, S) (the
     // sites do not move during a JFA; compaction only shrinks the count).

@@ -788,10 +788,15 @@ static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, ui
 // and so do the two largest dense steps, max(W, H) / 4 and / 8 (16.4 -> 14.4
 // at 512, 17.3 -> 14.1 at 256); steps 128 and 64 do not (13.7 -> 14.8,
 // 13.6 -> 16.2).
+// Runs also need work for every wave: a run is four items, so on small grids
+// (512^2: 256 runs at step 1 for 1024 waves) they would idle most waves.
 static const int kRunsMaxStep = 32;
-static bool use_runs(int W, int H, int step)
+static bool use_runs(const JfaPlan &p, int W, int H, int step)
 {
-    return step <= kRunsMaxStep || 8L * step >= (long)std::max(W, H);
+    if (step > kRunsMaxStep && 8L * step < (long)std::max(W, H)) return false;
+    const long n = (H + step - 1) / step;
+    const long runs = (long)std::min(step, H) * ((n + 3) / 4) * ((W + 255) / 256);
+    return runs >= (long)p.grid * (p.threads / 64);
 }
 
 void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
@@ -799,12 +804,16 @@ void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint3
                      int nclear, int rstep)
 {
     if (p.quad) {
-        if (step == 1) launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
+        const bool runs = use_runs(p, W, H, step);
+        if (step == 1 && runs)
+            launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
+        else if (step == 1)
+            launch_quad<STEP_1, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         // (step 2 with the global table keeps bands: six buffers of it spill)
-        else if (step == 2 && p.lds_entries > 0)
+        else if (step == 2 && runs && p.lds_entries > 0)
             launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
-        else if (use_runs(W, H, step))
+        else if (runs)
             launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         return;
