@@ -55,6 +55,7 @@ def check_moments(m, ref, W_tol=1e-6):
     (8, 8, 3, 11),            # narrowest grid of the 4-pixel kernel
     (260, 130, 5000, 12),     # 40 KB site table, width not a multiple of 256
     (512, 384, 8000, 10),     # 160 KB site table on a small grid
+    (2560, 1700, 8000, 13),   # runs layout over chains of unequal length
 ])
 def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
     xy = rand_sites(seed, W, H, S)
