@@ -38,6 +38,9 @@
  *                                 (source/discretization.cu:87-120)
  *   cvt_extract_points            the site scan of delaunayInput
  *                                 (source/delaunay.h:48-58)
+ *   cvt_locate_points             locate() and the vertex interpolation of
+ *                                 recover() (source/recover.h:63-81,
+ *                                 :109-125, :134-141)
  */
 #ifndef CVT_H
 #define CVT_H
@@ -148,6 +151,22 @@ int cvt_rasterize(const double *points, const double *weight, int num_point, con
  * out = NULL only counts. */
 int cvt_extract_points(const int16_t *voronoi, const uint8_t *mask, int n, double scale, double l,
                        double b, double *out, long capacity, long *count);
+
+/* Point location in a 2D triangle mesh (replaces the brute-force face scan
+ * locate() of recover(), source/recover.h:63-81, run for every site at
+ * :109-125 and for every output triangle's centroid at :134-141): for each
+ * query point q[k] the first face, in face order, whose float64 barycentric
+ * weights (the formula of source/recover.h:30-53) are all >= 0, or -1 for
+ * none (a face with a zero denominator holds no point; a non-finite point is
+ * in no face).  uv: [nv][2] vertex coordinates; tri: [nf][3] vertex indices
+ * (CVT_ERR_ARG if one is outside [0, nv)); q: [nq][2]; face: [nq].
+ * Optional: w [nq][3] receives the weights of the face's vertices 0, 1, 2
+ * (1 - w1 - w2, w1, w2 as recover() stores them; -1 where face is -1); with
+ * p3 [nv][3] (the 3D vertices), out3 [nq][3] receives sum_j w_j p3[v_j]
+ * summed in j order from 0.0 (source/recover.h:117-123; 0 where face is -1).
+ * Host or device pointers; the call returns when the results are written. */
+int cvt_locate_points(const double *uv, const double *p3, int nv, const int *tri, int nf,
+                      const double *q, int nq, int *face, double *w, double *out3);
 
 /* ---------------------------------------------------------------------------
  * Batched CVT: many independent patches of different sizes advanced

@@ -37,6 +37,11 @@
  *                         first uniform-density site (0, 437) at 512;
  *   orc_compat_lloyd   -- update rule on hand-checked cases, fixed sites
  *                         never move, sites never land on rho == 0, merges.
+ *   orc_locate         -- exact weights at the vertices ((1,0,0), (0,1,0),
+ *                         (0,0,1)), ~1/3 at centroids, reconstruction
+ *                         sum w_j uv_j == q, first face in order on shared
+ *                         edges and overlaps, degenerate faces never hit,
+ *                         linear 3D fields reproduced by the interpolation.
  */
 #include <math.h>
 #include <stdint.h>
@@ -615,6 +620,83 @@ long orc_extract_points(const int16_t *vor, const uint8_t *mask, int n,
             }
         }
     return k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Point location (source/recover.h:26-81) and recover()'s vertex step       */
+/* (source/recover.h:109-125)                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* source/recover.h:30-53: with v0 = p2 - p1, v1 = p3 - p1, v2 = q - p1 and
+ * the dot products d00, d01, d11, d20, d21 (x*x + y*y each), den = d00 d11 -
+ * d01^2; den == 0 gives -1 weights; w1 = (d11 d20 - d01 d21) / den,
+ * w2 = (d00 d21 - d01 d20) / den, and p1's weight is 1 - w1 - w2 (recover()
+ * passes its weight array as (w3, w1, w2), :79). */
+static void orc_rec_bary(double x1, double y1, double x2, double y2, double x3, double y3,
+                         double x0, double y0, double *w0, double *w1, double *w2)
+{
+    double v0x = x2 - x1, v0y = y2 - y1;
+    double v1x = x3 - x1, v1y = y3 - y1;
+    double v2x = x0 - x1, v2y = y0 - y1;
+    double d00 = v0x * v0x + v0y * v0y;
+    double d01 = v0x * v1x + v0y * v1y;
+    double d11 = v1x * v1x + v1y * v1y;
+    double d20 = v2x * v0x + v2y * v0y;
+    double d21 = v2x * v1x + v2y * v1y;
+    double den = d00 * d11 - d01 * d01;
+    if (den == 0) {
+        *w0 = *w1 = *w2 = -1;
+        return;
+    }
+    *w1 = (d11 * d20 - d01 * d21) / den;
+    *w2 = (d00 * d21 - d01 * d20) / den;
+    *w0 = 1.0 - *w1 - *w2;
+}
+
+/* locate() (source/recover.h:63-81): the first face in order whose three
+ * weights are all >= 0, by testing every face; -1 (weights -1) for none.  A
+ * non-finite point is in no face (the engine's rule; the reference would
+ * accept it in the first non-degenerate face).  With p3 and out3, the 3D
+ * point sum_j w_j p3[v_j], summed in j order from 0.0 (:117-123), 0 for
+ * none. */
+void orc_locate(const double *uv, const double *p3, const int *tri, int nf, const double *q,
+                int nq, int *face, double *w, double *out3)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int k = 0; k < nq; ++k) {
+        double x = q[2 * k], y = q[2 * k + 1];
+        int hit = -1;
+        double b0 = -1, b1 = -1, b2 = -1;
+        if (isfinite(x) && isfinite(y))
+            for (int i = 0; i < nf; ++i) {
+                const int *t = tri + 3 * i;
+                double c0, c1, c2;
+                orc_rec_bary(uv[2 * t[0]], uv[2 * t[0] + 1], uv[2 * t[1]], uv[2 * t[1] + 1],
+                             uv[2 * t[2]], uv[2 * t[2] + 1], x, y, &c0, &c1, &c2);
+                if (c0 < 0 || c1 < 0 || c2 < 0) continue;
+                hit = i;
+                b0 = c0; b1 = c1; b2 = c2;
+                break;
+            }
+        face[k] = hit;
+        if (w) {
+            w[3 * k] = b0;
+            w[3 * k + 1] = b1;
+            w[3 * k + 2] = b2;
+        }
+        if (p3 && out3) {
+            double r[3] = {0.0, 0.0, 0.0};
+            if (hit >= 0) {
+                const int *t = tri + 3 * hit;
+                const double wt[3] = {b0, b1, b2};
+                for (int c = 0; c < 3; ++c)
+                    for (int j = 0; j < 3; ++j) r[c] += p3[3 * t[j] + c] * wt[j];
+            }
+            out3[3 * k] = r[0];
+            out3[3 * k + 1] = r[1];
+            out3[3 * k + 2] = r[2];
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */

@@ -69,6 +69,7 @@ def lib():
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "orc_extract_points": (C.c_long, [_i16p, _u8p, C.c_int, C.c_double, C.c_double,
                                           C.c_double, _f64p]),
+        "orc_locate": (None, [_f64p, vp, _i32p, C.c_int, _f64p, C.c_int, _i32p, vp, vp]),
         "orc_max_threads": (C.c_int, []),
         "orc_set_threads": (None, [C.c_int]),
     }
@@ -261,6 +262,22 @@ def extract_points(vor, mask, scale, l, b):
     out = np.empty((n * n, 2), np.float64)
     k = lib().orc_extract_points(vor, np.ascontiguousarray(mask, np.uint8), n, scale, l, b, out)
     return out[:k].copy()
+
+
+def locate(uv, tri, q, p3=None):
+    """First face (in order) holding each query point, its three weights and,
+    with p3 ([nv][3]), the interpolated 3D point (source/recover.h:63-125)."""
+    uv = np.ascontiguousarray(uv, np.float64).reshape(-1)
+    tr = np.ascontiguousarray(tri, np.int32).reshape(-1)
+    q = np.ascontiguousarray(q, np.float64).reshape(-1, 2)
+    nq = q.shape[0]
+    face = np.empty(nq, np.int32)
+    w = np.empty((nq, 3), np.float64)
+    out3 = np.empty((nq, 3), np.float64) if p3 is not None else None
+    p3c = None if p3 is None else np.ascontiguousarray(p3, np.float64).reshape(-1)
+    lib().orc_locate(uv, _ptr(p3c), tr, tr.shape[0] // 3, q.reshape(-1), nq, face, _ptr(w),
+                     _ptr(out3))
+    return (face, w) if p3 is None else (face, w, out3)
 
 
 def set_threads(t):

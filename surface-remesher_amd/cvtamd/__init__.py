@@ -6,6 +6,6 @@ seeded workload generators used by the tests and the benchmark.
 """
 from ._lib import LIB_PATH, LABEL_NONE, CvtError, lib
 from .engine import Batch, Engine
-from . import compat, workloads
+from . import compat, recover, workloads
 
-__all__ = ["Batch", "Engine", "CvtError", "LABEL_NONE", "LIB_PATH", "lib", "compat", "workloads"]
+__all__ = ["Batch", "Engine", "CvtError", "LABEL_NONE", "LIB_PATH", "lib", "compat", "recover", "workloads"]

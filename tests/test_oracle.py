@@ -383,3 +383,77 @@ def test_extract_points_order_and_mask(orc):
     pts = orc.extract_points(vor, mask, 0.5, 10.0, 20.0)
     # x-major loop order: (1,1), (1,2), (3,0)
     assert pts.tolist() == [[10.5, 20.5], [10.5, 21.0], [11.5, 20.0]]
+
+
+# ------------------------------------------------------- point location --
+def jittered_mesh(m, seed, jitter=0.3):
+    """(m x m) jittered grid of [0, m-1]^2 split into 2 (m-1)^2 triangles,
+    faces shuffled and some wound clockwise."""
+    g = np.random.default_rng(seed)
+    x, y = np.meshgrid(np.arange(m, dtype=np.float64), np.arange(m, dtype=np.float64))
+    uv = np.stack([x.ravel(), y.ravel()], 1)
+    inner = (uv[:, 0] > 0) & (uv[:, 0] < m - 1) & (uv[:, 1] > 0) & (uv[:, 1] < m - 1)
+    uv[inner] += g.uniform(-jitter, jitter, (inner.sum(), 2))
+    tri = []
+    for j in range(m - 1):
+        for i in range(m - 1):
+            a, b, c, d = j * m + i, j * m + i + 1, (j + 1) * m + i, (j + 1) * m + i + 1
+            tri += [[a, b, d], [a, d, c]]
+    tri = np.array(tri, np.int32)[g.permutation(2 * (m - 1) ** 2)]
+    flip = g.random(len(tri)) < 0.5
+    tri[flip] = tri[flip][:, ::-1]
+    return uv, tri
+
+
+def test_locate_vertices_exact_weights(orc):
+    # at a face's own vertices the weights are exactly the unit vectors
+    uv = np.array([[0.3, 0.1], [2.7, 0.4], [1.1, 3.3]])
+    tri = np.array([[0, 1, 2]], np.int32)
+    face, w = orc.locate(uv, tri, uv)
+    assert np.all(face == 0)
+    assert np.array_equal(w, np.eye(3))
+
+
+def test_locate_centroid_and_reconstruction(orc):
+    uv, tri = jittered_mesh(12, 1)
+    g = np.random.default_rng(2)
+    q = g.uniform(0.01, 10.99, (2000, 2))
+    face, w = orc.locate(uv, tri, q)
+    assert np.all(face >= 0)
+    assert np.all(w >= 0) and np.allclose(w.sum(1), 1.0, atol=1e-12)
+    rec = np.einsum("kj,kjd->kd", w, uv[tri[face]])
+    assert np.abs(rec - q).max() < 1e-12
+    cen = uv[tri].mean(1)
+    fc, wc = orc.locate(uv, tri, cen)
+    assert np.array_equal(fc, np.arange(len(tri)))           # each centroid in its own face
+    assert np.abs(wc - 1.0 / 3.0).max() < 1e-12
+
+
+def test_locate_first_face_in_order(orc):
+    # the shared diagonal of a unit square lies in both faces: the first listed wins
+    uv = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+    a = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    q = np.array([[0.5, 0.5], [0.25, 0.25], [0.9, 0.1], [0.1, 0.9], [1.5, 0.5]])
+    assert orc.locate(uv, a, q)[0].tolist() == [0, 0, 0, 1, -1]
+    assert orc.locate(uv, a[::-1].copy(), q)[0].tolist() == [0, 0, 1, 0, -1]
+    # a duplicate face never wins over its earlier copy; a degenerate face never holds a point
+    b = np.array([[0, 2, 2], [1, 2, 0], [0, 1, 2]], np.int32)
+    f, w = orc.locate(uv, b, np.array([[0.5, 0.2]]))
+    assert f.tolist() == [1] and np.allclose(w[0], [0.3, 0.2, 0.5], rtol=0, atol=1e-15)
+
+
+def test_locate_outside_and_nonfinite(orc):
+    uv, tri = jittered_mesh(6, 3)
+    q = np.array([[-0.5, 2.0], [2.0, 5.5], [np.nan, 1.0], [1.0, np.inf]])
+    face, w = orc.locate(uv, tri, q)
+    assert np.all(face == -1) and np.all(w == -1)
+
+
+def test_locate_lifts_linear_fields(orc):
+    uv, tri = jittered_mesh(9, 4)
+    p3 = np.stack([uv[:, 0], uv[:, 1], 0.7 - 0.2 * uv[:, 0] + 0.35 * uv[:, 1]], 1)
+    q = np.random.default_rng(5).uniform(0.0, 8.0, (500, 2))
+    face, w, out3 = orc.locate(uv, tri, q, p3=p3)
+    assert np.all(face >= 0)
+    assert np.abs(out3[:, :2] - q).max() < 1e-12
+    assert np.abs(out3[:, 2] - (0.7 - 0.2 * q[:, 0] + 0.35 * q[:, 1])).max() < 1e-12
