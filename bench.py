@@ -203,6 +203,39 @@ def unet_mfma(batch: int = 16, steps: int = 10) -> dict:
             "train_frac": round(t_tf / BF16_PEAK_TFLOPS, 5)}
 
 
+def locate_sample(with_cpu: bool) -> dict:
+    """recover()'s point location on one patch mesh: 49152 queries (16384
+    sites + 32768 triangle centroids) in a 11858-face mesh, on the GPU
+    (inputs resident in device memory; the call includes its binning and
+    returns when the faces, weights and 3D points are written) and, as the
+    baseline, the oracle's face-by-face scan on the host cores."""
+    import numpy as np
+    import torch
+    from cvtamd import recover, workloads as wl
+    uv, p3, tri, q = wl.patch_mesh()
+    d = [torch.from_numpy(a).cuda() for a in (uv, p3, tri, q)]
+    recover.locate_points(d[0], d[2], d[3], p3=d[1])          # warm-up
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        face, w, out3 = recover.locate_points(d[0], d[2], d[3], p3=d[1])
+    gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    res = {"queries": int(len(q)), "faces": int(len(tri)), "located": int((face >= 0).sum()),
+           "gpu_ms": round(gpu_ms, 3), "gpu_queries_per_s": round(len(q) / gpu_ms * 1e3, 1)}
+    if with_cpu:
+        from oracle import oracle as orc
+        orc.set_threads(min(orc.max_threads(), int(os.environ.get("OMP_NUM_THREADS", "64"))))
+        t0 = time.perf_counter()
+        f_cpu, w_cpu, o_cpu = orc.locate(uv, tri, q, p3=p3)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        res.update({"cpu_ms": round(cpu_ms, 1), "cpu_threads": orc.max_threads(),
+                    "cpu_kind": "port (face-by-face scan, as source/recover.h:63-81)",
+                    "bit_exact": bool(np.array_equal(f_cpu, face) and np.array_equal(w_cpu, w)
+                                      and np.array_equal(o_cpu, out3)),
+                    "speedup": round(cpu_ms / gpu_ms, 1)})
+    return res
+
+
 def bench_batch(args):
     """The 64-patch batch: one step = one batched Lloyd iteration over every
     patch (one graph launch); value = batched iterations/s, with the
@@ -263,6 +296,8 @@ def main():
     ap.add_argument("--ttt", type=int, default=4, metavar="K",
                     help="time-to-tolerance of the learned warm start vs plain Lloyd on K "
                          "held-out 2048^2 / 16K patches (0 = skip; needs trained weights)")
+    ap.add_argument("--no-locate", dest="locate", action="store_false",
+                    help="skip the recover() point-location sample")
     ap.add_argument("--unet", type=int, default=16, metavar="B",
                     help="MFMA throughput of the warm-start network at batch B (0 = skip)")
     ap.add_argument("--unet-only", action="store_true",
@@ -365,6 +400,8 @@ def main():
     }
     if args.ttt > 0 and args.workload == "c2":
         out["time_to_tol"] = time_to_tol_sample(args.ttt)
+    if args.locate and args.workload == "c2":
+        out["locate"] = locate_sample(not args.no_cpu)
     if args.unet > 0 and args.workload == "c2":
         out["mfma"] = unet_mfma(args.unet)
     if not args.no_cpu:

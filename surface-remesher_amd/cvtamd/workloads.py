@@ -183,3 +183,26 @@ WARMSTART_EVAL_SEED_BASE = 1 << 40
 def warmstart_eval(index: int, n: int = 2048, num_sites: int = 16384) -> Workload:
     """Held-out evaluation patch at 2048^2 / 16384 sites (config-2 generator)."""
     return config2(WARMSTART_EVAL_SEED_BASE + index, n, num_sites)
+
+
+def patch_mesh(seed: int = 5, m: int = 78):
+    """A parameterized patch as recover() sees it: an m x m jittered grid of
+    [0, 1]^2 split into 2 (m-1)^2 triangles (faces shuffled), the 3D surface
+    z = 0.3 sin(3u) cos(2v) over it, and the query points of one recover()
+    call: 16384 sites sampled in the patch and the centroids of 32768
+    triangles over them.  Returns (uv, p3, tri, queries)."""
+    g = rng(seed)
+    h = 1.0 / (m - 1)
+    x, y = np.meshgrid(np.arange(m) * h, np.arange(m) * h)
+    uv = np.stack([x.ravel(), y.ravel()], 1)
+    inner = (uv[:, 0] > 0) & (uv[:, 0] < 1) & (uv[:, 1] > 0) & (uv[:, 1] < 1)
+    uv[inner] += g.uniform(-0.3 * h, 0.3 * h, (int(inner.sum()), 2))
+    i, j = np.meshgrid(np.arange(m - 1), np.arange(m - 1))
+    a = (j * m + i).ravel()
+    tri = np.concatenate([np.stack([a, a + 1, a + m + 1], 1), np.stack([a, a + m + 1, a + m], 1)])
+    tri = tri[g.permutation(len(tri))].astype(np.int32)
+    p3 = np.stack([uv[:, 0], uv[:, 1], 0.3 * np.sin(3 * uv[:, 0]) * np.cos(2 * uv[:, 1])], 1)
+    sites = g.random((16384, 2))
+    t = g.integers(0, len(sites), (32768, 3))
+    cen = (sites[t[:, 0]] + sites[t[:, 1]] + sites[t[:, 2]]) / 3.0
+    return uv, p3, tri, np.concatenate([sites, cen])
