@@ -2,9 +2,10 @@
 //
 // One cvt_ctx owns every device buffer of one grid for its whole life (the
 // reference allocates and frees about eleven N^2 buffers per gCVT call,
-// source/gcvt.cu:840-903).  A Lloyd iteration is P JFA launches, one moment
-// launch and one update launch (whose last block also adds the energy
-// partials and appends to the history); it is captured once into a hipGraph
+// source/gcvt.cu:840-903).  A Lloyd iteration is P JFA launches (the first a
+// scatter of the seeds), one moment launch and one update launch (whose last
+// block also adds the energy partials and appends to the history); it is
+// captured once into a hipGraph
 // and replayed, so the host does no work per iteration and never
 // synchronises inside the loop (the reference copies the energy back every
 // 10th iteration, source/gcvt.cu:1080).

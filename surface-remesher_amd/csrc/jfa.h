@@ -7,7 +7,8 @@
 namespace cvt {
 
 // Launch shape of a JFA pass for one grid: the quad kernel (4 pixels per
-// lane, persistent row bands) when W % 4 == 0, with the site table in a
+// lane, persistent workgroups over runs or bands of rows) when W % 4 == 0,
+// with the site table in a
 // static LDS array of lds_entries float2 (16, 40 or 160 KB) when it fits;
 // otherwise one pixel per lane.
 struct JfaPlan {
