@@ -15,13 +15,14 @@
 // fail no "< 0" test).
 //
 // Instead of testing every face for every point (O(Q F)), faces are binned
-// into a G x G grid over the vertices' bounding box by a conservative box,
-// each bin keeping its list in increasing face order, so the first hit in
-// the list is the first hit overall.  Faces so thin that rounding could
-// accept points well outside their box go to an "always" list every point
-// tests, merged in face order with its bin list (the rasterizer's scheme,
-// csrc/raster.hip).  Contraction is off for this file: every expression
-// rounds as written, as in the oracle.
+// into a G x G grid over the vertices' bounding box by a conservative box (a
+// counting sort: count, prefix sum, fill; linear in the number of (face, bin)
+// pairs).  A point tests the faces of its bin and keeps the lowest-index hit,
+// which is the first hit in face order: every face holding the point lists
+// it in that bin.  Faces so thin that rounding could accept points well
+// outside their box go to an "always" list that every point tests.
+// Contraction is off for this file: every expression rounds as written, as
+// in the oracle.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -137,50 +138,67 @@ __global__ void k_face_boxes(const double *__restrict__ uv, int nv, const int *_
     box[i] = bx;
 }
 
-// per bin: the faces whose box meets it, in face order (ballot + prefix
-// count keeps the order); a bin with more than cap faces is flagged and
-// later tests every face
-__global__ __launch_bounds__(256) void k_bin_lists(const FaceBox *__restrict__ box, int nf, int G,
-                                                   int *__restrict__ list, int *__restrict__ count,
-                                                   int cap)
+// (face, bin) pairs: count per bin
+__global__ void k_bin_count(const FaceBox *__restrict__ box, int nf, int G,
+                            int *__restrict__ count)
 {
-    const int bin = blockIdx.x;
-    const int bx = bin % G, by = bin / G;
-    __shared__ int wave_n[4];
-    __shared__ int total;
-    if (threadIdx.x == 0) total = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int base = 0; base < nf; base += 256) {
-        const int i = base + threadIdx.x;
-        bool hit = false;
-        if (i < nf) {
-            const FaceBox b = box[i];
-            hit = b.x0 <= b.x1 && b.x0 <= bx && b.x1 >= bx && b.y0 <= by && b.y1 >= by;
-        }
-        const unsigned long long m = __ballot(hit);
-        if (lane == 0) wave_n[wave] = __popcll(m);
-        __syncthreads();
-        int off = total;
-        for (int w = 0; w < wave; ++w) off += wave_n[w];
-        if (hit) {
-            const int k = off + __popcll(m & ((1ull << lane) - 1ull));
-            if (k < cap) list[(long)bin * cap + k] = i;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) total += wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) count[bin] = total;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const FaceBox b = box[i];
+    for (int y = b.y0; y <= b.y1; ++y)
+        for (int x = b.x0; x <= b.x1; ++x) atomicAdd(&count[y * G + x], 1);
 }
 
-// one thread per query point: walk its bin's list merged with the "always"
-// list in face order until the first hit; face -1 when none holds it
+// exclusive prefix sum of n counts into start[0 .. n] (one workgroup: each
+// thread sums a contiguous chunk, the chunk totals are scanned in LDS, then
+// each thread writes its chunk's starts)
+__global__ __launch_bounds__(1024) void k_bin_scan(const int *__restrict__ count, int n,
+                                                   int *__restrict__ start)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int a = min(n, t * per), b = min(n, a + per);
+    int sum = 0;
+    for (int k = a; k < b; ++k) sum += count[k];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int k = a; k < b; ++k) {
+        start[k] = run;
+        run += count[k];
+    }
+    if (t == 1023) start[n] = part[1023];
+}
+
+// (face, bin) pairs: each face writes its index into the lists of its bins
+__global__ void k_bin_fill(const FaceBox *__restrict__ box, int nf, int G,
+                           const int *__restrict__ start, int *__restrict__ cursor,
+                           int *__restrict__ list)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const FaceBox b = box[i];
+    for (int y = b.y0; y <= b.y1; ++y)
+        for (int x = b.x0; x <= b.x1; ++x) {
+            const int bin = y * G + x;
+            list[start[bin] + atomicAdd(&cursor[bin], 1)] = i;
+        }
+}
+
+// one thread per query point: the lowest-index face of its bin list and of
+// the "always" list that holds it; face -1 when none does
 __global__ __launch_bounds__(256) void k_locate(const double *__restrict__ uv,
                                                 const double *__restrict__ p3,
-                                                const int *__restrict__ tri, int nf,
+                                                const int *__restrict__ tri,
                                                 const int *__restrict__ list,
-                                                const int *__restrict__ count, int cap,
+                                                const int *__restrict__ start,
                                                 const int *__restrict__ alw, int nalw, BinGrid g,
                                                 const double *__restrict__ q, int nq,
                                                 int *__restrict__ face, double *__restrict__ w,
@@ -189,30 +207,23 @@ __global__ __launch_bounds__(256) void k_locate(const double *__restrict__ uv,
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nq) return;
     const double x = q[2 * k], y = q[2 * k + 1];
-    const int bin = bin_of(y, g.oy, g.inv, g.G) * g.G + bin_of(x, g.ox, g.inv, g.G);
-    const int cnt = count[bin];
-    const bool all = cnt > cap;                 // list overflowed: test everything
-    const int len = all ? nf : cnt;
     int hit = -1;
     double w0 = -1, w1 = -1, w2 = -1;
-    int ia = 0, il = 0;
     // a non-finite point is in no face
-    while (isfinite(x) && isfinite(y)) {
-        // next face in face order from the two sorted sources
-        const int cand_l = (il < len) ? (all ? il : list[(long)bin * cap + il]) : 0x7fffffff;
-        const int cand_a = (!all && ia < nalw) ? alw[ia] : 0x7fffffff;
-        int i;
-        if (cand_l == 0x7fffffff && cand_a == 0x7fffffff) break;
-        if (cand_l <= cand_a) { i = cand_l; ++il; if (cand_a == cand_l) ++ia; }
-        else { i = cand_a; ++ia; }
-        const int a = tri[3 * i], b = tri[3 * i + 1], c = tri[3 * i + 2];
-        double b0, b1, b2;
-        rec_bary(uv[2 * a], uv[2 * a + 1], uv[2 * b], uv[2 * b + 1], uv[2 * c], uv[2 * c + 1], x, y,
-                 b0, b1, b2);
-        if (b0 < 0 || b1 < 0 || b2 < 0) continue;
-        hit = i;
-        w0 = b0; w1 = b1; w2 = b2;
-        break;
+    if (isfinite(x) && isfinite(y)) {
+        const int bin = bin_of(y, g.oy, g.inv, g.G) * g.G + bin_of(x, g.ox, g.inv, g.G);
+        const int a = start[bin], n = start[bin + 1] - a;
+        for (int j = 0; j < n + nalw; ++j) {
+            const int i = (j < n) ? list[a + j] : alw[j - n];
+            if (hit >= 0 && i > hit) continue;      // a lower-index hit is known
+            const int p0 = tri[3 * i], p1 = tri[3 * i + 1], p2 = tri[3 * i + 2];
+            double b0, b1, b2;
+            rec_bary(uv[2 * p0], uv[2 * p0 + 1], uv[2 * p1], uv[2 * p1 + 1], uv[2 * p2],
+                     uv[2 * p2 + 1], x, y, b0, b1, b2);
+            if (b0 < 0 || b1 < 0 || b2 < 0) continue;
+            hit = i;
+            w0 = b0; w1 = b1; w2 = b2;
+        }
     }
     face[k] = hit;
     if (w) {
@@ -256,17 +267,15 @@ extern "C" int cvt_locate_points(const double *uv, const double *p3, int nv, con
     if (nq == 0) return CVT_OK;
     // about two faces per bin, at most 1024 x 1024 bins
     const int G = std::max(1, std::min(1024, (int)ceil(sqrt((double)std::max(nf, 1) / 2.0))));
-    const int nbins = G * G;
-    const int cap = 64;
     int rc = CVT_OK;
     hipStream_t st = nullptr;
     double *d_uv = nullptr, *d_p3 = nullptr, *d_q = nullptr, *d_w = nullptr, *d_out3 = nullptr,
            *d_bb = nullptr;
-    int *d_tri = nullptr, *d_list = nullptr, *d_count = nullptr, *d_alw = nullptr, *d_bad = nullptr,
-        *d_face = nullptr;
+    int *d_tri = nullptr, *d_list = nullptr, *d_count = nullptr, *d_start = nullptr,
+        *d_cursor = nullptr, *d_alw = nullptr, *d_bad = nullptr, *d_face = nullptr;
     FaceBox *d_box = nullptr;
     uint8_t *d_always = nullptr;
-    int bad = 0, nalw = 0;
+    int bad = 0, nalw = 0, total = 0, nbins = 0;
     double bb[4] = {0, 0, 0, 0};
     BinGrid g;
     std::vector<uint8_t> always;
@@ -279,8 +288,9 @@ extern "C" int cvt_locate_points(const double *uv, const double *p3, int nv, con
     LCHK(hipMalloc(&d_box, sizeof(FaceBox) * std::max(nf, 1)), "alloc");
     LCHK(hipMalloc(&d_always, std::max(nf, 1)), "alloc");
     LCHK(hipMalloc(&d_alw, sizeof(int) * std::max(nf, 1)), "alloc");
-    LCHK(hipMalloc(&d_list, sizeof(int) * (size_t)nbins * cap), "alloc");
-    LCHK(hipMalloc(&d_count, sizeof(int) * nbins), "alloc");
+    LCHK(hipMalloc(&d_count, sizeof(int) * (size_t)G * G), "alloc");
+    LCHK(hipMalloc(&d_cursor, sizeof(int) * (size_t)G * G), "alloc");
+    LCHK(hipMalloc(&d_start, sizeof(int) * ((size_t)G * G + 1)), "alloc");
     LCHK(hipMalloc(&d_bad, sizeof(int)), "alloc");
     LCHK(hipMalloc(&d_bb, sizeof(double) * 4), "alloc");
     if (w) LCHK(hipMalloc(&d_w, sizeof(double) * 3 * nq), "alloc");
@@ -308,9 +318,17 @@ extern "C" int cvt_locate_points(const double *uv, const double *p3, int nv, con
         g.inv = (ext > 0.0 && std::isfinite(ext)) ? (double)G / ext : 0.0;
         if (g.inv == 0.0) g.G = 1;
     }
-    if (nf > 0)
+    nbins = g.G * g.G;
+    LCHK(hipMemsetAsync(d_count, 0, sizeof(int) * nbins, st), "memset");
+    LCHK(hipMemsetAsync(d_cursor, 0, sizeof(int) * nbins, st), "memset");
+    if (nf > 0) {
         hipLaunchKernelGGL(k_face_boxes, dim3((nf + 255) / 256), dim3(256), 0, st, d_uv, nv, d_tri,
                            nf, g, d_box, d_always, d_bad);
+        hipLaunchKernelGGL(k_bin_count, dim3((nf + 255) / 256), dim3(256), 0, st, d_box, nf, g.G,
+                           d_count);
+    }
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, d_count, nbins, d_start);
+    LCHK(hipMemcpyAsync(&total, d_start + nbins, sizeof(int), hipMemcpyDeviceToHost, st), "total");
     LCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st), "flag");
     always.resize(std::max(nf, 1));
     if (nf > 0) LCHK(hipMemcpyAsync(always.data(), d_always, nf, hipMemcpyDeviceToHost, st), "flags");
@@ -320,10 +338,12 @@ extern "C" int cvt_locate_points(const double *uv, const double *p3, int nv, con
         if (always[i]) alw.push_back(i);
     nalw = (int)alw.size();
     if (nalw) LCHK(hipMemcpyAsync(d_alw, alw.data(), sizeof(int) * nalw, hipMemcpyHostToDevice, st), "upload");
-    hipLaunchKernelGGL(k_bin_lists, dim3(g.G * g.G), dim3(256), 0, st, d_box, nf, g.G, d_list,
-                       d_count, cap);
-    hipLaunchKernelGGL(k_locate, dim3((nq + 255) / 256), dim3(256), 0, st, d_uv, d_p3, d_tri, nf,
-                       d_list, d_count, cap, d_alw, nalw, g, d_q, nq, d_face, d_w, d_out3);
+    LCHK(hipMalloc(&d_list, sizeof(int) * (size_t)std::max(total, 1)), "alloc");
+    if (nf > 0)
+        hipLaunchKernelGGL(k_bin_fill, dim3((nf + 255) / 256), dim3(256), 0, st, d_box, nf, g.G,
+                           d_start, d_cursor, d_list);
+    hipLaunchKernelGGL(k_locate, dim3((nq + 255) / 256), dim3(256), 0, st, d_uv, d_p3, d_tri,
+                       d_list, d_start, d_alw, nalw, g, d_q, nq, d_face, d_w, d_out3);
     LCHK(hipGetLastError(), "locate launch");
     LCHK(hipMemcpyAsync(face, d_face, sizeof(int) * nq, hipMemcpyDefault, st), "download");
     if (w) LCHK(hipMemcpyAsync(w, d_w, sizeof(double) * 3 * nq, hipMemcpyDefault, st), "download");
@@ -331,8 +351,8 @@ extern "C" int cvt_locate_points(const double *uv, const double *p3, int nv, con
         LCHK(hipMemcpyAsync(out3, d_out3, sizeof(double) * 3 * nq, hipMemcpyDefault, st), "download");
     LCHK(hipStreamSynchronize(st), "locate");
 done:
-    void *bufs[] = {d_uv, d_p3, d_q, d_w, d_out3, d_bb, d_tri, d_list, d_count, d_alw, d_bad,
-                    d_face, d_box, d_always};
+    void *bufs[] = {d_uv, d_p3, d_q, d_w, d_out3, d_bb, d_tri, d_list, d_count, d_start,
+                    d_cursor, d_alw, d_bad, d_face, d_box, d_always};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (st) (void)hipStreamDestroy(st);

@@ -124,3 +124,12 @@ def test_recover_lifts_sites_and_filters_triangles(gpu):
     cen = recover.centroids(points, cdt)
     fc, _ = orc.locate(uv, tri, cen)
     assert np.array_equal(faces, cdt[fc >= 0])
+
+
+def test_locate_large_mesh(gpu):
+    # 358K faces: the binning is a counting sort, linear in (face, bin) pairs
+    uv, tri = jittered_mesh(424, 13)
+    g = np.random.default_rng(14)
+    q = np.concatenate([g.uniform(-1.0, 424.0, (2000, 2)), uv[g.integers(0, len(uv), 200)]])
+    face = check(uv, tri, q)[0]
+    assert (face >= 0).mean() > 0.9
