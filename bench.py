@@ -203,6 +203,45 @@ def unet_mfma(batch: int = 16, steps: int = 10) -> dict:
             "train_frac": round(t_tf / BF16_PEAK_TFLOPS, 5)}
 
 
+def raster_sample(with_cpu: bool) -> dict:
+    """The density rasterizer (discretization_d rule) on the patch mesh of
+    locate_sample, scaled to the grid: 11858 triangles at 2048^2 on the GPU
+    (host buffers in and out, as the reference's caller passes them); the
+    oracle's triangle-by-triangle scan on the host cores at 256^2 (a bounded
+    sample of the same mesh), compared per pixel, with a bit-exactness check
+    of the GPU at that size."""
+    import numpy as np
+    from cvtamd import compat, workloads as wl
+    uv, p3, tri, _ = wl.patch_mesh()
+    wt = 1.0 + p3[:, 2]                          # a smooth positive vertex weight
+    n = 2048
+    scale = 1.0 / (n - 1)                        # pixel corners span the unit patch
+    compat.rasterize(uv, wt, tri, scale, n)     # warm-up
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dens = compat.rasterize(uv, wt, tri, scale, n)
+    gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    res = {"grid": n, "triangles": int(len(tri)), "gpu_ms": round(gpu_ms, 3),
+           "gpu_pixels_per_s": round(n * n / gpu_ms * 1e3, 1),
+           "covered": round(float((dens > 0).mean()), 4)}
+    if with_cpu:
+        from oracle import oracle as orc
+        orc.set_threads(min(orc.max_threads(), int(os.environ.get("OMP_NUM_THREADS", "64"))))
+        ns = 256
+        ss = 1.0 / (ns - 1)
+        t0 = time.perf_counter()
+        ref = orc.raster(uv, wt, tri, ss, ns)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        got = compat.rasterize(uv, wt, tri, ss, ns)
+        cpu_pps = ns * ns / cpu_ms * 1e3
+        res.update({"cpu_sample": f"{ns}^2", "cpu_ms": round(cpu_ms, 1),
+                    "cpu_threads": orc.max_threads(), "cpu_pixels_per_s": round(cpu_pps, 1),
+                    "bit_exact_at_sample": bool(np.array_equal(got, ref)),
+                    "speedup_per_pixel": round(res["gpu_pixels_per_s"] / cpu_pps, 1)})
+    return res
+
+
 def locate_sample(with_cpu: bool) -> dict:
     """recover()'s point location on one patch mesh: 49152 queries (16384
     sites + 32768 triangle centroids) in a 11858-face mesh, on the GPU
@@ -297,7 +336,7 @@ def main():
                     help="time-to-tolerance of the learned warm start vs plain Lloyd on K "
                          "held-out 2048^2 / 16K patches (0 = skip; needs trained weights)")
     ap.add_argument("--no-locate", dest="locate", action="store_false",
-                    help="skip the recover() point-location sample")
+                    help="skip the rasterizer and recover() point-location samples")
     ap.add_argument("--unet", type=int, default=16, metavar="B",
                     help="MFMA throughput of the warm-start network at batch B (0 = skip)")
     ap.add_argument("--unet-only", action="store_true",
@@ -401,6 +440,7 @@ def main():
     if args.ttt > 0 and args.workload == "c2":
         out["time_to_tol"] = time_to_tol_sample(args.ttt)
     if args.locate and args.workload == "c2":
+        out["raster"] = raster_sample(not args.no_cpu)
         out["locate"] = locate_sample(not args.no_cpu)
     if args.unet > 0 and args.workload == "c2":
         out["mfma"] = unet_mfma(args.unet)
