@@ -14,6 +14,9 @@ void clear_error();
 const char *last_error();
 int jfa_first_step(int W, int H);
 int jfa_num_passes(int W, int H);
+// exclusive prefix sum of n counts into start[0 .. n] on stream st (one
+// workgroup; for bin lists of at most a few million entries)
+void launch_exclusive_scan(hipStream_t st, const int *count, int n, int *start);
 
 struct Ctx {
     int W = 0, H = 0, maxS = 0, S = 0;

@@ -245,6 +245,11 @@ __global__ __launch_bounds__(256) void k_locate(const double *__restrict__ uv,
     }
 }
 
+void launch_exclusive_scan(hipStream_t st, const int *count, int n, int *start)
+{
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, count, n, start);
+}
+
 }  // namespace cvt
 
 using namespace cvt;

@@ -8,13 +8,15 @@
 // Degenerate triangles (zero denominator) never hit.
 //
 // Instead of testing every triangle at every pixel (O(n^2 F)), triangles are
-// binned into 16 x 16 pixel tiles by a conservative bounding box, and each
-// tile keeps its list in increasing triangle order, so the first hit in the
-// list is the first hit overall.  A triangle so thin that rounding could
+// binned into 16 x 16 pixel tiles by a conservative bounding box (a counting
+// sort: count, prefix sum, fill; linear in the number of (triangle, tile)
+// pairs).  A pixel tests its tile's triangles and keeps the lowest-index hit,
+// which is the first hit in index order: every triangle holding the sample
+// point lists it in that tile.  A triangle so thin that rounding could
 // accept points well outside its box (sin^2 of its angle at vertex 1 below
-// 1e-12) goes to an "always" list that every pixel tests, merged in index
-// order with its tile list.  The barycentric arithmetic is the oracle's,
-// operation for operation (contraction is off for this file).
+// 1e-12) goes to an "always" list that every pixel tests.  The barycentric
+// arithmetic is the oracle's, operation for operation (contraction is off
+// for this file).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -97,79 +99,73 @@ __global__ void k_tri_boxes(const double *__restrict__ pts, int nv, const int *_
     box[i] = bx;
 }
 
-// per tile: the triangles whose box meets the tile, in index order
-// (ballot + prefix count keeps the order); a tile with more than cap
-// triangles is flagged and later tests every triangle
-__global__ __launch_bounds__(256) void k_tile_lists(const TriBox *__restrict__ box, int nt,
-                                                    int tiles_x, int *__restrict__ list,
-                                                    int *__restrict__ count, int cap)
+// (triangle, tile) pairs of a pixel box, as tile ranges
+__device__ __forceinline__ void tile_range(const TriBox &b, int &tx0, int &tx1, int &ty0, int &ty1)
 {
-    const int tile = blockIdx.x;
-    const int tx0 = (tile % tiles_x) * RT, ty0 = (tile / tiles_x) * RT;
-    const int tx1 = tx0 + RT - 1, ty1 = ty0 + RT - 1;
-    __shared__ int wave_n[4];
-    __shared__ int total;
-    if (threadIdx.x == 0) total = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int base = 0; base < nt; base += 256) {
-        const int i = base + threadIdx.x;
-        bool hit = false;
-        if (i < nt) {
-            const TriBox b = box[i];
-            hit = b.x0 <= b.x1 && b.x0 <= tx1 && b.x1 >= tx0 && b.y0 <= ty1 && b.y1 >= ty0;
-        }
-        const unsigned long long m = __ballot(hit);
-        if (lane == 0) wave_n[wave] = __popcll(m);
-        __syncthreads();
-        int off = total;
-        for (int w = 0; w < wave; ++w) off += wave_n[w];
-        if (hit) {
-            const int k = off + __popcll(m & ((1ull << lane) - 1ull));
-            if (k < cap) list[(long)tile * cap + k] = i;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) total += wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) count[tile] = total;
+    tx0 = b.x0 / RT; tx1 = b.x1 / RT;
+    ty0 = b.y0 / RT; ty1 = b.y1 / RT;
 }
 
-// one workgroup per tile, one thread per pixel: walk the tile list merged with
-// the "always" list in index order until the first hit
+// (triangle, tile) pairs: count per tile
+__global__ void k_tile_count(const TriBox *__restrict__ box, int nt, int tiles_x,
+                             int *__restrict__ count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const TriBox b = box[i];
+    if (b.x0 > b.x1 || b.y0 > b.y1) return;
+    int tx0, tx1, ty0, ty1;
+    tile_range(b, tx0, tx1, ty0, ty1);
+    for (int y = ty0; y <= ty1; ++y)
+        for (int x = tx0; x <= tx1; ++x) atomicAdd(&count[y * tiles_x + x], 1);
+}
+
+// (triangle, tile) pairs: each triangle writes its index into its tiles' lists
+__global__ void k_tile_fill(const TriBox *__restrict__ box, int nt, int tiles_x,
+                            const int *__restrict__ start, int *__restrict__ cursor,
+                            int *__restrict__ list)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const TriBox b = box[i];
+    if (b.x0 > b.x1 || b.y0 > b.y1) return;
+    int tx0, tx1, ty0, ty1;
+    tile_range(b, tx0, tx1, ty0, ty1);
+    for (int y = ty0; y <= ty1; ++y)
+        for (int x = tx0; x <= tx1; ++x) {
+            const int tile = y * tiles_x + x;
+            list[start[tile] + atomicAdd(&cursor[tile], 1)] = i;
+        }
+}
+
+// one workgroup per tile, one thread per pixel: the lowest-index triangle of
+// the tile list and of the "always" list holding the sample point
 __global__ __launch_bounds__(256) void k_raster(const double *__restrict__ pts,
                                                 const double *__restrict__ wt,
-                                                const int *__restrict__ tri, int nt,
+                                                const int *__restrict__ tri,
                                                 const int *__restrict__ list,
-                                                const int *__restrict__ count, int cap,
+                                                const int *__restrict__ start,
                                                 const int *__restrict__ alw, int nalw, int tiles_x,
                                                 double scale, int n, float *__restrict__ density)
 {
     const int tile = blockIdx.x;
     const int tx = (tile % tiles_x) * RT + (threadIdx.x & (RT - 1));
     const int ty = (tile / tiles_x) * RT + (threadIdx.x / RT);
-    const int cnt = count[tile];
-    const bool all = cnt > cap;                 // list overflowed: test everything
-    const int len = all ? nt : cnt;
     if (tx >= n || ty >= n) return;
+    const int a = start[tile], len = start[tile + 1] - a;
     const double x = tx * scale, y = ty * scale;
     float res = 0.0f;
-    int ia = 0, il = 0;
-    while (true) {
-        // next triangle in index order from the two sorted sources
-        int cand_l = (il < len) ? (all ? il : list[(long)tile * cap + il]) : 0x7fffffff;
-        int cand_a = (!all && ia < nalw) ? alw[ia] : 0x7fffffff;
-        int i;
-        if (cand_l == 0x7fffffff && cand_a == 0x7fffffff) break;
-        if (cand_l <= cand_a) { i = cand_l; ++il; if (cand_a == cand_l) ++ia; }
-        else { i = cand_a; ++ia; }
+    int hit = -1;
+    for (int j = 0; j < len + nalw; ++j) {
+        const int i = (j < len) ? list[a + j] : alw[j - len];
+        if (hit >= 0 && i > hit) continue;          // a lower-index hit is known
         const int p1 = tri[3 * i], p2 = tri[3 * i + 1], p3 = tri[3 * i + 2];
         double b1, b2, b3;
         bary(pts[2 * p1], pts[2 * p1 + 1], pts[2 * p2], pts[2 * p2 + 1], pts[2 * p3],
              pts[2 * p3 + 1], x, y, b1, b2, b3);
         if (b1 < 0 || b2 < 0 || b3 < 0) continue;
         res = (float)(b1 * wt[p1] + b2 * wt[p2] + b3 * wt[p3]);
-        break;
+        hit = i;
     }
     density[(long)ty * n + tx] = res;
 }
@@ -194,15 +190,15 @@ extern "C" int cvt_rasterize(const double *points, const double *weight, int num
         return fail(CVT_ERR_ARG, "null array");
     if (!(scale > 0.0) || !isfinite(scale)) return fail(CVT_ERR_ARG, "scale %g", scale);
     const int tiles_x = (n + RT - 1) / RT, ntiles = tiles_x * tiles_x;
-    const int cap = 256;
     int rc = CVT_OK;
     hipStream_t st = nullptr;
     double *d_pts = nullptr, *d_wt = nullptr;
-    int *d_tri = nullptr, *d_list = nullptr, *d_count = nullptr, *d_alw = nullptr, *d_bad = nullptr;
+    int *d_tri = nullptr, *d_list = nullptr, *d_count = nullptr, *d_start = nullptr,
+        *d_cursor = nullptr, *d_alw = nullptr, *d_bad = nullptr;
     TriBox *d_box = nullptr;
     uint8_t *d_always = nullptr;
     float *d_den = nullptr;
-    int bad = 0, nalw = 0;
+    int bad = 0, nalw = 0, total = 0;
     std::vector<uint8_t> always;
     std::vector<int> alw;
     const long npx = (long)n * n;
@@ -213,8 +209,9 @@ extern "C" int cvt_rasterize(const double *points, const double *weight, int num
     RCHK(hipMalloc(&d_box, sizeof(TriBox) * std::max(num_tri, 1)), "alloc");
     RCHK(hipMalloc(&d_always, std::max(num_tri, 1)), "alloc");
     RCHK(hipMalloc(&d_alw, sizeof(int) * std::max(num_tri, 1)), "alloc");
-    RCHK(hipMalloc(&d_list, sizeof(int) * (size_t)ntiles * cap), "alloc");
     RCHK(hipMalloc(&d_count, sizeof(int) * ntiles), "alloc");
+    RCHK(hipMalloc(&d_cursor, sizeof(int) * ntiles), "alloc");
+    RCHK(hipMalloc(&d_start, sizeof(int) * ((size_t)ntiles + 1)), "alloc");
     RCHK(hipMalloc(&d_bad, sizeof(int)), "alloc");
     RCHK(hipMalloc(&d_den, sizeof(float) * npx), "alloc");
     if (num_point > 0) {
@@ -224,9 +221,16 @@ extern "C" int cvt_rasterize(const double *points, const double *weight, int num
     if (num_tri > 0)
         RCHK(hipMemcpyAsync(d_tri, triangle, sizeof(int) * 3 * num_tri, hipMemcpyDefault, st), "upload");
     RCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st), "memset");
-    if (num_tri > 0)
+    RCHK(hipMemsetAsync(d_count, 0, sizeof(int) * ntiles, st), "memset");
+    RCHK(hipMemsetAsync(d_cursor, 0, sizeof(int) * ntiles, st), "memset");
+    if (num_tri > 0) {
         hipLaunchKernelGGL(k_tri_boxes, dim3((num_tri + 255) / 256), dim3(256), 0, st, d_pts,
                            num_point, d_tri, num_tri, scale, n, d_box, d_always, d_bad);
+        hipLaunchKernelGGL(k_tile_count, dim3((num_tri + 255) / 256), dim3(256), 0, st, d_box,
+                           num_tri, tiles_x, d_count);
+    }
+    launch_exclusive_scan(st, d_count, ntiles, d_start);
+    RCHK(hipMemcpyAsync(&total, d_start + ntiles, sizeof(int), hipMemcpyDeviceToHost, st), "total");
     RCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st), "flag");
     always.resize(std::max(num_tri, 1));
     if (num_tri > 0)
@@ -237,15 +241,18 @@ extern "C" int cvt_rasterize(const double *points, const double *weight, int num
         if (always[i]) alw.push_back(i);
     nalw = (int)alw.size();
     if (nalw) RCHK(hipMemcpyAsync(d_alw, alw.data(), sizeof(int) * nalw, hipMemcpyHostToDevice, st), "upload");
-    hipLaunchKernelGGL(k_tile_lists, dim3(ntiles), dim3(256), 0, st, d_box, num_tri, tiles_x, d_list,
-                       d_count, cap);
-    hipLaunchKernelGGL(k_raster, dim3(ntiles), dim3(256), 0, st, d_pts, d_wt, d_tri, num_tri, d_list,
-                       d_count, cap, d_alw, nalw, tiles_x, scale, n, d_den);
+    RCHK(hipMalloc(&d_list, sizeof(int) * (size_t)std::max(total, 1)), "alloc");
+    if (num_tri > 0)
+        hipLaunchKernelGGL(k_tile_fill, dim3((num_tri + 255) / 256), dim3(256), 0, st, d_box,
+                           num_tri, tiles_x, d_start, d_cursor, d_list);
+    hipLaunchKernelGGL(k_raster, dim3(ntiles), dim3(256), 0, st, d_pts, d_wt, d_tri, d_list,
+                       d_start, d_alw, nalw, tiles_x, scale, n, d_den);
     RCHK(hipGetLastError(), "raster launch");
     RCHK(hipMemcpyAsync(density, d_den, sizeof(float) * npx, hipMemcpyDefault, st), "download");
     RCHK(hipStreamSynchronize(st), "raster");
 done:
-    void *bufs[] = {d_pts, d_wt, d_tri, d_box, d_always, d_alw, d_list, d_count, d_bad, d_den};
+    void *bufs[] = {d_pts, d_wt, d_tri, d_box, d_always, d_alw, d_list, d_count, d_start,
+                    d_cursor, d_bad, d_den};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (st) (void)hipStreamDestroy(st);
