@@ -483,6 +483,20 @@ __device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__
 
 
 
+// The fixed pairwise tree v[t] += v[t + o], o = 32, 16, ..., 1, over the 64
+// lanes of a wave (lane 0 ends with the sum).  Shuffles instead of LDS
+// steps: the same additions in the same order, so the same bits as the LDS
+// tree it replaces.
+__device__ __forceinline__ double wave_tree_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double u = __shfl_down(v, o, 64);
+        if ((threadIdx.x & 63) < (unsigned)o) v += u;
+    }
+    return v;
+}
+
 static __global__ __launch_bounds__(256) void k_update(
     float2 *__restrict__ sites, const uint8_t *__restrict__ fixed, int S,
     const Slot *__restrict__ slots, long nrows, const OvfRec *__restrict__ ovf,
@@ -538,15 +552,21 @@ static __global__ __launch_bounds__(256) void k_update(
             atomicMin(&seed[(long)y * W + x], 8u * (uint32_t)i);
         }
     }
+    // block sums: tree steps 128 and 64 in LDS, 32 .. 1 in wave 0
     red_e[threadIdx.x] = e;
     red_g[threadIdx.x] = g;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = 128; o >= 64; o >>= 1) {
         if (threadIdx.x < o) {
             red_e[threadIdx.x] += red_e[threadIdx.x + o];
             red_g[threadIdx.x] += red_g[threadIdx.x + o];
         }
         __syncthreads();
+    }
+    double be = 0.0, bg = 0.0;
+    if (threadIdx.x < 64) {
+        be = wave_tree_sum(red_e[threadIdx.x]);
+        bg = wave_tree_sum(red_g[threadIdx.x]);
     }
     // every chain was consumed above: the record pool restarts empty
     if (blockIdx.x == 0 && threadIdx.x == 0) dstate[ST_OVFN] = 0;
@@ -554,34 +574,38 @@ static __global__ __launch_bounds__(256) void k_update(
     // them in block order (the release / acquire pair at agent scope makes
     // every block's partials visible to it, whichever XCD wrote them)
     if (threadIdx.x == 0) {
-        blockpart[2 * blockIdx.x] = red_e[0];
-        blockpart[2 * blockIdx.x + 1] = red_g[0];
+        blockpart[2 * blockIdx.x] = be;
+        blockpart[2 * blockIdx.x + 1] = bg;
         const int done = __hip_atomic_fetch_add(&dstate[ST_DONE], 1, __ATOMIC_ACQ_REL,
                                                 __HIP_MEMORY_SCOPE_AGENT);
         last = (done == (int)gridDim.x - 1);
     }
     __syncthreads();
     if (!last) return;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // thread t sums partials t, t + 256, ...; then the tree of 256 (steps
+    // 128 and 64 in LDS, 32 .. 1 in wave 0)
     double te = 0.0, tg = 0.0;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) {
         te += __hip_atomic_load(&blockpart[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tg += __hip_atomic_load(&blockpart[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
     red_e[threadIdx.x] = te;
     red_g[threadIdx.x] = tg;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = 128; o >= 64; o >>= 1) {
         if (threadIdx.x < o) {
             red_e[threadIdx.x] += red_e[threadIdx.x + o];
             red_g[threadIdx.x] += red_g[threadIdx.x + o];
         }
         __syncthreads();
     }
+    if (threadIdx.x >= 64) return;
+    const double se = wave_tree_sum(red_e[threadIdx.x]);
+    const double sg = wave_tree_sum(red_g[threadIdx.x]);
     if (threadIdx.x == 0) {
-        const double E = red_e[0] * fa.e_scale;
-        const double G = sqrt(red_g[0]) * fa.g_scale;
+        const double E = se * fa.e_scale;
+        const double G = sqrt(sg) * fa.g_scale;
         if (fa.record) {
             const int h = dstate[ST_HIST];
             if (h < fa.hist_cap) { fa.hist[2 * h] = E; fa.hist[2 * h + 1] = G; }
