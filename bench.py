@@ -275,6 +275,40 @@ def locate_sample(with_cpu: bool) -> dict:
     return res
 
 
+def weights_sample(with_cpu: bool) -> dict:
+    """Per-vertex density weights (weighting_vertices_with_area rule) on the
+    patch mesh of locate_sample (11858 faces) and on a 1024 x 1024 vertex
+    patch (2.09M faces), inputs resident in device memory, the call
+    returning when the weights are written; the baseline is the oracle's
+    face loop, single-threaded as the reference's host loop is (its sums
+    are sequential in face order)."""
+    import numpy as np
+    import torch
+    from cvtamd import weighting, workloads as wl
+    res = {}
+    for name, m, reps in (("patch", 78, 20), ("large", 1024, 5)):
+        uv, p3, tri, _ = wl.patch_mesh(m=m)
+        d = [torch.from_numpy(a).cuda() for a in (uv, p3, tri)]
+        weighting.vertex_weights(*d)                          # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            w = weighting.vertex_weights(*d)
+        gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+        r = {"faces": int(len(tri)), "vertices": int(len(uv)), "gpu_ms": round(gpu_ms, 3),
+             "gpu_faces_per_s": round(len(tri) / gpu_ms * 1e3, 1)}
+        if with_cpu:
+            from oracle import oracle as orc
+            t0 = time.perf_counter()
+            ref = orc.vertex_weights(uv, p3, tri)
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            r.update({"cpu_ms": round(cpu_ms, 3), "cpu_threads": 1,
+                      "cpu_kind": "port (face loop, as source/weighting.h:53-92)",
+                      "bit_exact": bool(np.array_equal(w, ref, equal_nan=True)),
+                      "speedup": round(cpu_ms / gpu_ms, 2)})
+        res[name] = r
+    return res
+
+
 def bench_batch(args):
     """The 64-patch batch: one step = one batched Lloyd iteration over every
     patch (one graph launch); value = batched iterations/s, with the
@@ -336,7 +370,7 @@ def main():
                     help="time-to-tolerance of the learned warm start vs plain Lloyd on K "
                          "held-out 2048^2 / 16K patches (0 = skip; needs trained weights)")
     ap.add_argument("--no-locate", dest="locate", action="store_false",
-                    help="skip the rasterizer and recover() point-location samples")
+                    help="skip the rasterizer, vertex-weight and recover() point-location samples")
     ap.add_argument("--unet", type=int, default=16, metavar="B",
                     help="MFMA throughput of the warm-start network at batch B (0 = skip)")
     ap.add_argument("--unet-only", action="store_true",
@@ -442,6 +476,7 @@ def main():
     if args.locate and args.workload == "c2":
         out["raster"] = raster_sample(not args.no_cpu)
         out["locate"] = locate_sample(not args.no_cpu)
+        out["weights"] = weights_sample(not args.no_cpu)
     if args.unet > 0 and args.workload == "c2":
         out["mfma"] = unet_mfma(args.unet)
     if not args.no_cpu:

@@ -41,6 +41,8 @@
  *   cvt_locate_points             locate() and the vertex interpolation of
  *                                 recover() (source/recover.h:63-81,
  *                                 :109-125, :134-141)
+ *   cvt_vertex_weights            weighting_vertices_with_area
+ *                                 (source/weighting.h:37-97)
  */
 #ifndef CVT_H
 #define CVT_H
@@ -167,6 +169,20 @@ int cvt_extract_points(const int16_t *voronoi, const uint8_t *mask, int n, doubl
  * Host or device pointers; the call returns when the results are written. */
 int cvt_locate_points(const double *uv, const double *p3, int nv, const int *tri, int nf,
                       const double *q, int nq, int *face, double *w, double *out3);
+
+/* Per-vertex density weights of a parameterized mesh, the rasterizer's vertex
+ * weights (replaces weighting_vertices_with_area, source/weighting.h:37-97):
+ * weight[v] = (sum of the 3D areas of v's faces) / (sum of their 2D areas),
+ * each sum taken in face order; a weight above 1e10 is replaced by the
+ * largest other weight (0 if there is none); a vertex in no face gets NaN.
+ * uv: [nv][2] 2D vertices; p3: [nv][3] the 3D position of each 2D vertex;
+ * tri: [nf][3] vertex indices in vertices_around_face order (CVT_ERR_ARG if
+ * one is outside [0, nv)); weight: [nv].  The 2D area is get_area
+ * (source/weighting.h:29-35); the 3D area is CGAL's PMP::face_area on the
+ * points (tri[2], tri[0], tri[1]).  Host or device pointers; the call
+ * returns when weight is written. */
+int cvt_vertex_weights(const double *uv, const double *p3, int nv, const int *tri, int nf,
+                       double *weight);
 
 /* ---------------------------------------------------------------------------
  * Batched CVT: many independent patches of different sizes advanced

@@ -42,6 +42,11 @@
  *                         sum w_j uv_j == q, first face in order on shared
  *                         edges and overlaps, degenerate faces never hit,
  *                         linear 3D fields reproduced by the interpolation.
+ *   orc_vertex_weights -- closed forms: a flat copy of the mesh gives 1, a
+ *                         mesh scaled by a gives a^2, one tilted by z = k u
+ *                         gives sqrt(1 + k^2), a hand-summed two-face fan,
+ *                         the > 1e10 rule (largest other weight, 0 when
+ *                         there is none), NaN for a vertex in no face.
  */
 #include <math.h>
 #include <stdint.h>
@@ -697,6 +702,68 @@ void orc_locate(const double *uv, const double *p3, const int *tri, int nf, cons
             out3[3 * k + 2] = r[2];
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Vertex density weights                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* The 3D face area as CGAL 5.0.2 computes it (PMP::face_area: the square root
+ * of Compute_squared_area_3 on source(h), target(h), target(next(h)) of the
+ * face's halfedge h; the squared area is the squared length of the cross
+ * product of q - p and r - p, over 4).  A face listed as vertices_around_face
+ * (target(h), target(next(h)), source(h)) = (t0, t1, t2) hands it the points
+ * (p, q, r) = (t2, t0, t1). */
+static double orc_area3(const double *p3, const int *t)
+{
+    const double *p = p3 + 3 * t[2], *q = p3 + 3 * t[0], *r = p3 + 3 * t[1];
+    double ux = q[0] - p[0], uy = q[1] - p[1], uz = q[2] - p[2];
+    double vx = r[0] - p[0], vy = r[1] - p[1], vz = r[2] - p[2];
+    double cx = uy * vz - uz * vy;
+    double cy = uz * vx - ux * vz;
+    double cz = ux * vy - uy * vx;
+    return sqrt((cx * cx + cy * cy + cz * cz) / 4);
+}
+
+/* get_area (source/weighting.h:29-35): |cross(B - A, C - A)| / 2 with
+ * cross(P, Q) = P.x Q.y - Q.x P.y, (A, B, C) the face's vertices in order. */
+static double orc_area2(const double *uv, const int *t)
+{
+    const double *a = uv + 2 * t[0], *b = uv + 2 * t[1], *c = uv + 2 * t[2];
+    double px = b[0] - a[0], py = b[1] - a[1];
+    double qx = c[0] - a[0], qy = c[1] - a[1];
+    return fabs(px * qy - qx * py) / 2.;
+}
+
+/* weighting_vertices_with_area (source/weighting.h:37-97) on arrays: each
+ * face adds its 3D area and its 2D area to each of its corners' sums (in
+ * face order, :53-74); a vertex's weight is area3 / area2 (:79); weights
+ * above 1e10 are replaced by the largest other weight, which starts at 0
+ * (:76-92).  A vertex in no face gets 0 / 0 = NaN, which takes part in
+ * neither test.  Both sums run in the order of tri (the reference sums the
+ * 3D areas in the seam mesh's face order and the 2D ones in mesh_2d's). */
+void orc_vertex_weights(const double *uv, const double *p3, int nv, const int *tri, int nf,
+                        double *weight)
+{
+    double *a3 = (double *)calloc((size_t)(nv > 0 ? nv : 1), sizeof(double));
+    double *a2 = (double *)calloc((size_t)(nv > 0 ? nv : 1), sizeof(double));
+    for (int f = 0; f < nf; ++f) {
+        const int *t = tri + 3 * f;
+        const double s3 = orc_area3(p3, t), s2 = orc_area2(uv, t);
+        for (int c = 0; c < 3; ++c) a3[t[c]] += s3;
+        for (int c = 0; c < 3; ++c) a2[t[c]] += s2;
+    }
+    double big = 1e10, mx = 0;
+    for (int v = 0; v < nv; ++v) {
+        double w = a3[v] / a2[v];
+        weight[v] = w;
+        if (w > big) continue;
+        if (w > mx) mx = w;
+    }
+    for (int v = 0; v < nv; ++v)
+        if (weight[v] > big) weight[v] = mx;
+    free(a3);
+    free(a2);
 }
 
 /* ------------------------------------------------------------------------ */

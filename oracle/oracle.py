@@ -70,6 +70,7 @@ def lib():
         "orc_extract_points": (C.c_long, [_i16p, _u8p, C.c_int, C.c_double, C.c_double,
                                           C.c_double, _f64p]),
         "orc_locate": (None, [_f64p, vp, _i32p, C.c_int, _f64p, C.c_int, _i32p, vp, vp]),
+        "orc_vertex_weights": (None, [_f64p, _f64p, C.c_int, _i32p, C.c_int, _f64p]),
         "orc_max_threads": (C.c_int, []),
         "orc_set_threads": (None, [C.c_int]),
     }
@@ -278,6 +279,18 @@ def locate(uv, tri, q, p3=None):
     lib().orc_locate(uv, _ptr(p3c), tr, tr.shape[0] // 3, q.reshape(-1), nq, face, _ptr(w),
                      _ptr(out3))
     return (face, w) if p3 is None else (face, w, out3)
+
+
+def vertex_weights(uv, p3, tri):
+    """Per-vertex density weight, 3D area over 2D area of the incident faces,
+    with the > 1e10 rule (source/weighting.h:37-97)."""
+    uv = np.ascontiguousarray(uv, np.float64).reshape(-1)
+    p3 = np.ascontiguousarray(p3, np.float64).reshape(-1)
+    tr = np.ascontiguousarray(tri, np.int32).reshape(-1)
+    nv = uv.shape[0] // 2
+    out = np.empty(nv, np.float64)
+    lib().orc_vertex_weights(uv, p3, nv, tr, tr.shape[0] // 3, out)
+    return out
 
 
 def set_threads(t):

@@ -14,8 +14,8 @@ void clear_error();
 const char *last_error();
 int jfa_first_step(int W, int H);
 int jfa_num_passes(int W, int H);
-// exclusive prefix sum of n counts into start[0 .. n] on stream st (one
-// workgroup; for bin lists of at most a few million entries)
+// exclusive prefix sum of n counts into start[0 .. n] on stream st (three
+// launches over tiles of 4096 counts; count must be 16-byte aligned)
 void launch_exclusive_scan(hipStream_t st, const int *count, int n, int *start);
 
 struct Ctx {

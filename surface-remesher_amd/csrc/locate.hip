@@ -149,32 +149,98 @@ __global__ void k_bin_count(const FaceBox *__restrict__ box, int nf, int G,
         for (int x = b.x0; x <= b.x1; ++x) atomicAdd(&count[y * G + x], 1);
 }
 
-// exclusive prefix sum of n counts into start[0 .. n] (one workgroup: each
-// thread sums a contiguous chunk, the chunk totals are scanned in LDS, then
-// each thread writes its chunk's starts)
-__global__ __launch_bounds__(1024) void k_bin_scan(const int *__restrict__ count, int n,
-                                                   int *__restrict__ start)
+// Exclusive prefix sum of n counts into start[0 .. n], in three launches
+// over tiles of 4096 counts (1024 threads x 4 consecutive counts):
+//   k_scan_tiles        each tile's total, parked in start[tile * 4096];
+//   k_scan_tile_totals  one workgroup turns the parked totals into the tiles'
+//                       offsets in place and writes start[n];
+//   k_scan_apply        each tile reads its offset, then writes its starts
+//                       (its first entry, the parked offset, last).
+constexpr int kScanTile = 4096;
+
+// exclusive prefix sum over a 1024-thread workgroup of one value per thread;
+// total receives the workgroup's sum (lds: 32 ints)
+__device__ __forceinline__ int wg_exclusive_scan(int v, int *lds, int &total)
 {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int a = min(n, t * per), b = min(n, a + per);
-    int sum = 0;
-    for (int k = a; k < b; ++k) sum += count[k];
-    part[t] = sum;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    __syncthreads();                       // lds is free (a previous call has read it)
+    if (lane == 63) lds[wv] = x;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = (t >= off) ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    if (wv == 0) {
+        int t = (lane < 16) ? lds[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(t, o, 64);
+            if (lane >= o) t += y;
+        }
+        if (lane < 16) lds[16 + lane] = t;
     }
-    int run = part[t] - sum;
-    for (int k = a; k < b; ++k) {
-        start[k] = run;
-        run += count[k];
+    __syncthreads();
+    total = lds[31];
+    return (wv > 0 ? lds[15 + wv] : 0) + x - v;
+}
+
+// the thread's 4 counts (0 past n)
+__device__ __forceinline__ int4 tile_counts(const int *__restrict__ count, long base, int n)
+{
+    if (base + 3 < n) return *reinterpret_cast<const int4 *>(count + base);
+    int4 c = {0, 0, 0, 0};
+    if (base < n) c.x = count[base];
+    if (base + 1 < n) c.y = count[base + 1];
+    if (base + 2 < n) c.z = count[base + 2];
+    return c;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(const int *__restrict__ count, int n,
+                                                     int *__restrict__ start)
+{
+    __shared__ int lds[32];
+    const long tile0 = (long)blockIdx.x * kScanTile;
+    const int4 c = tile_counts(count, tile0 + 4 * threadIdx.x, n);
+    int total;
+    wg_exclusive_scan(c.x + c.y + c.z + c.w, lds, total);
+    if (threadIdx.x == 0) start[tile0] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tile_totals(int *__restrict__ start, int n,
+                                                           int ntiles)
+{
+    __shared__ int lds[32];
+    int carry = 0;
+    for (int c = 0; c < ntiles; c += 1024) {
+        const int b = c + (int)threadIdx.x;
+        const long at = (long)b * kScanTile;
+        const int v = (b < ntiles) ? start[at] : 0;
+        int total;
+        const int ex = wg_exclusive_scan(v, lds, total);
+        if (b < ntiles) start[at] = carry + ex;
+        carry += total;
     }
-    if (t == 1023) start[n] = part[1023];
+    if (threadIdx.x == 0) start[n] = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(const int *__restrict__ count, int n,
+                                                     int *__restrict__ start)
+{
+    __shared__ int lds[32];
+    const long tile0 = (long)blockIdx.x * kScanTile;
+    const int off = start[tile0];          // read by every thread before the scan's barriers
+    const long base = tile0 + 4 * threadIdx.x;
+    const int4 c = tile_counts(count, base, n);
+    int total;
+    int run = off + wg_exclusive_scan(c.x + c.y + c.z + c.w, lds, total);
+    const int v[4] = {c.x, c.y, c.z, c.w};
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) {
+            start[base + k] = run;
+            run += v[k];
+        }
 }
 
 // (face, bin) pairs: each face writes its index into the lists of its bins
@@ -247,7 +313,12 @@ __global__ __launch_bounds__(256) void k_locate(const double *__restrict__ uv,
 
 void launch_exclusive_scan(hipStream_t st, const int *count, int n, int *start)
 {
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, count, n, start);
+    const int ntiles = (int)(((long)n + kScanTile - 1) / kScanTile);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(1024), 0, st, count, n, start);
+    hipLaunchKernelGGL(k_scan_tile_totals, dim3(1), dim3(1024), 0, st, start, n, ntiles);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(1024), 0, st, count, n, start);
 }
 
 }  // namespace cvt
@@ -332,7 +403,7 @@ extern "C" int cvt_locate_points(const double *uv, const double *p3, int nv, con
         hipLaunchKernelGGL(k_bin_count, dim3((nf + 255) / 256), dim3(256), 0, st, d_box, nf, g.G,
                            d_count);
     }
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, d_count, nbins, d_start);
+    launch_exclusive_scan(st, d_count, nbins, d_start);
     LCHK(hipMemcpyAsync(&total, d_start + nbins, sizeof(int), hipMemcpyDeviceToHost, st), "total");
     LCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st), "flag");
     always.resize(std::max(nf, 1));

@@ -6,6 +6,6 @@ seeded workload generators used by the tests and the benchmark.
 """
 from ._lib import LIB_PATH, LABEL_NONE, CvtError, lib
 from .engine import Batch, Engine
-from . import compat, recover, workloads
+from . import compat, recover, weighting, workloads
 
-__all__ = ["Batch", "Engine", "CvtError", "LABEL_NONE", "LIB_PATH", "lib", "compat", "recover", "workloads"]
+__all__ = ["Batch", "Engine", "CvtError", "LABEL_NONE", "LIB_PATH", "lib", "compat", "recover", "weighting", "workloads"]

@@ -61,6 +61,7 @@ SIGNATURES = {
     "cvt_rasterize": (_i, [_vp, _vp, _i, _vp, _i, _vp, _d, _i]),
     "cvt_extract_points": (_i, [_vp, _vp, _i, _d, _d, _d, _vp, C.c_long, C.POINTER(C.c_long)]),
     "cvt_locate_points": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _vp]),
+    "cvt_vertex_weights": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     "cvt_batch_create": (_i, [C.POINTER(_vp), _i, _vp, _vp, _vp]),
     "cvt_batch_destroy": (_i, [_vp]),
     "cvt_batch_set_stream": (_i, [_vp, _vp]),

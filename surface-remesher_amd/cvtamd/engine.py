@@ -21,6 +21,10 @@ def _ptr(a, dtype=None):
     if hasattr(a, "data_ptr"):
         if not a.is_contiguous():
             raise ValueError("tensor must be contiguous")
+        if dtype is not None:
+            got, want = str(a.dtype).replace("torch.", ""), np.dtype(dtype).name
+            if got != want and not (got == "bool" and want == "uint8"):
+                raise TypeError(f"tensor dtype {a.dtype}, expected {want}")
         return C.c_void_p(a.data_ptr()), a
     arr = np.ascontiguousarray(a, dtype=dtype)
     return arr.ctypes.data_as(C.c_void_p), arr

@@ -457,3 +457,70 @@ def test_locate_lifts_linear_fields(orc):
     assert np.all(face >= 0)
     assert np.abs(out3[:, :2] - q).max() < 1e-12
     assert np.abs(out3[:, 2] - (0.7 - 0.2 * q[:, 0] + 0.35 * q[:, 1])).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# vertex density weights (source/weighting.h:37-97)
+
+def test_weights_flat_and_scaled_copies_exact(orc):
+    # integer grid: every face has 2D area 1/2 and every step is exact, so a
+    # flat copy weighs exactly 1 and a copy scaled by a exactly a^2
+    uv, tri = jittered_mesh(7, 1, jitter=0.0)
+    for a in (1.0, 2.0, 3.0):
+        p3 = np.stack([a * uv[:, 0], a * uv[:, 1], np.zeros(len(uv))], 1)
+        assert np.array_equal(orc.vertex_weights(uv, p3, tri), np.full(len(uv), a * a))
+    # a jittered flat copy: 1 up to rounding; the z axis and the other planes
+    uv, tri = jittered_mesh(12, 2)
+    for p3 in (np.stack([uv[:, 0], uv[:, 1], np.zeros(len(uv))], 1),
+               np.stack([np.zeros(len(uv)), uv[:, 0], uv[:, 1]], 1),
+               np.stack([uv[:, 1], np.full(len(uv), 4.0), uv[:, 0]], 1)):
+        assert np.abs(orc.vertex_weights(uv, p3, tri) - 1.0).max() < 1e-13
+
+
+def test_weights_tilted_plane(orc):
+    # over the plane z = a u + b v every 3D area is sqrt(1 + a^2 + b^2) times
+    # the 2D one: all three cross-product components are non-zero there
+    uv, tri = jittered_mesh(15, 3)
+    for a, b in ((1.0, 0.0), (0.0, -2.0), (0.6, 1.7), (-3.0, 0.25)):
+        p3 = np.stack([uv[:, 0], uv[:, 1], a * uv[:, 0] + b * uv[:, 1] + 0.5], 1)
+        w = orc.vertex_weights(uv, p3, tri)
+        assert np.abs(w / np.sqrt(1 + a * a + b * b) - 1.0).max() < 1e-13
+    # the vertex order inside a face changes the area only by rounding
+    p3 = np.stack([uv[:, 0], np.sin(uv[:, 1]), np.cos(uv[:, 0] * uv[:, 1] / 7)], 1)
+    w0 = orc.vertex_weights(uv, p3, tri)
+    w1 = orc.vertex_weights(uv, p3, np.roll(tri, 1, axis=1))
+    assert np.abs(w1 / w0 - 1.0).max() < 1e-13
+
+
+def test_weights_hand_summed_fan(orc):
+    # two faces around vertex 0; vertex 3 lifted to z = 1: face 1's 3D area is
+    # |(0,1,0) x (-1,0,1)| / 2 = sqrt(2) / 2, every 2D area is 1/2
+    uv = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [-1.0, 0.0]])
+    p3 = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [-1.0, 0.0, 1.0]])
+    tri = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    h = np.sqrt(2.0) / 2
+    want = np.array([0.5 + h, 1.0, 0.5 + h, 2 * h])
+    assert np.abs(orc.vertex_weights(uv, p3, tri) - want).max() < 1e-15
+
+
+def test_weights_large_rule_and_isolated_vertex(orc):
+    uv, tri = jittered_mesh(5, 4, jitter=0.0)
+    nv = len(uv)
+    p3 = np.stack([uv[:, 0], uv[:, 1], 0.5 * uv[:, 0]], 1)
+    base = orc.vertex_weights(uv, p3, tri)
+    # a face collinear in 2D (area 0, weight +inf), a sliver whose weight is
+    # finite but above 1e10, and a vertex in no face (0 / 0 = NaN)
+    extra_uv = np.array([[10.0, 10.0], [11.0, 11.0], [12.0, 12.0],
+                         [20.0, 20.0], [20.0 + 1e-6, 20.0], [20.0, 20.0 + 1e-6], [30.0, 30.0]])
+    extra_p3 = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0],
+                         [0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [9.0, 9.0, 9.0]])
+    uv2 = np.concatenate([uv, extra_uv])
+    p32 = np.concatenate([p3, extra_p3])
+    tri2 = np.concatenate([tri, np.array([[nv, nv + 1, nv + 2], [nv + 3, nv + 4, nv + 5]], np.int32)])
+    w = orc.vertex_weights(uv2, p32, tri2)
+    assert np.array_equal(w[:nv], base)
+    assert np.all(w[nv:nv + 6] == base.max()) and base.max() > 1.0
+    assert np.isnan(w[nv + 6])
+    # no weight at or below 1e10: the replacement is the initial maximum, 0
+    w = orc.vertex_weights(extra_uv[:3], extra_p3[:3], np.array([[0, 1, 2]], np.int32))
+    assert np.array_equal(w, np.zeros(3))
