@@ -7,6 +7,10 @@
  * synchronise the engine's stream.  Every function returns CVT_OK (0) or a
  * negative status; cvt_last_error() gives the message of the calling
  * thread's last failure.  A context is not re-entrant: one thread at a time.
+ * The one-shot calls without a context (cvt_rasterize, cvt_extract_points,
+ * cvt_locate_points, cvt_vertex_weights) run on the legacy default stream,
+ * as the reference's calls do: they wait for work queued there and on
+ * blocking streams, and return when their results are written.
  *
  * Conventions: a grid of width x height pixels, row-major (pixel (x, y) at
  * y*width + x, the reference's TOID, source/gcvt.h:26).  Sites are fp32

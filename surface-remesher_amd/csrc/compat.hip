@@ -366,11 +366,10 @@ extern "C" int cvt_extract_points(const int16_t *voronoi, const uint8_t *mask, i
     int *dcnt = nullptr;
     long *doff = nullptr;
     double *dout = nullptr;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;   // the legacy default stream, as the reference's calls use
     std::vector<int> cnt(n);
     std::vector<long> off(n);
     long total = 0;
-    CCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
     CCHK(hipMalloc(&dv, sizeof(short2) * np), "alloc");
     CCHK(hipMalloc(&dcnt, sizeof(int) * n), "alloc");
     CCHK(hipMalloc(&doff, sizeof(long) * n), "alloc");
@@ -400,7 +399,6 @@ done:
     void *bufs[] = {dv, dm, dcnt, doff, dout};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
-    if (st) (void)hipStreamDestroy(st);
     return rc;
 }
 

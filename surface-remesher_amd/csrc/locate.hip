@@ -28,6 +28,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cvt.h"
@@ -344,38 +345,43 @@ extern "C" int cvt_locate_points(const double *uv, const double *p3, int nv, con
     // about two faces per bin, at most 1024 x 1024 bins
     const int G = std::max(1, std::min(1024, (int)ceil(sqrt((double)std::max(nf, 1) / 2.0))));
     int rc = CVT_OK;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;   // the legacy default stream, as the reference's calls use
     double *d_uv = nullptr, *d_p3 = nullptr, *d_q = nullptr, *d_w = nullptr, *d_out3 = nullptr,
            *d_bb = nullptr;
     int *d_tri = nullptr, *d_list = nullptr, *d_count = nullptr, *d_start = nullptr,
         *d_cursor = nullptr, *d_alw = nullptr, *d_bad = nullptr, *d_face = nullptr;
     FaceBox *d_box = nullptr;
     uint8_t *d_always = nullptr;
+    char *arena = nullptr;
     int bad = 0, nalw = 0, total = 0, nbins = 0;
     double bb[4] = {0, 0, 0, 0};
     BinGrid g;
     std::vector<uint8_t> always;
     std::vector<int> alw;
-    LCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
-    LCHK(hipMalloc(&d_uv, sizeof(double) * 2 * std::max(nv, 1)), "alloc");
-    LCHK(hipMalloc(&d_tri, sizeof(int) * 3 * std::max(nf, 1)), "alloc");
-    LCHK(hipMalloc(&d_q, sizeof(double) * 2 * nq), "alloc");
-    LCHK(hipMalloc(&d_face, sizeof(int) * nq), "alloc");
-    LCHK(hipMalloc(&d_box, sizeof(FaceBox) * std::max(nf, 1)), "alloc");
-    LCHK(hipMalloc(&d_always, std::max(nf, 1)), "alloc");
-    LCHK(hipMalloc(&d_alw, sizeof(int) * std::max(nf, 1)), "alloc");
-    LCHK(hipMalloc(&d_count, sizeof(int) * (size_t)G * G), "alloc");
-    LCHK(hipMalloc(&d_cursor, sizeof(int) * (size_t)G * G), "alloc");
-    LCHK(hipMalloc(&d_start, sizeof(int) * ((size_t)G * G + 1)), "alloc");
-    LCHK(hipMalloc(&d_bad, sizeof(int)), "alloc");
-    LCHK(hipMalloc(&d_bb, sizeof(double) * 4), "alloc");
-    if (w) LCHK(hipMalloc(&d_w, sizeof(double) * 3 * nq), "alloc");
-    if (out3) {
-        LCHK(hipMalloc(&d_out3, sizeof(double) * 3 * nq), "alloc");
-        LCHK(hipMalloc(&d_p3, sizeof(double) * 3 * std::max(nv, 1)), "alloc");
-        if (nv > 0)
-            LCHK(hipMemcpyAsync(d_p3, p3, sizeof(double) * 3 * nv, hipMemcpyDefault, st), "upload");
-    }
+    const size_t nva = (size_t)std::max(nv, 1), nfa = (size_t)std::max(nf, 1);
+    const size_t nb = (size_t)G * G;
+    // the call's fixed-size buffers in one allocation, each at a 256-byte
+    // boundary (the bin lists, sized by the count, get their own)
+    size_t at = 0;
+    auto carve = [&](auto *&p, size_t n) {
+        at = (at + 255) & ~(size_t)255;
+        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(arena + at);
+        at += n * sizeof(*p);
+    };
+    auto layout = [&]() {
+        at = 0;
+        carve(d_uv, 2 * nva); carve(d_tri, 3 * nfa); carve(d_q, 2 * (size_t)nq);
+        carve(d_face, (size_t)nq); carve(d_box, nfa); carve(d_always, nfa); carve(d_alw, nfa);
+        carve(d_count, nb); carve(d_cursor, nb); carve(d_start, nb + 1); carve(d_bad, 1);
+        carve(d_bb, 4);
+        if (w) carve(d_w, 3 * (size_t)nq);
+        if (out3) { carve(d_out3, 3 * (size_t)nq); carve(d_p3, 3 * nva); }
+    };
+    layout();
+    LCHK(hipMalloc(&arena, at), "alloc");
+    layout();
+    if (out3 && nv > 0)
+        LCHK(hipMemcpyAsync(d_p3, p3, sizeof(double) * 3 * nv, hipMemcpyDefault, st), "upload");
     if (nv > 0) LCHK(hipMemcpyAsync(d_uv, uv, sizeof(double) * 2 * nv, hipMemcpyDefault, st), "upload");
     if (nf > 0) LCHK(hipMemcpyAsync(d_tri, tri, sizeof(int) * 3 * nf, hipMemcpyDefault, st), "upload");
     LCHK(hipMemcpyAsync(d_q, q, sizeof(double) * 2 * nq, hipMemcpyDefault, st), "upload");
@@ -427,10 +433,7 @@ extern "C" int cvt_locate_points(const double *uv, const double *p3, int nv, con
         LCHK(hipMemcpyAsync(out3, d_out3, sizeof(double) * 3 * nq, hipMemcpyDefault, st), "download");
     LCHK(hipStreamSynchronize(st), "locate");
 done:
-    void *bufs[] = {d_uv, d_p3, d_q, d_w, d_out3, d_bb, d_tri, d_list, d_count, d_start,
-                    d_cursor, d_alw, d_bad, d_face, d_box, d_always};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
-    if (st) (void)hipStreamDestroy(st);
+    if (d_list) (void)hipFree(d_list);
+    if (arena) (void)hipFree(arena);
     return rc;
 }

@@ -191,7 +191,7 @@ extern "C" int cvt_rasterize(const double *points, const double *weight, int num
     if (!(scale > 0.0) || !isfinite(scale)) return fail(CVT_ERR_ARG, "scale %g", scale);
     const int tiles_x = (n + RT - 1) / RT, ntiles = tiles_x * tiles_x;
     int rc = CVT_OK;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;   // the legacy default stream, as the reference's calls use
     double *d_pts = nullptr, *d_wt = nullptr;
     int *d_tri = nullptr, *d_list = nullptr, *d_count = nullptr, *d_start = nullptr,
         *d_cursor = nullptr, *d_alw = nullptr, *d_bad = nullptr;
@@ -202,7 +202,6 @@ extern "C" int cvt_rasterize(const double *points, const double *weight, int num
     std::vector<uint8_t> always;
     std::vector<int> alw;
     const long npx = (long)n * n;
-    RCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
     RCHK(hipMalloc(&d_pts, sizeof(double) * 2 * std::max(num_point, 1)), "alloc");
     RCHK(hipMalloc(&d_wt, sizeof(double) * std::max(num_point, 1)), "alloc");
     RCHK(hipMalloc(&d_tri, sizeof(int) * 3 * std::max(num_tri, 1)), "alloc");
@@ -255,6 +254,5 @@ done:
                     d_cursor, d_bad, d_den};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    if (st) (void)hipStreamDestroy(st);
     return rc;
 }

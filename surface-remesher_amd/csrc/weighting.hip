@@ -33,6 +33,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "../../include/cvt.h"
 #include "cvt_engine.h"
@@ -78,18 +79,20 @@ __global__ __launch_bounds__(256) void k_face_areas(const double *__restrict__ u
     atomicAdd(&count[t2], 1);
 }
 
-// each corner writes its key 3 f + c into its vertex's list
-__global__ __launch_bounds__(256) void k_corner_fill(const int *__restrict__ tri, int nf,
+// each corner writes its key 3 f + c into its vertex's list (a face with an
+// index outside [0, nv) was not counted and is skipped; the call fails)
+__global__ __launch_bounds__(256) void k_corner_fill(const int *__restrict__ tri, int nv, int nf,
                                                      const int *__restrict__ start,
                                                      int *__restrict__ cursor,
                                                      int *__restrict__ list)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf) return;
-    for (int c = 0; c < 3; ++c) {
-        const int v = tri[3 * i + c];
-        list[start[v] + atomicAdd(&cursor[v], 1)] = 3 * i + c;
-    }
+    const int t[3] = {tri[3 * i], tri[3 * i + 1], tri[3 * i + 2]};
+    if ((unsigned)t[0] >= (unsigned)nv || (unsigned)t[1] >= (unsigned)nv ||
+        (unsigned)t[2] >= (unsigned)nv)
+        return;
+    for (int c = 0; c < 3; ++c) list[start[t[c]] + atomicAdd(&cursor[t[c]], 1)] = 3 * i + c;
 }
 
 // ascending heapsort of a[0 .. n), in place (a thread's own list)
@@ -171,25 +174,30 @@ extern "C" int cvt_vertex_weights(const double *uv, const double *p3, int nv, co
     if (nv == 0) return CVT_OK;
     if ((long)nf * 3 > INT32_MAX) return fail(CVT_ERR_ARG, "too many faces");
     int rc = CVT_OK, bad = 0;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;   // the legacy default stream, as the reference's calls use
     double *d_uv = nullptr, *d_p3 = nullptr, *d_a3 = nullptr, *d_a2 = nullptr, *d_w = nullptr;
     int *d_tri = nullptr, *d_count = nullptr, *d_start = nullptr, *d_cursor = nullptr,
         *d_list = nullptr, *d_bad = nullptr;
     unsigned long long *d_mx = nullptr;
-    const int nfa = std::max(nf, 1);
-    WCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
-    WCHK(hipMalloc(&d_uv, sizeof(double) * 2 * nv), "alloc");
-    WCHK(hipMalloc(&d_p3, sizeof(double) * 3 * nv), "alloc");
-    WCHK(hipMalloc(&d_w, sizeof(double) * nv), "alloc");
-    WCHK(hipMalloc(&d_tri, sizeof(int) * 3 * nfa), "alloc");
-    WCHK(hipMalloc(&d_a3, sizeof(double) * nfa), "alloc");
-    WCHK(hipMalloc(&d_a2, sizeof(double) * nfa), "alloc");
-    WCHK(hipMalloc(&d_list, sizeof(int) * 3 * nfa), "alloc");
-    WCHK(hipMalloc(&d_count, sizeof(int) * nv), "alloc");
-    WCHK(hipMalloc(&d_cursor, sizeof(int) * nv), "alloc");
-    WCHK(hipMalloc(&d_start, sizeof(int) * ((size_t)nv + 1)), "alloc");
-    WCHK(hipMalloc(&d_bad, sizeof(int)), "alloc");
-    WCHK(hipMalloc(&d_mx, sizeof(unsigned long long)), "alloc");
+    char *arena = nullptr;
+    const size_t nfa = (size_t)std::max(nf, 1), nvs = (size_t)nv;
+    // every buffer of the call in one allocation, each at a 256-byte boundary
+    size_t at = 0;
+    auto carve = [&](auto *&p, size_t n) {
+        at = (at + 255) & ~(size_t)255;
+        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(arena + at);
+        at += n * sizeof(*p);
+    };
+    auto layout = [&]() {
+        at = 0;
+        carve(d_uv, 2 * nvs); carve(d_p3, 3 * nvs); carve(d_w, nvs);
+        carve(d_tri, 3 * nfa); carve(d_a3, nfa); carve(d_a2, nfa); carve(d_list, 3 * nfa);
+        carve(d_count, nvs); carve(d_cursor, nvs); carve(d_start, nvs + 1);
+        carve(d_bad, 1); carve(d_mx, 1);
+    };
+    layout();
+    WCHK(hipMalloc(&arena, at), "alloc");
+    layout();
     WCHK(hipMemcpyAsync(d_uv, uv, sizeof(double) * 2 * nv, hipMemcpyDefault, st), "upload");
     WCHK(hipMemcpyAsync(d_p3, p3, sizeof(double) * 3 * nv, hipMemcpyDefault, st), "upload");
     if (nf > 0) WCHK(hipMemcpyAsync(d_tri, tri, sizeof(int) * 3 * nf, hipMemcpyDefault, st), "upload");
@@ -200,25 +208,20 @@ extern "C" int cvt_vertex_weights(const double *uv, const double *p3, int nv, co
     if (nf > 0)
         hipLaunchKernelGGL(k_face_areas, dim3((nf + 255) / 256), dim3(256), 0, st, d_uv, d_p3, nv,
                            d_tri, nf, d_a3, d_a2, d_count, d_bad);
-    WCHK(hipGetLastError(), "areas launch");
-    WCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st), "flag");
-    WCHK(hipStreamSynchronize(st), "areas");
-    if (bad) { rc = fail(CVT_ERR_ARG, "a face indexes a vertex outside [0, %d)", nv); goto done; }
     launch_exclusive_scan(st, d_count, nv, d_start);
     if (nf > 0)
-        hipLaunchKernelGGL(k_corner_fill, dim3((nf + 255) / 256), dim3(256), 0, st, d_tri, nf,
-                           d_start, d_cursor, d_list);
+        hipLaunchKernelGGL(k_corner_fill, dim3((nf + 255) / 256), dim3(256), 0, st, d_tri, nv,
+                           nf, d_start, d_cursor, d_list);
     hipLaunchKernelGGL(k_vertex_weight, dim3((nv + 255) / 256), dim3(256), 0, st, d_start, d_list,
                        d_a3, d_a2, nv, d_w, d_mx);
     hipLaunchKernelGGL(k_weight_cap, dim3((nv + 255) / 256), dim3(256), 0, st, d_w, nv, d_mx);
     WCHK(hipGetLastError(), "weights launch");
+    WCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st), "flag");
+    WCHK(hipStreamSynchronize(st), "weights");
+    if (bad) { rc = fail(CVT_ERR_ARG, "a face indexes a vertex outside [0, %d)", nv); goto done; }
     WCHK(hipMemcpyAsync(weight, d_w, sizeof(double) * nv, hipMemcpyDefault, st), "download");
     WCHK(hipStreamSynchronize(st), "weights");
 done:
-    void *bufs[] = {d_uv, d_p3, d_a3, d_a2, d_w, d_tri, d_count, d_start, d_cursor, d_list,
-                    d_bad, d_mx};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
-    if (st) (void)hipStreamDestroy(st);
+    if (arena) (void)hipFree(arena);
     return rc;
 }
