@@ -281,7 +281,7 @@ def weights_sample(with_cpu: bool) -> dict:
     patch (2.09M faces), inputs resident in device memory, the call
     returning when the weights are written; the baseline is the oracle's
     face loop, single-threaded as the reference's host loop is (its sums
-    are sequential in face order)."""
+    are sequential in face order), median of as many runs as the GPU's."""
     import numpy as np
     import torch
     from cvtamd import weighting, workloads as wl
@@ -298,9 +298,12 @@ def weights_sample(with_cpu: bool) -> dict:
              "gpu_faces_per_s": round(len(tri) / gpu_ms * 1e3, 1)}
         if with_cpu:
             from oracle import oracle as orc
-            t0 = time.perf_counter()
-            ref = orc.vertex_weights(uv, p3, tri)
-            cpu_ms = (time.perf_counter() - t0) * 1e3
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                ref = orc.vertex_weights(uv, p3, tri)
+                ts.append(time.perf_counter() - t0)
+            cpu_ms = float(np.median(ts)) * 1e3
             r.update({"cpu_ms": round(cpu_ms, 3), "cpu_threads": 1,
                       "cpu_kind": "port (face loop, as source/weighting.h:53-92)",
                       "bit_exact": bool(np.array_equal(w, ref, equal_nan=True)),

@@ -65,12 +65,14 @@ static __global__ __launch_bounds__(256) void k_update_batch(
     double e = 0.0, g = 0.0;
     if (i < P.S) {
         const int gi = P.site_off + i;
+        // the site and its fixed flag are loaded with the moment slots
+        float2 s = sites[gi];
+        const int pinned = fixed[gi];
         double w, dx, dy, ee;
         cvt_site_totals(gi, dstate[ST_EPOCH], slots, nrows, ovf, ovf_head, w, dx, dy, ee);
         e = ee;
         g = dx * dx + dy * dy;
-        float2 s = sites[gi];
-        if ((mode & UPD_MOVE) && w > 0.0 && !fixed[gi]) {
+        if ((mode & UPD_MOVE) && w > 0.0 && pinned == 0) {
             s.x = (float)((double)s.x + (double)omega * (dx / w));
             s.y = (float)((double)s.y + (double)omega * (dy / w));
             sites[gi] = s;

@@ -458,8 +458,10 @@ __device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__
                                                 double &dy, double &ee)
 {
     __int128 sw = 0, sx = 0, sy = 0, se = 0;
-    // all nine slots are loaded before any is tested, so the loads overlap
-    // (slot k of consecutive sites is contiguous: a wave's loads coalesce)
+    // the chain head and all nine slots are loaded before any is used, so
+    // the loads overlap (slot k of consecutive sites is contiguous: a wave's
+    // loads coalesce)
+    const int head = ovf_head[i];
     Slot v[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) v[k] = slots[(long)k * nrows + i];
@@ -471,7 +473,7 @@ __device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__
         sy += on ? v[k].dy : 0;
         se += on ? v[k].e : 0;
     }
-    for (int k = ovf_head[i]; k >= 0; k = ovf[k].next) {
+    for (int k = head; k >= 0; k = ovf[k].next) {
         sw += ovf[k].w; sx += ovf[k].dx; sy += ovf[k].dy; se += ovf[k].e;
     }
     ovf_head[i] = -1;
@@ -512,6 +514,10 @@ static __global__ __launch_bounds__(256) void k_update(
     const float omega = domega ? *domega : omega_arg;
     double e = 0.0, g = 0.0;
     if (i < live) {
+        // the site and its fixed flag (the context's flags, never null) are
+        // loaded with the moment slots; nothing waits for them until used
+        float2 s = sites[i];
+        const int pinned = fixed[i];
         double w, dx, dy, ee;
         cvt_site_totals(i, dstate[ST_EPOCH], slots, nrows, ovf, ovf_head, w, dx, dy, ee);
         e = ee;
@@ -522,8 +528,7 @@ static __global__ __launch_bounds__(256) void k_update(
             mom_out[4L * i + 2] = dy * rho_unscale;
             mom_out[4L * i + 3] = ee * rho_unscale;
         }
-        float2 s = sites[i];
-        if ((mode & UPD_MOVE) && w > 0.0 && !(fixed && fixed[i])) {
+        if ((mode & UPD_MOVE) && w > 0.0 && pinned == 0) {
             s.x = (float)((double)s.x + (double)omega * (dx / w));
             s.y = (float)((double)s.y + (double)omega * (dy / w));
             sites[i] = s;
