@@ -6,15 +6,18 @@
 //   seed     uint32 [H][W]    seed map: label of the site on that pixel, `none`
 //                             elsewhere; read by JFA pass 0, cleared by pass 1,
 //                             re-seeded by the site update
+//   first    uint32 [H][W]    output of JFA pass 0 (the seeds scattered to
+//                             their receivers); `none` elsewhere between JFAs
 //   lab[2]   uint32 [H][W]    JFA ping-pong label maps
 //   sites    float2 [capS+1]  site table, pixel units; sites[capS] = (inf, inf)
 //                             is the sentinel
 // A label is the byte offset of its site in the table (8 * index) and `none`
 // = 8 * capS, so the JFA gather needs no address arithmetic; labels leave the
 // engine as int32 indices (CVT_NONE for none).
-//   slots    Slot [S][9]      per-site partial moments, one per tile of the 3x3
+//   slots    Slot [9][capS]   per-site partial moments, one per tile of the 3x3
 //                             tile window around the site's home tile (unique
-//                             writer per slot, tagged with the epoch)
+//                             writer per slot, tagged with the epoch); slot-major,
+//                             so slot k of consecutive sites is contiguous
 //   ovf      OvfRec [cap]     partial moments of tiles outside that window,
 //                             chained per site from ovf_head[S]
 //
@@ -29,6 +32,11 @@
 #include <stdint.h>
 
 #include "cvt_types.h"
+
+// Each translation unit that includes this header launches only some of its
+// kernels.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wunused-function"
 
 // ---------------------------------------------------------------------------
 // distance order (must round exactly like the oracle's orc_dist / orc_better)
@@ -769,3 +777,5 @@ static __global__ __launch_bounds__(256) void k_density_stats(const float *__res
     }
     if (threadIdx.x == 0) block_max[blockIdx.x] = red[0];
 }
+
+#pragma clang diagnostic pop
