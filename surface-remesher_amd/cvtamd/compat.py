@@ -28,16 +28,25 @@ def gcvt(voronoi: np.ndarray, density, mask=None, depth: int = 1, max_iter: int 
 
 
 def rasterize(points, weight, triangles, scale: float, n: int, out=None):
-    """Density grid (n, n) float32 of a 2-D triangle mesh (discretization_d rule)."""
-    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 2)
-    wt = np.ascontiguousarray(weight, np.float64).reshape(-1)
-    tri = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    """Density grid (n, n) float32 of a 2-D triangle mesh (discretization_d rule).
+
+    points (nv, 2) float64, weight (nv,) float64, triangles (nf, 3) int32:
+    numpy arrays or torch tensors in host or device memory.  out: an
+    (n, n) float32 array or tensor to write (host or device), or None for a
+    new numpy array."""
+    pp, kp = _ptr(points, np.float64)
+    pw, kw = _ptr(weight, np.float64)
+    pt, kt = _ptr(triangles, np.int32)
+    nv = int(np.prod(kp.shape)) // 2
+    nf = int(np.prod(kt.shape)) // 3
+    if int(np.prod(kw.shape)) != nv:
+        raise ValueError("weight must hold one value per point")
     if out is None:
         out = np.empty((n, n), np.float32)
-    po, k = _ptr(out)
-    call("cvt_rasterize", pts.ctypes.data_as(C.c_void_p), wt.ctypes.data_as(C.c_void_p),
-         int(pts.shape[0]), tri.ctypes.data_as(C.c_void_p), int(tri.shape[0]), po,
-         float(scale), int(n))
+    if int(np.prod(out.shape)) != n * n:
+        raise ValueError("out must hold n * n values")
+    po, k = _ptr(out, np.float32)
+    call("cvt_rasterize", pp, pw, nv, pt, nf, po, float(scale), int(n))
     return out
 
 

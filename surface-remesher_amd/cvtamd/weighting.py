@@ -15,14 +15,17 @@ from ._lib import call
 from .engine import _ptr
 
 
-def vertex_weights(uv, p3, tri):
+def vertex_weights(uv, p3, tri, out=None):
     """Density weight of every vertex.
 
     uv: (nv, 2) float64 2D vertices; p3: (nv, 3) float64 3D position of each
     2D vertex; tri: (nf, 3) int32 faces in vertices_around_face order.
     numpy arrays or torch tensors (host or device memory) are accepted.
     Returns (nv,) float64: area3 / area2 summed over each vertex's faces,
-    with the > 1e10 rule; NaN for a vertex in no face."""
+    with the > 1e10 rule; NaN for a vertex in no face.  out: an (nv,)
+    float64 array or tensor to write (host or device; a device tensor keeps
+    the weights on the GPU for the rasterizer), or None for a new numpy
+    array."""
     uvp, k0 = _ptr(uv, np.float64)
     p3p, k1 = _ptr(p3, np.float64)
     trp, k2 = _ptr(tri, np.int32)
@@ -30,6 +33,10 @@ def vertex_weights(uv, p3, tri):
     if int(np.prod(k1.shape)) != 3 * nv:
         raise ValueError("p3 must hold one 3D point per uv vertex")
     nf = int(np.prod(k2.shape)) // 3
-    out = np.empty(nv, np.float64)
-    call("cvt_vertex_weights", uvp, p3p, nv, trp, nf, _ptr(out)[0])
+    if out is None:
+        out = np.empty(nv, np.float64)
+    if int(np.prod(out.shape)) != nv:
+        raise ValueError("out must hold one value per vertex")
+    po, ko = _ptr(out, np.float64)
+    call("cvt_vertex_weights", uvp, p3p, nv, trp, nf, po)
     return out

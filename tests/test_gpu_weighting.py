@@ -107,6 +107,28 @@ def test_weights_device_pointers_and_errors(gpu):
     assert weighting.vertex_weights(uv[:0], p3[:0], tri[:0]).shape == (0,)
 
 
+def test_weights_and_raster_stay_on_the_device(gpu):
+    """Weights written into a device tensor feed the rasterizer without a
+    host copy; the density lands in a device tensor; both equal the host
+    path's results."""
+    import torch
+    from cvtamd import compat, weighting
+    uv, tri = jittered_mesh(18, 14)
+    p3 = lift(uv)
+    n = 200
+    scale = 17.0 / (n - 1)
+    d_uv, d_p3, d_tri = (torch.from_numpy(a).cuda() for a in (uv, p3, tri))
+    d_w = torch.empty(len(uv), dtype=torch.float64, device="cuda")
+    assert weighting.vertex_weights(d_uv, d_p3, d_tri, out=d_w) is d_w
+    d_den = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    compat.rasterize(d_uv, d_w, d_tri, scale, n, out=d_den)
+    w_host = weighting.vertex_weights(uv, p3, tri)
+    assert np.array_equal(d_w.cpu().numpy(), w_host)
+    assert np.array_equal(d_den.cpu().numpy(), compat.rasterize(uv, w_host, tri, scale, n))
+    with pytest.raises(ValueError):
+        compat.rasterize(uv, w_host[:-1], tri, scale, n)
+
+
 def test_weights_feed_the_rasterizer(gpu):
     """The two GPU stages chained: weights, then the density raster, equal
     the oracle's weights rasterized by the oracle."""
