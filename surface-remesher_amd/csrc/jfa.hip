@@ -155,18 +155,22 @@ typedef const __attribute__((address_space(3))) f32x2 lds_cf32x2;
 typedef __attribute__((address_space(3))) f32x4 lds_f32x4;
 
 enum { STEP_ANY = 0, STEP_1 = 1, STEP_2 = 2 };
-enum { TAB_LDS0 = 0, TAB_LDSB = 1, TAB_GLOBAL = 2 };
+enum { TAB_LDS0 = 0, TAB_LDSB = 1, TAB_GLOBAL = 2, TAB_SPLIT = 3 };
 
 // Site lookup by label (a byte offset into the site table).
 //   TAB_LDS0:   the table is staged at LDS byte 0, so the label is the LDS
 //               address and a lookup is one ds_read_b64;
 //   TAB_LDSB:   one patch's table is staged in LDS; `base` is the label of
 //               its first entry;
-//   TAB_GLOBAL: a buffer load from the global table.
+//   TAB_GLOBAL: a buffer load from the global table;
+//   TAB_SPLIT:  the table's first entries (labels below `split`) are staged
+//               at LDS byte 0 and read from there; the lanes whose labels lie
+//               above take a buffer load from the global table.
 template <int TAB> struct SiteTab {
     lds_cchar *lds;
     __amdgpu_buffer_rsrc_t g;
     uint32_t base;
+    uint32_t split;       // TAB_SPLIT: the first label not staged in LDS
     const f32x4 *src;     // staging source (LDS kinds)
     int n4;               // float4 count to stage
 
@@ -178,6 +182,19 @@ template <int TAB> struct SiteTab {
         } else if constexpr (TAB == TAB_LDSB) {
             const f32x2 v = *(lds_cf32x2 *)(lds + (l - base));
             return make_float2(v.x, v.y);
+        } else if constexpr (TAB == TAB_SPLIT) {
+            // branch-free: every lane reads LDS (at entry 0 when its label is
+            // above the split) and issues the gather, whose offset is out of
+            // the buffer's range for the LDS lanes: such a lane reads 0 and
+            // fetches nothing
+            // (the LDS value is zeroed on the other lanes, and the two are
+            // combined by OR, as one of them is 0)
+            const bool in_lds = l < split;
+            const f32x2 a = *(lds_cf32x2 *)(lds + (in_lds ? l : 0u));
+            const u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(g, in_lds ? 0xfffffff0u : l, 0, 0);
+            const uint32_t ax = in_lds ? __float_as_uint(a.x) : 0u;
+            const uint32_t ay = in_lds ? __float_as_uint(a.y) : 0u;
+            return make_float2(__uint_as_float(ax | b.x), __uint_as_float(ay | b.y));
         } else {
             const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(g, l, 0, 0);
             return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
@@ -561,11 +578,12 @@ __device__ __forceinline__ void quad_runs(const MapGeo &m, const SiteTab<TAB> &t
 }
 
 // Single map, site table staged at LDS byte 0 in a static array of ENTRIES
-// float2 (the kernel's only LDS), or gathered from global memory when
-// ENTRIES == 0.  RUNS picks the work layout: runs down chains of rows, or
-// bands of rows (quad_rows).
-#define JFA_THREADS(E) ((E) > 5120 ? 1024 : 256)
-#define JFA_MINB(E) ((E) > 5120 ? 1 : 4)
+// float2 (the kernel's only LDS), gathered from global memory when
+// ENTRIES == 0, or split when ENTRIES < 0: the first -ENTRIES sites staged
+// in LDS, the rest gathered.  RUNS picks the work layout: runs down chains
+// of rows, or bands of rows (quad_rows).
+#define JFA_THREADS(E) ((E) > 5120 || (E) < 0 ? 1024 : 256)
+#define JFA_MINB(E) ((E) > 5120 || (E) < 0 ? 1 : 4)
 
 template <int MODE, int TAB>
 __device__ __forceinline__ void quad_layout(const MapGeo &m, const SiteTab<TAB> &tab, bool runs)
@@ -623,6 +641,16 @@ __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa
         tab.n4 = table_f4;
         tab.base = 0;
         quad_layout<MODE, TAB_LDS0>(m, tab, RUNS);
+    } else if constexpr (ENTRIES < 0) {
+        __shared__ float4 lds_tab[-ENTRIES / 2];
+        SiteTab<TAB_SPLIT> tab;
+        tab.lds = (lds_cchar *)lds_tab;
+        tab.src = reinterpret_cast<const f32x4 *>(sites);
+        tab.n4 = -ENTRIES / 2;
+        tab.g = buf_rsrc(sites, 16u * (uint32_t)table_f4);
+        tab.split = 8u * (uint32_t)(-ENTRIES);
+        tab.base = 0;
+        quad_layout<MODE, TAB_SPLIT>(m, tab, RUNS);
     } else {
         SiteTab<TAB_GLOBAL> tab;
         tab.g = buf_rsrc(sites, 16u * (uint32_t)table_f4);
@@ -737,13 +765,16 @@ JfaPlan plan_jfa(int W, int H, int capS)
     p.table_f4 = (capS + 2) / 2;
     const int entries = 2 * p.table_f4;
     p.lds_entries = 0;
-    if (p.quad)
+    if (p.quad) {
         for (int e : kLdsEntries)
             if (entries <= e) { p.lds_entries = e; break; }
+        // too large for LDS: the first 20480 entries there, the rest gathered
+        if (p.lds_entries == 0) p.lds_entries = -kLdsEntries[2];
+    }
     p.lds = p.lds_entries > 0;
-    p.lds_bytes = p.lds ? (size_t)p.lds_entries * 8 : 0;
+    p.lds_bytes = p.lds_entries != 0 ? (size_t)std::abs(p.lds_entries) * 8 : 0;
     const long items = (long)H * ((W + 255) / 256);
-    if (p.lds_entries > 5120) {
+    if (p.lds_entries > 5120 || p.lds_entries < 0) {
         p.threads = 1024;                               // one workgroup per CU
         p.grid = g_num_cu;
     } else {
@@ -774,6 +805,10 @@ static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, ui
         break;
     case 20480:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 20480, RUNS>), g, b, 0, st, in, out, sites,
+                           p.table_f4, W, H, step, clear, none, nclear, rstep);
+        break;
+    case -20480:
+        hipLaunchKernelGGL((k_jfa_quad<MODE, -20480, RUNS>), g, b, 0, st, in, out, sites,
                            p.table_f4, W, H, step, clear, none, nclear, rstep);
         break;
     default:

@@ -50,7 +50,9 @@ def check_moments(m, ref, W_tol=1e-6):
     (100, 70, 1, 5),          # one site
     (64, 64, 64, 6),
     (98, 77, 150, 7),         # width not a multiple of 4: one pixel per lane
-    (1024, 1024, 30000, 8),   # site table too large for LDS: global gathers
+    (1024, 1024, 30000, 8),   # site table too large for LDS: split LDS / global
+    (700, 600, 20479, 14),    # the largest table that fits in LDS
+    (700, 600, 20481, 15),    # just too large: all but the last sites in LDS
     (12, 9, 5, 9),            # tiny grid
     (8, 8, 3, 11),            # narrowest grid of the 4-pixel kernel
     (260, 130, 5000, 12),     # 40 KB site table, width not a multiple of 256
@@ -69,7 +71,7 @@ def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
 @pytest.mark.parametrize("W,H,S,seed", [
     (512, 512, 1024, 31),     # 4-pixel kernel, table in LDS
     (98, 77, 150, 32),        # one pixel per lane
-    (1024, 1024, 30000, 33),  # global gathers
+    (1024, 1024, 30000, 33),  # split LDS / global table
 ])
 def test_jfa_after_lloyd_and_reseeding(gpu, orc, W, H, S, seed):
     """The seed map and the first-pass map are reset sparsely (around the seed
