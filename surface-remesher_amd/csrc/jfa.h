@@ -10,7 +10,8 @@ namespace cvt {
 // lane, persistent workgroups over runs or bands of rows) when W % 4 == 0,
 // with the site table in a static LDS array of lds_entries float2 (16, 40 or
 // 160 KB) when it fits, or, for larger tables, lds_entries = -20480: the
-// first 20480 entries in LDS and the rest gathered from global memory;
+// first 20478 entries and a zero entry in LDS, the rest gathered from global
+// memory;
 // otherwise one pixel per lane.
 struct JfaPlan {
     bool quad = false, lds = false;

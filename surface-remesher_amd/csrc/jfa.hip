@@ -164,13 +164,14 @@ enum { TAB_LDS0 = 0, TAB_LDSB = 1, TAB_GLOBAL = 2, TAB_SPLIT = 3 };
 //               its first entry;
 //   TAB_GLOBAL: a buffer load from the global table;
 //   TAB_SPLIT:  the table's first entries (labels below `split`) are staged
-//               at LDS byte 0 and read from there; the lanes whose labels lie
-//               above take a buffer load from the global table.
+//               at LDS byte 0, followed by a zero entry at byte `split`; the
+//               other labels are read from a buffer that starts at the
+//               first entry not staged (`g`, offset l - split).
 template <int TAB> struct SiteTab {
     lds_cchar *lds;
     __amdgpu_buffer_rsrc_t g;
     uint32_t base;
-    uint32_t split;       // TAB_SPLIT: the first label not staged in LDS
+    uint32_t split;       // TAB_SPLIT: the first label not staged in LDS (bytes)
     const f32x4 *src;     // staging source (LDS kinds)
     int n4;               // float4 count to stage
 
@@ -183,18 +184,14 @@ template <int TAB> struct SiteTab {
             const f32x2 v = *(lds_cf32x2 *)(lds + (l - base));
             return make_float2(v.x, v.y);
         } else if constexpr (TAB == TAB_SPLIT) {
-            // branch-free: every lane reads LDS (at entry 0 when its label is
-            // above the split) and issues the gather, whose offset is out of
-            // the buffer's range for the LDS lanes: such a lane reads 0 and
-            // fetches nothing
-            // (the LDS value is zeroed on the other lanes, and the two are
-            // combined by OR, as one of them is 0)
-            const bool in_lds = l < split;
-            const f32x2 a = *(lds_cf32x2 *)(lds + (in_lds ? l : 0u));
-            const u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(g, in_lds ? 0xfffffff0u : l, 0, 0);
-            const uint32_t ax = in_lds ? __float_as_uint(a.x) : 0u;
-            const uint32_t ay = in_lds ? __float_as_uint(a.y) : 0u;
-            return make_float2(__uint_as_float(ax | b.x), __uint_as_float(ay | b.y));
+            // branch-free: every lane reads LDS, clamped to the zero entry,
+            // and issues the gather; for a label below the split, l - split
+            // wraps past the buffer's range, so the gather reads 0 and
+            // fetches nothing.  One of the two is 0: OR combines them.
+            const f32x2 a = *(lds_cf32x2 *)(lds + min(l, split));
+            const u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(g, l - split, 0, 0);
+            return make_float2(__uint_as_float(__float_as_uint(a.x) | b.x),
+                               __uint_as_float(__float_as_uint(a.y) | b.y));
         } else {
             const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(g, l, 0, 0);
             return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
@@ -225,6 +222,8 @@ template <int TAB> struct SiteTab {
                         dst[i0 + k * stride] = f;
                     }
             }
+            // the split table's zero entry (and the unused one beside it)
+            if (TAB == TAB_SPLIT && threadIdx.x == 0) dst[n4] = f32x4{0.f, 0.f, 0.f, 0.f};
             __syncthreads();
         }
     }
@@ -643,12 +642,15 @@ __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa
         quad_layout<MODE, TAB_LDS0>(m, tab, RUNS);
     } else if constexpr (ENTRIES < 0) {
         __shared__ float4 lds_tab[-ENTRIES / 2];
+        // entries 0 .. -ENTRIES - 3 staged, then a zero pair; the rest of the
+        // table (from entry -ENTRIES - 2) is gathered
+        constexpr int kept = -ENTRIES - 2;
         SiteTab<TAB_SPLIT> tab;
         tab.lds = (lds_cchar *)lds_tab;
         tab.src = reinterpret_cast<const f32x4 *>(sites);
-        tab.n4 = -ENTRIES / 2;
-        tab.g = buf_rsrc(sites, 16u * (uint32_t)table_f4);
-        tab.split = 8u * (uint32_t)(-ENTRIES);
+        tab.n4 = kept / 2;
+        tab.g = buf_rsrc(sites + kept, 8u * (uint32_t)(2 * table_f4 - kept));
+        tab.split = 8u * (uint32_t)kept;
         tab.base = 0;
         quad_layout<MODE, TAB_SPLIT>(m, tab, RUNS);
     } else {
