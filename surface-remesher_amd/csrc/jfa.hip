@@ -828,9 +828,13 @@ static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, ui
 // Runs also need work for every wave: a run is four items, so on small grids
 // (512^2: 256 runs at step 1 for 1024 waves) they would idle most waves.
 static const int kRunsMaxStep = 32;
+// With the split site table (C3) the pass is bound by the remaining gathers,
+// and runs (fewer label loads) win at every step: 683 -> 701 iter/s at
+// 4096^2 / 64K sites.
 static bool use_runs(const JfaPlan &p, int W, int H, int step)
 {
-    if (step > kRunsMaxStep && 8L * step < (long)std::max(W, H)) return false;
+    if (p.lds_entries >= 0 && step > kRunsMaxStep && 8L * step < (long)std::max(W, H))
+        return false;
     const long n = (H + step - 1) / step;
     const long runs = (long)std::min(step, H) * ((n + 3) / 4) * ((W + 255) / 256);
     return runs >= (long)p.grid * (p.threads / 64);
@@ -846,8 +850,9 @@ void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint3
             launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         else if (step == 1)
             launch_quad<STEP_1, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
-        // (step 2 with the global table keeps bands: six buffers of it spill)
-        else if (step == 2 && runs && p.lds_entries > 0)
+        // (step 2 with the all-global table keeps bands: six buffers of it
+        // spill)
+        else if (step == 2 && runs && p.lds_entries != 0)
             launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
         else if (runs)
