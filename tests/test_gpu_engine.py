@@ -53,6 +53,7 @@ def check_moments(m, ref, W_tol=1e-6):
     (1024, 1024, 30000, 8),   # site table too large for LDS: split LDS / global
     (700, 600, 20479, 14),    # the largest table that fits in LDS
     (700, 600, 20481, 15),    # just too large: all but the last sites in LDS
+    (1000, 600, 25000, 16),   # split table, width not a multiple of 256
     (12, 9, 5, 9),            # tiny grid
     (8, 8, 3, 11),            # narrowest grid of the 4-pixel kernel
     (260, 130, 5000, 12),     # 40 KB site table, width not a multiple of 256
