@@ -384,9 +384,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this engine runs one grid per device and "
-              "has no multi-GPU path (see DESIGN.md for how it would shard); use --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: the engine runs one grid per device and has "
+              "no multi-GPU code path; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -12,7 +12,7 @@ seeds the training never uses):
 3. Learned: from the same initial sites, timed from the start: 10 plain
    Lloyd iterations, then from the initial sites again the warm start (input
    preparation, network, site move: cvt_unet_warm_start) and 10 iterations;
-   the lower-energy run continues (plain Lloyd is the judge and the
+   the lower-energy run continues (plain Lloyd is the arbiter and the
    fallback) until the target.  Both trials are inside the learned time.
 
 Both runs check the energy history every CHUNK iterations (one graph launch
@@ -30,7 +30,7 @@ from pathlib import Path
 
 import numpy as np
 
-from .warmstart import TRIAL, warm_start_judged
+from .warmstart import TRIAL, warm_start_checked
 
 CHUNK = 10
 TOL = 1e-4
@@ -82,12 +82,12 @@ def time_to_tol(e, net, rho, xy0, cap=CAP, trial=TRIAL):
     it_p, ok_p = run_to(e, target, cap, e.synchronize)
     t_plain = time.perf_counter() - t0
 
-    # learned: plain Lloyd judges the warm start over the first TRIAL
-    # iterations (cvtamd.warmstart.warm_start_judged); both trials are timed
+    # learned: plain Lloyd checks the warm start over the first TRIAL
+    # iterations (cvtamd.warmstart.warm_start_checked); both trials are timed
     e.clear_history()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    j = warm_start_judged(net, e, xy0, trial)
+    j = warm_start_checked(net, e, xy0, trial)
     hit = np.nonzero(j["trial_energies"] <= target)[0]
     if len(hit):
         it_l, ok_l = int(hit[0]) + 1, True
@@ -138,7 +138,7 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--out", type=Path, default=None)
     ap.add_argument("--trial", type=int, default=TRIAL,
-                    help="Lloyd iterations of each judged trial (plain and warm-started)")
+                    help="Lloyd iterations of each checked trial (plain and warm-started)")
     args = ap.parse_args(argv)
 
     import multiprocessing as mp

@@ -1,5 +1,5 @@
 """Host API of the learned warm start (cvt_unet_* of include/cvt.h) and the
-judged warm start: plain Lloyd decides whether the network's sites are used."""
+checked warm start: plain Lloyd decides whether the network's sites are used."""
 from __future__ import annotations
 
 import ctypes as C
@@ -117,8 +117,8 @@ class UNet:
 TRIAL = 10
 
 
-def warm_start_judged(net: UNet, engine, xy0, trial: int = TRIAL) -> dict:
-    """Warm start with plain Lloyd as the judge and the fallback.
+def warm_start_checked(net: UNet, engine, xy0, trial: int = TRIAL) -> dict:
+    """Warm start with plain Lloyd as the arbiter and the fallback.
 
     From the initial sites xy0 (device or host, pixel units), run `trial`
     plain Lloyd iterations and keep their sites; then, from xy0 again, move
