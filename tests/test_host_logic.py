@@ -1,5 +1,6 @@
 """CPU checks of the host-side logic around the kernels: the tile tuner's
-table keys and lookups, and the time-to-tolerance summary."""
+table keys and lookups, the time-to-tolerance summary and the pointer helper
+of the ctypes binding."""
 import json
 
 import numpy as np
@@ -60,3 +61,22 @@ def test_time_to_tol_summary():
     assert s["fallback_rate"] == pytest.approx(0.25)
     assert s["median_ratio"] == pytest.approx(50.0)
     assert s["p10_ratio"] == pytest.approx(np.percentile([50.0, 40.0, 60.0], 10))
+
+
+def test_pointer_helper_checks_tensor_dtypes():
+    """Arrays are converted to the C ABI's element type; a tensor of another
+    type is refused rather than reinterpreted (bool is accepted as uint8)."""
+    import torch
+    from cvtamd.engine import _ptr
+    p, keep = _ptr([[1, 2], [3, 4]], np.float64)
+    assert keep.dtype == np.float64 and keep.flags["C_CONTIGUOUS"]
+    t = torch.zeros(6, dtype=torch.float64)
+    assert _ptr(t, np.float64)[1] is t
+    with pytest.raises(TypeError):
+        _ptr(torch.zeros(6, dtype=torch.float32), np.float64)
+    with pytest.raises(TypeError):
+        _ptr(torch.zeros(6, dtype=torch.int64), np.int32)
+    _ptr(torch.zeros(6, dtype=torch.bool), np.uint8)
+    with pytest.raises(ValueError):
+        _ptr(torch.zeros((4, 4), dtype=torch.float64).t(), np.float64)
+    assert _ptr(None, np.float64) == (None, None)
