@@ -133,3 +133,16 @@ def test_locate_large_mesh(gpu):
     q = np.concatenate([g.uniform(-1.0, 424.0, (2000, 2)), uv[g.integers(0, len(uv), 200)]])
     face = check(uv, tri, q)[0]
     assert (face >= 0).mean() > 0.9
+
+
+@pytest.mark.parametrize("npts,seed", [(300, 31), (5000, 32)])
+def test_locate_delaunay_mesh(gpu, npts, seed):
+    # irregular triangles of every shape and size, faces shuffled
+    from scipy.spatial import Delaunay
+    g = np.random.default_rng(seed)
+    uv = g.random((npts, 2)) * 10.0
+    tri = Delaunay(uv).simplices.astype(np.int32)
+    tri = tri[g.permutation(len(tri))]
+    p3 = np.stack([uv[:, 0], uv[:, 1], uv[:, 0] * 0.3 - uv[:, 1] * 0.2], 1)
+    face = check(uv, tri, queries(uv, tri, seed, 4000), p3)[0]
+    assert (face >= 0).mean() > 0.5

@@ -149,3 +149,17 @@ def test_weights_many_vertices(gpu):
     from cvtamd import workloads as wl
     uv, p3, tri, _ = wl.patch_mesh(m=2100)
     check(uv, p3, tri)
+
+
+@pytest.mark.parametrize("npts,seed", [(300, 21), (5000, 22)])
+def test_weights_delaunay_mesh(gpu, npts, seed):
+    # irregular valences: a Delaunay triangulation of random points, faces
+    # shuffled, over a curved surface
+    from scipy.spatial import Delaunay
+    g = np.random.default_rng(seed)
+    uv = g.random((npts, 2)) * 10.0
+    tri = Delaunay(uv).simplices.astype(np.int32)
+    tri = tri[g.permutation(len(tri))]
+    p3 = np.stack([uv[:, 0], uv[:, 1], np.sin(uv[:, 0]) * np.cos(0.7 * uv[:, 1])], 1)
+    w = check(uv, p3, tri)
+    assert np.all(np.isfinite(w)) and np.all(w >= 1.0 - 1e-12)
