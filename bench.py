@@ -246,7 +246,8 @@ def locate_sample(with_cpu: bool) -> dict:
     """recover()'s point location on one patch mesh: 49152 queries (16384
     sites + 32768 triangle centroids) in a 11858-face mesh, on the GPU
     (inputs resident in device memory; the call includes its binning and
-    returns when the faces, weights and 3D points are written) and, as the
+    returns when the faces, weights and 3D points are written, to host
+    arrays and, in gpu_ms_device_out, to device tensors) and, as the
     baseline, the oracle's face-by-face scan on the host cores."""
     import numpy as np
     import torch
@@ -259,8 +260,20 @@ def locate_sample(with_cpu: bool) -> dict:
     for _ in range(reps):
         face, w, out3 = recover.locate_points(d[0], d[2], d[3], p3=d[1])
     gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    # the same calls with the results kept in device memory (no download)
+    dev = (torch.empty(len(q), dtype=torch.int32, device="cuda"),
+           torch.empty((len(q), 3), dtype=torch.float64, device="cuda"),
+           torch.empty((len(q), 3), dtype=torch.float64, device="cuda"))
+    recover.locate_points(d[0], d[2], d[3], p3=d[1], out=dev)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        recover.locate_points(d[0], d[2], d[3], p3=d[1], out=dev)
+    dev_ms = (time.perf_counter() - t0) / reps * 1e3
     res = {"queries": int(len(q)), "faces": int(len(tri)), "located": int((face >= 0).sum()),
-           "gpu_ms": round(gpu_ms, 3), "gpu_queries_per_s": round(len(q) / gpu_ms * 1e3, 1)}
+           "gpu_ms": round(gpu_ms, 3), "gpu_queries_per_s": round(len(q) / gpu_ms * 1e3, 1),
+           "gpu_ms_device_out": round(dev_ms, 3),
+           "device_out_equal": bool(np.array_equal(dev[0].cpu().numpy(), face)
+                                    and np.array_equal(dev[2].cpu().numpy(), out3))}
     if with_cpu:
         from oracle import oracle as orc
         orc.set_threads(min(orc.max_threads(), int(os.environ.get("OMP_NUM_THREADS", "64"))))

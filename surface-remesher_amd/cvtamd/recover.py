@@ -20,14 +20,16 @@ from ._lib import call
 from .engine import _ptr
 
 
-def locate_points(uv, tri, q, p3=None):
+def locate_points(uv, tri, q, p3=None, out=None):
     """First face (in face order) holding each query point, -1 for none.
 
     uv: (nv, 2) float64 vertex coordinates; tri: (nf, 3) int32 vertex
     indices; q: (nq, 2) float64 points; p3: optional (nv, 3) float64 3D
     vertices.  Returns (face (nq,) int32, weights (nq, 3) float64) and, with
     p3, the interpolated 3D points (nq, 3).  numpy arrays or torch tensors
-    (host or device memory) are accepted; results are numpy arrays."""
+    (host or device memory) are accepted.  out: None for new numpy arrays,
+    or the arrays/tensors to write (host or device memory), (face, w) or,
+    with p3, (face, w, xyz)."""
     uvp, k0 = _ptr(uv, np.float64)
     trp, k1 = _ptr(tri, np.int32)
     qp, k2 = _ptr(q, np.float64)
@@ -35,11 +37,22 @@ def locate_points(uv, tri, q, p3=None):
     nv = int(np.prod(k0.shape)) // 2 if k0 is not None else 0
     nf = int(np.prod(k1.shape)) // 3 if k1 is not None else 0
     nq = int(np.prod(k2.shape)) // 2
-    face = np.empty(nq, np.int32)
-    w = np.empty((nq, 3), np.float64)
-    out3 = np.empty((nq, 3), np.float64) if p3 is not None else None
-    call("cvt_locate_points", uvp, p3p, nv, trp, nf, qp, nq, _ptr(face)[0], _ptr(w)[0],
-         _ptr(out3)[0] if out3 is not None else None)
+    if out is None:
+        face = np.empty(nq, np.int32)
+        w = np.empty((nq, 3), np.float64)
+        out3 = np.empty((nq, 3), np.float64) if p3 is not None else None
+    else:
+        if len(out) != (2 if p3 is None else 3):
+            raise ValueError("out must be (face, w) or, with p3, (face, w, xyz)")
+        face, w = out[0], out[1]
+        out3 = out[2] if p3 is not None else None
+        sizes = [(face, nq), (w, 3 * nq)] + ([(out3, 3 * nq)] if out3 is not None else [])
+        if any(int(np.prod(a.shape)) != n for a, n in sizes):
+            raise ValueError("out arrays must hold nq, 3 nq (and 3 nq) values")
+    pf, kf = _ptr(face, np.int32)
+    pw, kw = _ptr(w, np.float64)
+    po, ko = _ptr(out3, np.float64)
+    call("cvt_locate_points", uvp, p3p, nv, trp, nf, qp, nq, pf, pw, po)
     return (face, w) if p3 is None else (face, w, out3)
 
 

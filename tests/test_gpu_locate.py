@@ -146,3 +146,23 @@ def test_locate_delaunay_mesh(gpu, npts, seed):
     p3 = np.stack([uv[:, 0], uv[:, 1], uv[:, 0] * 0.3 - uv[:, 1] * 0.2], 1)
     face = check(uv, tri, queries(uv, tri, seed, 4000), p3)[0]
     assert (face >= 0).mean() > 0.5
+
+
+def test_locate_writes_device_outputs(gpu):
+    import torch
+    from cvtamd import recover
+    uv, tri = jittered_mesh(25, 15)
+    p3 = np.stack([uv[:, 0], uv[:, 1], 0.5 * uv[:, 0]], 1)
+    q = queries(uv, tri, 16, 2000)
+    host = recover.locate_points(uv, tri, q, p3=p3)
+    n = len(q)
+    out = (torch.full((n,), 7, dtype=torch.int32, device="cuda"),
+           torch.empty((n, 3), dtype=torch.float64, device="cuda"),
+           torch.empty((n, 3), dtype=torch.float64, device="cuda"))
+    got = recover.locate_points(torch.from_numpy(uv).cuda(), torch.from_numpy(tri).cuda(),
+                                torch.from_numpy(q).cuda(), p3=torch.from_numpy(p3).cuda(), out=out)
+    assert all(a is b for a, b in zip(got, out))
+    for a, b in zip(host, out):
+        assert np.array_equal(a, b.cpu().numpy())
+    with pytest.raises(ValueError):
+        recover.locate_points(uv, tri, q, p3=p3, out=out[:2])
