@@ -206,7 +206,8 @@ def unet_mfma(batch: int = 16, steps: int = 10) -> dict:
 def raster_sample(with_cpu: bool) -> dict:
     """The density rasterizer (discretization_d rule) on the patch mesh of
     locate_sample, scaled to the grid: 11858 triangles at 2048^2 on the GPU
-    (host buffers in and out, as the reference's caller passes them); the
+    (host buffers in and out, as the reference's caller passes them, and in
+    gpu_ms_device_io with the mesh and the density in device memory); the
     oracle's triangle-by-triangle scan on the host cores at 256^2 (a bounded
     sample of the same mesh), compared per pixel, with a bit-exactness check
     of the GPU at that size."""
@@ -222,8 +223,19 @@ def raster_sample(with_cpu: bool) -> dict:
     for _ in range(reps):
         dens = compat.rasterize(uv, wt, tri, scale, n)
     gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    # the same calls with the mesh and the density in device memory
+    import torch
+    d_uv, d_wt, d_tri = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (uv, wt, tri))
+    d_den = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    compat.rasterize(d_uv, d_wt, d_tri, scale, n, out=d_den)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        compat.rasterize(d_uv, d_wt, d_tri, scale, n, out=d_den)
+    dev_ms = (time.perf_counter() - t0) / reps * 1e3
     res = {"grid": n, "triangles": int(len(tri)), "gpu_ms": round(gpu_ms, 3),
            "gpu_pixels_per_s": round(n * n / gpu_ms * 1e3, 1),
+           "gpu_ms_device_io": round(dev_ms, 3),
+           "device_io_equal": bool(np.array_equal(d_den.cpu().numpy(), dens)),
            "covered": round(float((dens > 0).mean()), 4)}
     if with_cpu:
         from oracle import oracle as orc
