@@ -304,7 +304,8 @@ def weights_sample(with_cpu: bool) -> dict:
     """Per-vertex density weights (weighting_vertices_with_area rule) on the
     patch mesh of locate_sample (11858 faces) and on a 1024 x 1024 vertex
     patch (2.09M faces), inputs resident in device memory, the call
-    returning when the weights are written; the baseline is the oracle's
+    returning when the weights are written (to a host array, and in
+    gpu_ms_device_out to a device tensor); the baseline is the oracle's
     face loop, single-threaded as the reference's host loop is (its sums
     are sequential in face order), median of as many runs as the GPU's."""
     import numpy as np
@@ -319,8 +320,16 @@ def weights_sample(with_cpu: bool) -> dict:
         for _ in range(reps):
             w = weighting.vertex_weights(*d)
         gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+        d_w = torch.empty(len(uv), dtype=torch.float64, device="cuda")
+        weighting.vertex_weights(*d, out=d_w)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            weighting.vertex_weights(*d, out=d_w)
+        dev_ms = (time.perf_counter() - t0) / reps * 1e3
         r = {"faces": int(len(tri)), "vertices": int(len(uv)), "gpu_ms": round(gpu_ms, 3),
-             "gpu_faces_per_s": round(len(tri) / gpu_ms * 1e3, 1)}
+             "gpu_faces_per_s": round(len(tri) / gpu_ms * 1e3, 1),
+             "gpu_ms_device_out": round(dev_ms, 3),
+             "device_out_equal": bool(np.array_equal(d_w.cpu().numpy(), w, equal_nan=True))}
         if with_cpu:
             from oracle import oracle as orc
             ts = []
