@@ -133,12 +133,42 @@ int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms);
  * labelled with their own coordinates) and overwritten with the final label
  * map; density is size x size fp32; mask (optional, one byte per pixel)
  * marks fixed sites.  depth > 1 adds coarse levels down to 256 on a 2x2-mean
- * density pyramid.  iterations / num_sites (optional) receive the iteration
- * count and the final site count.  Differences from the reference: JFA(+1)
- * instead of exact PBA+ for the Voronoi map, float64 moments and energy, any
- * size in [2, 8192] (the reference needs a power of two >= 256). */
+ * density pyramid, as the reference does (source/gcvt.cu:1087-1147): the
+ * initial sites are read from the first nc^2 (x, y) pairs of voronoi as an
+ * nc x nc map (nc = the coarsest side), the mask is read at each level with
+ * that level's row stride, the iteration count, omega and the energies of the
+ * decade checks run on across levels, and each level runs at least one
+ * iteration (so the count can exceed max_iter by depth - 1).
+ * iterations / num_sites (optional) receive the iteration count and the final
+ * site count.  Differences from the reference: JFA(+1) instead of exact PBA+
+ * for the Voronoi map; float64 moments and energy; a pixel is a site only
+ * when it holds its own coordinates (PBA+ also takes a pixel holding other
+ * coordinates as a site, and in the coarse map read from a full-size one
+ * such values can lie outside the coarse grid, where the reference's
+ * kernelVoronoi1D writes outside its shared array); any size in [2, 8192]
+ * (the reference needs a power of two >= 256). */
 int cvt_compat_gcvt(int16_t *voronoi, const float *density, const uint8_t *mask, int size,
                     int depth, int max_iter, int *iterations, int *num_sites);
+
+/* The reference's initial sites for gCVT (putConstrains + randomPoints,
+ * source/gcvt.h:76-122, run by centroidalVoronoi at :124-139), as a call a
+ * Linux host can make: the reference's gcvt.h does not compile on glibc
+ * (its `double random()` collides with POSIX `long random()`).  voronoi
+ * (size x size interleaved (x, y) int16) is overwritten: mask pixels (mask
+ * may be null) hold their own coordinates, every other pixel MARKER; then
+ * num sites are drawn by rejection (x = floor(u size), y = floor(u size),
+ * z = u mx, accepted when the pixel is free and density > z, with
+ * mx = min(max density, 100 * mean of the non-zero density)).  u is the KISS
+ * generator of source/gcvt.h:58-74 with the reference's LLP64 semantics
+ * (32-bit words, u = draw / 2^32).  rng_state: the generator's four words
+ * (z, w, jsr, jcong), read and advanced; NULL starts from four zero words,
+ * as the reference does (it never calls randinit).  draws (optional)
+ * receives the number of generator draws.  Runs on the host (the loop is
+ * sequential); buffers may be host or device memory.  CVT_ERR_ARG when
+ * fewer free pixels of positive density exist than num (the reference's loop
+ * would not end) or the density holds a negative, NaN or infinite value. */
+int cvt_ref_init_sites(int16_t *voronoi, const float *density, const uint8_t *mask, int size,
+                       int num, uint32_t *rng_state, long long *draws);
 
 /* Density rasterizer (the semantics of discretization_d,
  * source/discretization.cu:57-120): density[y*n + x] for the sample point

@@ -36,7 +36,15 @@
  *   orc_ref_rand_*     -- the LCG values 1234567, 3667164066, ... and the
  *                         first uniform-density site (0, 437) at 512;
  *   orc_compat_lloyd   -- update rule on hand-checked cases, fixed sites
- *                         never move, sites never land on rho == 0, merges.
+ *                         never move, sites never land on rho == 0, merges;
+ *   orc_compat_gcvt    -- hand-traced schedules: a lattice fixed point stops
+ *                         after exactly 20 iterations; a one-site case whose
+ *                         energies, omega = 1.25 after the second decade and
+ *                         final position (1, 4) after 30 iterations follow
+ *                         from closed forms; a two-level case where omega and
+ *                         lastE carried from the coarse level give 30
+ *                         iterations and the site (256, 256); depth 2 with
+ *                         maxIter 15 runs 16 iterations; the level count.
  *   orc_locate         -- exact weights at the vertices ((1,0,0), (0,1,0),
  *                         (0,0,1)), ~1/3 at centroids, reconstruction
  *                         sum w_j uv_j == q, first face in order on shared
@@ -546,20 +554,52 @@ int orc_compat_update(int n, const float *rho, const uint8_t *mask,
     return k;
 }
 
-/* Integer-site Lloyd with the reference schedule (source/gcvt.cu:1105-1147):
- * omega = 2; the energy is taken on iterations 0, 10, 20, ... before that
- * iteration's update; after every 10th update dE = E_prev - E,
- * omega = min(2, 1 + dE) and the run stops when dE / 10 < 1e-5; at most
- * maxIter iterations.  The energy here is the true float64 value, not the
- * reference's biased float32 reduction.  One final Voronoi pass labels the
- * result; vor receives the full (x, y) coordinate label map.  Single level
- * (depth 1).  exact selects the exact nearest-site map instead of JFA(+1).
- * Returns the number of iterations run; elog (optional, >= maxIter/10 + 1
- * entries) receives the sampled energies and *nlog their count. */
-int orc_compat_lloyd(int n, const float *rho, int16_t *vor, const uint8_t *mask,
-                     int maxIter, int exact, double *elog, int *nlog,
-                     int *num_sites_out)
+/* Number of pyramid levels gCVT runs for a side n and a requested depth
+ * (source/gcvt.cu:1091): the first level whose side falls below 256 and
+ * every coarser one are dropped.  The reference takes only powers of two
+ * >= 256; for other sides a level is added only while the side halves
+ * exactly, and a side below 256 runs one level (the reference would read
+ * past the caller's map there). */
+int orc_compat_levels(int n, int depth)
 {
+    int levels = 1;
+    while (levels < depth && (n % (1 << levels)) == 0 && (n >> levels) >= 256) ++levels;
+    return levels;
+}
+
+/* Integer-site CVT with the reference's schedule and pyramid
+ * (source/gcvt.cu:1087-1147).  With L levels (orc_compat_levels), level l has
+ * side m = n >> l:
+ *   - densities: level 0 is rho; level l is the 2x2 mean of level l - 1,
+ *     summed in the order (2x, 2y), (2x, 2y+1), (2x+1, 2y), (2x+1, 2y+1) and
+ *     divided by 4.0 (kernelDensityScaling, :497-511);
+ *   - initial sites: the reference copies the first m_c^2 (x, y) pairs of the
+ *     caller's map, m_c = n >> (L-1), into its m_c x m_c map (:1099-1103), so
+ *     the initial sites are the self-labelled pixels of that m_c x m_c
+ *     row-major prefix (for L = 1 the whole map);
+ *   - fixed sites: the update reads mask at the site's pixel index of the
+ *     current level, y * m + x (kernelUpdateSites indexes the full-size mask
+ *     with the level's TOID, :1033), so level l uses the first m^2 bytes of
+ *     mask as an m x m map;
+ *   - schedule: one iteration counter `it`, omega, the last sampled energy E
+ *     and lastE are shared by all levels (set once at :1105-1108).  On
+ *     iterations with it % 10 == 0 the energy of the current map is sampled
+ *     before the update and scaled by 4^l (:1082); after every update with
+ *     it % 10 == 0: dE = lastE - E, omega = min(2, 1 + dE), stop the level
+ *     when dE / 10 < 0.3 (coarse levels) or < 1e-5 (level 0), else
+ *     lastE = E.  Each level runs at least one iteration (do ... while
+ *     (it < maxIter)), so the total can exceed maxIter by L - 1;
+ *   - between levels the site coordinates double (kernelZoomIn, :485-495).
+ * The energy is the true float64 value (the reference's float32 reduction
+ * is biased, :1070-1078).  One final Voronoi pass labels the result at side n;
+ * vor receives the full (x, y) coordinate label map.  exact selects the exact
+ * nearest-site map instead of JFA(+1).  Returns the number of iterations;
+ * elog (optional, >= maxIter/10 + L + 1 entries) receives the sampled
+ * energies (scaled) and *nlog their count. */
+int orc_compat_gcvt(int n, const float *rho, int16_t *vor, const uint8_t *mask, int depth,
+                    int maxIter, int exact, double *elog, int *nlog, int *num_sites_out)
+{
+    const int L = orc_compat_levels(n, depth < 1 ? 1 : depth);
     size_t np = (size_t)n * n;
     float *xy = (float *)malloc(sizeof(float) * 2 * np);
     float *xy2 = (float *)malloc(sizeof(float) * 2 * np);
@@ -567,30 +607,55 @@ int orc_compat_lloyd(int n, const float *rho, int16_t *vor, const uint8_t *mask,
     int32_t *lab = (int32_t *)malloc(sizeof(int32_t) * np);
     int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * np);
     double *m = (double *)malloc(sizeof(double) * 4 * np);
-    int S = orc_compat_decode(vor, mask, n, xy, fx);
+    const float *lev[16];
+    lev[0] = rho;
+    for (int l = 1; l < L; ++l) {
+        const int ms = n >> l, w = 2 * ms;
+        float *d = (float *)malloc(sizeof(float) * (size_t)ms * ms);
+        for (int ty = 0; ty < ms; ++ty)
+            for (int tx = 0; tx < ms; ++tx) {
+                float s = 0.0f;
+                for (int x = 2 * tx; x < 2 * tx + 2; ++x)
+                    for (int y = 2 * ty; y < 2 * ty + 2; ++y) s += lev[l - 1][(size_t)y * w + x];
+                d[(size_t)ty * ms + tx] = (float)((double)s / 4.0);
+            }
+        lev[l] = d;
+    }
+    int S = orc_compat_decode(vor, NULL, n >> (L - 1), xy, fx);
     double E = 0, lastE = 1e18;
     float omega = 2.0f;
     int it = 0, nl = 0;
-    do {
-        if (exact) orc_voronoi_exact(n, n, xy, S, lab);
-        else orc_jfa(n, n, xy, S, lab, tmp);
-        orc_moments(n, n, rho, lab, xy, S, m);
-        if (it % 10 == 0) {
-            orc_energy_from_moments(n, n, m, S, &E, NULL);
-            if (elog) elog[nl] = E;
-            ++nl;
-        }
-        S = orc_compat_update(n, rho, mask, m, S, omega, xy, xy2, tmp);
-        { float *t = xy; xy = xy2; xy2 = t; }
-        ++it;
-        if (it % 10 == 0) {
-            double dE = lastE - E;
-            double om = 1.0 + dE;
-            omega = (float)(om < 2.0 ? om : 2.0);
-            if (dE / 10.0 < 1e-5) break;
-            lastE = E;
-        }
-    } while (it < maxIter);
+    for (int l = L - 1; l >= 0; --l) {
+        const int ms = n >> l;
+        const double escale = ldexp(1.0, 2 * l);
+        const double threshold = l ? 3e-1 : 1e-5;
+        do {
+            if (exact) orc_voronoi_exact(ms, ms, xy, S, lab);
+            else orc_jfa(ms, ms, xy, S, lab, tmp);
+            orc_moments(ms, ms, lev[l], lab, xy, S, m);
+            if (it % 10 == 0) {
+                orc_energy_from_moments(ms, ms, m, S, &E, NULL);
+                E *= escale;
+                if (elog) elog[nl] = E;
+                ++nl;
+            }
+            S = orc_compat_update(ms, lev[l], mask, m, S, omega, xy, xy2, tmp);
+            { float *t = xy; xy = xy2; xy2 = t; }
+            ++it;
+            if (it % 10 == 0) {
+                double dE = lastE - E;
+                double om = 1.0 + dE;
+                omega = (float)(om < 2.0 ? om : 2.0);
+                if (dE / 10.0 < threshold) break;
+                lastE = E;
+            }
+        } while (it < maxIter);
+        if (l > 0)
+            for (int i = 0; i < S; ++i) {
+                xy[2 * i] = (float)(2 * (int)floorf(xy[2 * i])) + 0.5f;
+                xy[2 * i + 1] = (float)(2 * (int)floorf(xy[2 * i + 1])) + 0.5f;
+            }
+    }
     if (exact) orc_voronoi_exact(n, n, xy, S, lab);
     else orc_jfa(n, n, xy, S, lab, tmp);
     for (size_t p = 0; p < np; ++p) {
@@ -601,8 +666,17 @@ int orc_compat_lloyd(int n, const float *rho, int16_t *vor, const uint8_t *mask,
     }
     if (nlog) *nlog = nl;
     if (num_sites_out) *num_sites_out = S;
+    for (int l = 1; l < L; ++l) free((void *)lev[l]);
     free(xy); free(xy2); free(fx); free(lab); free(tmp); free(m);
     return it;
+}
+
+/* The single-level form (depth 1) of orc_compat_gcvt. */
+int orc_compat_lloyd(int n, const float *rho, int16_t *vor, const uint8_t *mask,
+                     int maxIter, int exact, double *elog, int *nlog,
+                     int *num_sites_out)
+{
+    return orc_compat_gcvt(n, rho, vor, mask, 1, maxIter, exact, elog, nlog, num_sites_out);
 }
 
 /* ------------------------------------------------------------------------ */

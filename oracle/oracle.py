@@ -67,6 +67,9 @@ def lib():
                                         _f32p, _i32p]),
         "orc_compat_lloyd": (C.c_int, [C.c_int, _f32p, _i16p, vp, C.c_int, C.c_int, vp,
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "orc_compat_levels": (C.c_int, [C.c_int, C.c_int]),
+        "orc_compat_gcvt": (C.c_int, [C.c_int, _f32p, _i16p, vp, C.c_int, C.c_int, C.c_int, vp,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "orc_extract_points": (C.c_long, [_i16p, _u8p, C.c_int, C.c_double, C.c_double,
                                           C.c_double, _f64p]),
         "orc_locate": (None, [_f64p, vp, _i32p, C.c_int, _f64p, C.c_int, _i32p, vp, vp]),
@@ -242,19 +245,30 @@ def compat_update(rho, m, xy, omega, mask=None):
     return out[:k].copy()
 
 
-def compat_lloyd(rho, vor, mask=None, max_iter=1000, exact=False):
-    """Reference-compatible integer-site Lloyd; vor [n, n, 2] int16 is updated in place.
+def compat_levels(n, depth):
+    return lib().orc_compat_levels(int(n), int(depth))
+
+
+def compat_gcvt(rho, vor, mask=None, depth=1, max_iter=1000, exact=False):
+    """Reference-compatible integer-site CVT with gCVT's pyramid and schedule;
+    vor [n, n, 2] int16 is updated in place (for depth > 1 its first
+    (n >> (L-1))^2 pairs are read as the coarsest level's map).
     Returns (iterations, energies sampled every 10 iterations, number of sites)."""
     rho = np.ascontiguousarray(rho, np.float32)
     n = rho.shape[0]
     assert vor.flags["C_CONTIGUOUS"] and vor.dtype == np.int16
-    elog = np.zeros(max_iter // 10 + 2, np.float64)
+    elog = np.zeros(max_iter // 10 + 16, np.float64)
     nlog = C.c_int()
     ns = C.c_int()
     mk = None if mask is None else np.ascontiguousarray(mask, np.uint8)
-    it = lib().orc_compat_lloyd(n, rho, vor, _ptr(mk), max_iter, int(exact), _ptr(elog),
-                                C.byref(nlog), C.byref(ns))
+    it = lib().orc_compat_gcvt(n, rho, vor, _ptr(mk), int(depth), int(max_iter), int(exact),
+                               _ptr(elog), C.byref(nlog), C.byref(ns))
     return it, elog[: nlog.value].copy(), ns.value
+
+
+def compat_lloyd(rho, vor, mask=None, max_iter=1000, exact=False):
+    """The single-level (depth 1) form of compat_gcvt."""
+    return compat_gcvt(rho, vor, mask, 1, max_iter, exact)
 
 
 def extract_points(vor, mask, scale, l, b):

@@ -17,8 +17,15 @@
 //   * the energy is the exact float64 value, not the reference's biased
 //     float32 reduction (source/gcvt.cu:1070-1078);
 //   * one final Voronoi pass; the output is the full (x, y) label map.
-// depth > 1 runs the coarse-to-fine schedule of source/gcvt.cu:1110-1147 on
-// a 2x2-mean density pyramid (see run_level below).
+// depth > 1 runs the coarse-to-fine loop of source/gcvt.cu:1087-1147 on the
+// 2x2-mean density pyramid: the initial sites are read from the first nc^2
+// pairs of the caller's map as an nc x nc map (the coarsest side; the
+// reference copies exactly those), each level reads the caller's mask with
+// its own row stride, the iteration count, omega, the sampled energy and
+// lastE run on across levels (a level whose first decade check falls mid-way
+// uses the coarser level's last sample), the coarse stop threshold is 0.3,
+// every level runs at least one iteration, and site coordinates double
+// between levels.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -50,84 +57,97 @@ static __global__ void k_density_half(const float *__restrict__ in, float *__res
     out[i] = d / 4.0f;
 }
 
-// mask pooled to a coarser level: a coarse pixel is fixed if any of its
-// fine pixels is
-static __global__ void k_mask_half(const uint8_t *__restrict__ in, uint8_t *__restrict__ out, int n)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)n * n) return;
-    const int x = (int)(i % n), y = (int)(i / n);
-    const long w = 2L * n;
-    out[i] = in[(2L * y) * w + 2 * x] | in[(2L * y + 1) * w + 2 * x] |
-             in[(2L * y) * w + 2 * x + 1] | in[(2L * y + 1) * w + 2 * x + 1];
-}
-
-struct Level {
-    int n = 0;
-    float *rho = nullptr;     // device, n x n
-    uint8_t *mask = nullptr;  // device, n x n
+// State of gCVT's schedule shared by all levels (source/gcvt.cu:1105-1108):
+// the iteration count, omega, the last sampled energy and the energy of the
+// last decade check that went on.
+struct Schedule {
+    int it = 0;
+    float omega = 2.0f;
+    double E = 0.0, lastE = 1e18;
 };
 
 // One level of the integer-site Lloyd schedule on ctx (density and sites
-// already set).  Returns CVT_OK and advances *it.
-static int run_level(Ctx *c, const Level &lv, int max_iter, bool finest, int *it, int S0,
-                     float *domega, int *tgt, uint32_t *owner, double escale)
+// already set), continuing sc.  mask: the caller's mask, read with this
+// level's row stride (the reference indexes it with the level's TOID,
+// source/gcvt.cu:1033).  The level runs at least one iteration (do ...
+// while, source/gcvt.cu:1111-1142).
+static int run_level(Ctx *c, const float *rho, const uint8_t *mask, int max_iter, bool finest,
+                     Schedule &sc, int S0, float *domega, int *tgt, uint32_t *owner, double escale)
 {
     const double threshold = finest ? 1e-5 : 3e-1;   // source/gcvt.cu:1131-1137
     int nst[1] = {S0};
     if (hipMemcpyAsync(c->dstate + ST_NSITES, nst, sizeof(int), hipMemcpyHostToDevice, c->stream) !=
         hipSuccess)
         return fail(CVT_ERR_HIP, "state upload");
-    float omega = 2.0f;
-    if (hipMemcpyAsync(domega, &omega, sizeof omega, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    if (hipMemcpyAsync(domega, &sc.omega, sizeof sc.omega, hipMemcpyHostToDevice, c->stream) !=
+        hipSuccess)
         return fail(CVT_ERR_HIP, "omega upload");
     // one iteration, captured once
-    CompatArgs ca{lv.mask, lv.rho, tgt, owner};
+    CompatArgs ca{mask, rho, tgt, owner};
     hipGraph_t g = nullptr;
     hipGraphExec_t gx = nullptr;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return fail(CVT_ERR_HIP, "capture");
     int rc = c->enqueue_jfa();
     if (!rc) rc = c->enqueue_moments();
-    if (!rc) rc = c->enqueue_update(UPD_COMPAT | UPD_DEV_S, omega, nullptr, 1, nullptr, domega, ca);
+    if (!rc) rc = c->enqueue_update(UPD_COMPAT | UPD_DEV_S, sc.omega, nullptr, 1, nullptr, domega, ca);
     if (!rc)
         hipLaunchKernelGGL(k_compat_compact, dim3(1), dim3(1024), 0, c->stream, c->sites, tgt, owner,
-                           c->seed, lv.n, c->dstate);
+                           c->seed, c->W, c->dstate);
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (e != hipSuccess || hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return fail(CVT_ERR_HIP, "graph");
     }
-    double lastE = 1e18;
-    const int start = *it;
-    std::vector<double> h(2);
+    // the device history of this context holds one energy per iteration of
+    // this level, before its update: entry it - start
+    const int start = sc.it;
+    double h[2];
+    auto sample = [&](int k) -> int {
+        if (hipMemcpyAsync(h, c->hist + 2 * k, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail(CVT_ERR_HIP, "energy readback");
+        sc.E = h[0] * escale;   // coarse levels: x 4^level (source/gcvt.cu:1082)
+        return CVT_OK;
+    };
     do {
         if (hipGraphLaunch(gx, c->stream) != hipSuccess) { rc = fail(CVT_ERR_HIP, "launch"); break; }
-        ++*it;
-        if ((*it) % 10 == 0) {
-            // energy sampled before the update of iteration it - 10
-            const int k = (*it - 10) - start;
-            if (k < 0) continue;      // the level started mid-decade: no sample yet
-            if (hipMemcpyAsync(h.data(), c->hist + 2 * k, 2 * sizeof(double),
-                               hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-                hipStreamSynchronize(c->stream) != hipSuccess) {
-                rc = fail(CVT_ERR_HIP, "energy readback");
-                break;
-            }
-            const double E = h[0] * escale;   // coarse levels: x 4^level (source/gcvt.cu:1082)
-            const double dE = lastE - E;
-            omega = (float)std::min(2.0, 1.0 + dE);
+        ++sc.it;
+        if (sc.it % 10 == 0) {
+            // E is the energy of iteration it - 10; when that iteration ran on
+            // a coarser level, sc.E already holds it
+            const int k = (sc.it - 10) - start;
+            if (k >= 0 && (rc = sample(k))) break;
+            const double dE = sc.lastE - sc.E;
+            sc.omega = (float)std::min(2.0, 1.0 + dE);
             if (dE / 10.0 < threshold) break;
-            lastE = E;
-            if (hipMemcpyAsync(domega, &omega, sizeof omega, hipMemcpyHostToDevice, c->stream) !=
-                hipSuccess) { rc = fail(CVT_ERR_HIP, "omega upload"); break; }
+            sc.lastE = sc.E;
+            if (hipMemcpyAsync(domega, &sc.omega, sizeof sc.omega, hipMemcpyHostToDevice,
+                               c->stream) != hipSuccess) { rc = fail(CVT_ERR_HIP, "omega upload"); break; }
         }
-    } while (*it < max_iter);
+    } while (sc.it < max_iter);
+    // the level's last sample (the last multiple of 10 it ran) is the energy
+    // the next level's first decade check reads, as the reference's Energy
+    // variable carries it
+    const int last = ((sc.it - 1) / 10) * 10;
+    if (!rc && last >= start) rc = sample(last - start);
     (void)hipGraphExecDestroy(gx);
     (void)hipGraphDestroy(g);
     if (!rc) rc = c->check_flags();
     return rc;
+}
+
+// Pyramid levels for a side and a requested depth (source/gcvt.cu:1091):
+// levels stop before a side below 256; for sides the reference does not take
+// (not a power of two >= 256) a level is added only while the side halves
+// exactly, and a side below 256 runs one level.
+static int compat_levels(int n, int depth)
+{
+    int levels = 1;
+    while (levels < depth && (n % (1 << levels)) == 0 && (n >> levels) >= 256) ++levels;
+    return levels;
 }
 
 }  // namespace cvt
@@ -147,148 +167,95 @@ extern "C" int cvt_compat_gcvt(int16_t *voronoi, const float *density, const uin
     if (!voronoi || !density) return fail(CVT_ERR_ARG, "null array");
     if (size < 2 || size > 8192) return fail(CVT_ERR_ARG, "size %d outside [2, 8192]", size);
     if (depth < 1) return fail(CVT_ERR_ARG, "depth %d < 1", depth);
-    // levels: like the reference, coarse levels stop at 256 (source/gcvt.cu:1091),
-    // and each level halves the side exactly
-    int levels = 1;
-    while (levels < depth && (size % (1 << levels)) == 0 && (size >> levels) >= 256) ++levels;
-
+    const int levels = compat_levels(size, depth);
     const int n = size;
     const long np = (long)n * n;
+    // the coarsest level's side: its initial map is the first nc^2 (x, y)
+    // pairs of the caller's map, read with row stride nc (the reference
+    // copies exactly those into its nc x nc texture, source/gcvt.cu:1099-1103;
+    // for one level that is the whole map)
+    const int nc = n >> (levels - 1);
+    const long npc = (long)nc * nc;
     int rc = CVT_OK;
-    int it = 0, S = 0;
+    int S = 0;
+    Schedule sc;
     cvt_ctx *h = nullptr;
     Ctx *c = nullptr;
-    std::vector<Level> lv(levels);
+    std::vector<float *> lrho(levels, nullptr);
+    uint8_t *dmask = nullptr;
     float *domega = nullptr;
     int *tgt = nullptr;
     uint32_t *owner = nullptr;
     short2 *dcoords = nullptr;
-    std::vector<float> xy;
-    // sites from the input map (row-major); the map may live on either side
-    std::vector<int16_t> vin(2 * (size_t)np);
-    if (hipMemcpy(vin.data(), voronoi, sizeof(int16_t) * 2 * np, hipMemcpyDefault) != hipSuccess)
+    std::vector<float> cur;
+    // sites: the self-labelled pixels of that map, in row-major order; the
+    // map may live on either side
+    std::vector<int16_t> vin(2 * (size_t)npc);
+    if (hipMemcpy(vin.data(), voronoi, sizeof(int16_t) * 2 * npc, hipMemcpyDefault) != hipSuccess)
         return fail(CVT_ERR_HIP, "label map download");
-    for (int y = 0; y < n; ++y)
-        for (int x = 0; x < n; ++x) {
-            const long p = (long)y * n + x;
+    for (int y = 0; y < nc; ++y)
+        for (int x = 0; x < nc; ++x) {
+            const long p = (long)y * nc + x;
             if (vin[2 * p] == x && vin[2 * p + 1] == y) {
-                xy.push_back((float)x + 0.5f);
-                xy.push_back((float)y + 0.5f);
+                cur.push_back((float)x + 0.5f);
+                cur.push_back((float)y + 0.5f);
             }
         }
-    S = (int)(xy.size() / 2);
+    S = (int)(cur.size() / 2);
+    // the finest level's context (sites only merge, so S bounds every level)
     rc = cvt_create(&h, n, n, std::max(S, 1));
     if (rc) return rc;
     c = reinterpret_cast<Ctx *>(h);
     c->use_graph = false;
-    // device copies of the level densities and masks
-    lv[0].n = n;
-    lv[0].rho = c->rho;
-    CCHK(hipMalloc(&lv[0].mask, np), "alloc");
-    if (mask) CCHK(hipMemcpyAsync(lv[0].mask, mask, np, hipMemcpyDefault, c->stream), "mask upload");
-    else CCHK(hipMemsetAsync(lv[0].mask, 0, np, c->stream), "mask clear");
     rc = cvt_set_density(h, density);
     if (rc) goto done;
+    // the density pyramid (source/gcvt.cu:985-993) and the caller's mask,
+    // which every level reads with its own row stride
+    lrho[0] = c->rho;
     for (int l = 1; l < levels; ++l) {
-        lv[l].n = n >> l;
-        const long m = (long)lv[l].n * lv[l].n;
-        CCHK(hipMalloc(&lv[l].rho, sizeof(float) * m), "alloc");
-        CCHK(hipMalloc(&lv[l].mask, m), "alloc");
+        const long m = (long)(n >> l) * (n >> l);
+        CCHK(hipMalloc(&lrho[l], sizeof(float) * m), "alloc");
         hipLaunchKernelGGL(k_density_half, dim3((m + 255) / 256), dim3(256), 0, c->stream,
-                           lv[l - 1].rho, lv[l].rho, lv[l].n);
-        hipLaunchKernelGGL(k_mask_half, dim3((m + 255) / 256), dim3(256), 0, c->stream,
-                           lv[l - 1].mask, lv[l].mask, lv[l].n);
+                           lrho[l - 1], lrho[l], n >> l);
     }
+    CCHK(hipMalloc(&dmask, np), "alloc");
+    if (mask) CCHK(hipMemcpyAsync(dmask, mask, np, hipMemcpyDefault, c->stream), "mask upload");
+    else CCHK(hipMemsetAsync(dmask, 0, np, c->stream), "mask clear");
     CCHK(hipMalloc(&domega, sizeof(float)), "alloc");
     CCHK(hipMalloc(&tgt, sizeof(int) * std::max(S, 1)), "alloc");
     CCHK(hipMalloc(&owner, sizeof(uint32_t) * np), "alloc");
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, owner, (uint32_t)CVT_NONE, np);
     CCHK(hipStreamSynchronize(c->stream), "setup");
 
-    if (levels == 1) {
-        rc = cvt_set_sites(h, xy.data(), nullptr, S);
-        if (rc) goto done;
-        rc = run_level(c, lv[0], std::max(max_iter, 1), true, &it, S, domega, tgt, owner, 1.0);
-        if (rc) goto done;
-    } else {
-        // coarse to fine: sites at (x >> l, y >> l) merged per coarse pixel;
-        // after each coarser level the site coordinates double (source/gcvt.cu:485-495)
-        std::vector<float> cur;
-        for (int l = levels - 1; l >= 0; --l) {
-            const int ln = lv[l].n;
-            std::vector<float> lxy;
-            if (l == levels - 1) {
-                std::vector<uint8_t> seen((size_t)ln * ln, 0);
-                for (int k = 0; k < S; ++k) {
-                    const int x = (int)xy[2 * k] >> l, y = (int)xy[2 * k + 1] >> l;
-                    if (seen[(size_t)y * ln + x]) continue;
-                    seen[(size_t)y * ln + x] = 1;
-                    lxy.push_back((float)x + 0.5f);
-                    lxy.push_back((float)y + 0.5f);
-                }
-            } else {
-                for (size_t k = 0; k + 1 < cur.size(); k += 2) {
-                    lxy.push_back((float)(2 * (int)cur[k]) + 0.5f);
-                    lxy.push_back((float)(2 * (int)cur[k + 1]) + 0.5f);
-                }
-            }
-            // re-point the context at this level's grid
-            cvt_ctx *hl = nullptr;
-            const int Sl = (int)(lxy.size() / 2);
-            rc = cvt_create(&hl, ln, ln, std::max(Sl, 1));
+    // coarse to fine (source/gcvt.cu:1110-1147); between levels the site
+    // coordinates double (kernelZoomIn, source/gcvt.cu:485-495)
+    for (int l = levels - 1; l >= 0; --l) {
+        const int m = n >> l;
+        cvt_ctx *hl = h;
+        if (l > 0) {
+            rc = cvt_create(&hl, m, m, std::max(S, 1));
             if (rc) goto done;
-            Ctx *cl = reinterpret_cast<Ctx *>(hl);
-            cl->use_graph = false;
-            if (l == 0) rc = cvt_set_density(hl, density);
-            else rc = cvt_set_density(hl, lv[l].rho);
-            Level use = lv[l];
-            if (l == 0) use.rho = cl->rho;
-            if (!rc) rc = cvt_set_sites(hl, lxy.data(), nullptr, Sl);
-            if (!rc)
-                rc = run_level(cl, use, std::max(max_iter, 1), l == 0, &it, Sl, domega, tgt, owner,
-                               ldexp(1.0, 2 * l));
-            if (!rc) {
-                int nst = 0;
-                if (hipMemcpy(&nst, cl->dstate + ST_NSITES, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-                    rc = fail(CVT_ERR_HIP, "readback");
-                cur.resize(2 * (size_t)nst);
-                if (!rc && nst > 0 && hipMemcpy(cur.data(), cl->sites, sizeof(float2) * nst,
-                                                hipMemcpyDeviceToHost) != hipSuccess)
-                    rc = fail(CVT_ERR_HIP, "readback");
-            }
-            if (l == 0 && !rc) {
-                // keep the finest context for the final pass
-                (void)cvt_destroy(h);
-                h = hl;
-                c = cl;
-            } else {
-                (void)cvt_destroy(hl);
-            }
-            if (rc) goto done;
-            if (it >= std::max(max_iter, 1) && l > 0) {
-                // iteration budget spent on a coarse level: zoom the rest
-                for (int z = l - 1; z >= 0; --z) {
-                    std::vector<float> nxt;
-                    for (size_t k = 0; k + 1 < cur.size(); k += 2) {
-                        nxt.push_back((float)(2 * (int)cur[k]) + 0.5f);
-                        nxt.push_back((float)(2 * (int)cur[k + 1]) + 0.5f);
-                    }
-                    cur.swap(nxt);
-                }
-                cvt_ctx *hf = nullptr;
-                rc = cvt_create(&hf, n, n, std::max((int)(cur.size() / 2), 1));
-                if (rc) goto done;
-                (void)cvt_destroy(h);
-                h = hf;
-                c = reinterpret_cast<Ctx *>(hf);
-                rc = cvt_set_density(h, density);
-                if (!rc) rc = cvt_set_sites(h, cur.data(), nullptr, (int)(cur.size() / 2));
-                if (rc) goto done;
-                int nst = (int)(cur.size() / 2);
-                CCHK(hipMemcpy(c->dstate + ST_NSITES, &nst, sizeof(int), hipMemcpyHostToDevice), "state");
-                break;
-            }
+            reinterpret_cast<Ctx *>(hl)->use_graph = false;
+            rc = cvt_set_density(hl, lrho[l]);
         }
+        Ctx *cl = reinterpret_cast<Ctx *>(hl);
+        const int Sl = (int)(cur.size() / 2);
+        if (!rc) rc = cvt_set_sites(hl, cur.data(), nullptr, Sl);
+        if (!rc)
+            rc = run_level(cl, cl->rho, dmask, max_iter, l == 0, sc, Sl, domega, tgt, owner,
+                           ldexp(1.0, 2 * l));
+        if (!rc && l > 0) {
+            int nst = 0;
+            if (hipMemcpy(&nst, cl->dstate + ST_NSITES, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+                rc = fail(CVT_ERR_HIP, "readback");
+            cur.resize(2 * (size_t)nst);
+            if (!rc && nst > 0 && hipMemcpy(cur.data(), cl->sites, sizeof(float2) * nst,
+                                            hipMemcpyDeviceToHost) != hipSuccess)
+                rc = fail(CVT_ERR_HIP, "readback");
+            for (size_t k = 0; k < cur.size(); ++k) cur[k] = (float)(2 * (int)cur[k]) + 0.5f;
+        }
+        if (l > 0) (void)cvt_destroy(hl);
+        if (rc) goto done;
     }
     // final Voronoi of the surviving sites, as (x, y) coordinates
     {
@@ -306,13 +273,12 @@ extern "C" int cvt_compat_gcvt(int16_t *voronoi, const float *density, const uin
              "download");
         CCHK(hipStreamSynchronize(c->stream), "final pass");
         if (num_sites) *num_sites = nst;
-        if (iterations) *iterations = it;
+        if (iterations) *iterations = sc.it;
     }
 done:
-    for (int l = 0; l < levels; ++l) {
-        if (lv[l].mask) (void)hipFree(lv[l].mask);
-        if (l > 0 && lv[l].rho) (void)hipFree(lv[l].rho);
-    }
+    for (int l = 1; l < levels; ++l)
+        if (lrho[l]) (void)hipFree(lrho[l]);
+    if (dmask) (void)hipFree(dmask);
     if (domega) (void)hipFree(domega);
     if (tgt) (void)hipFree(tgt);
     if (owner) (void)hipFree(owner);

@@ -9,7 +9,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import call, lib
-from .engine import _ptr
+from .engine import _out_ptr, _ptr
 
 MARKER = -32768
 
@@ -25,6 +25,25 @@ def gcvt(voronoi: np.ndarray, density, mask=None, depth: int = 1, max_iter: int 
     pm, k2 = _ptr(mask, np.uint8)
     call("cvt_compat_gcvt", pv, pd, pm, int(n), int(depth), int(max_iter), C.byref(it), C.byref(ns))
     return it.value, ns.value
+
+
+def ref_init_sites(density, num: int, mask=None, rng_state=None, out=None):
+    """The reference's initial sites for gCVT (putConstrains + randomPoints,
+    cvt_ref_init_sites): an (n, n, 2) int16 map with the mask pixels as fixed
+    sites and num sites drawn by the reference's rejection rule and LLP64
+    KISS generator.  rng_state: None (the reference's zero state) or a
+    4-element uint32 array, advanced in place.  out: an (n, n, 2) int16 array
+    or tensor to write instead of a new array.  Returns (map, draws)."""
+    n = int(density.shape[0])
+    if out is None:
+        out = np.empty((n, n, 2), np.int16)
+    po = _out_ptr(out, np.int16, 2 * n * n)
+    pd, kd = _ptr(density, np.float32)
+    pm, km = _ptr(mask, np.uint8)
+    ps = None if rng_state is None else _out_ptr(rng_state, np.uint32, 4)
+    draws = C.c_longlong()
+    call("cvt_ref_init_sites", po, pd, pm, n, int(num), ps, C.byref(draws))
+    return out, draws.value
 
 
 def rasterize(points, weight, triangles, scale: float, n: int, out=None):
@@ -43,9 +62,7 @@ def rasterize(points, weight, triangles, scale: float, n: int, out=None):
         raise ValueError("weight must hold one value per point")
     if out is None:
         out = np.empty((n, n), np.float32)
-    if int(np.prod(out.shape)) != n * n:
-        raise ValueError("out must hold n * n values")
-    po, k = _ptr(out, np.float32)
+    po = _out_ptr(out, np.float32, n * n)
     call("cvt_rasterize", pp, pw, nv, pt, nf, po, float(scale), int(n))
     return out
 

@@ -14,8 +14,19 @@ import numpy as np
 from ._lib import LABEL_NONE, call, lib
 
 
+def _torch_ready(a):
+    """A torch device tensor may still be written by work queued on the
+    caller's current torch stream, which the library's streams do not wait
+    for: wait for that stream before the library reads or writes it."""
+    if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
+        import torch
+        torch.cuda.current_stream(a.device).synchronize()
+
+
 def _ptr(a, dtype=None):
-    """(pointer, keep-alive) for a numpy array or torch tensor."""
+    """(pointer, keep-alive) for an input: a numpy array (converted to a
+    contiguous copy of dtype when needed) or a contiguous torch tensor of
+    that dtype."""
     if a is None:
         return None, None
     if hasattr(a, "data_ptr"):
@@ -25,9 +36,38 @@ def _ptr(a, dtype=None):
             got, want = str(a.dtype).replace("torch.", ""), np.dtype(dtype).name
             if got != want and not (got == "bool" and want == "uint8"):
                 raise TypeError(f"tensor dtype {a.dtype}, expected {want}")
+        _torch_ready(a)
         return C.c_void_p(a.data_ptr()), a
     arr = np.ascontiguousarray(a, dtype=dtype)
     return arr.ctypes.data_as(C.c_void_p), arr
+
+
+def _out_ptr(a, dtype, count, torch_dtypes=()):
+    """Pointer to an output the library writes in place: a numpy array or
+    torch tensor of exactly dtype (or, for a tensor, one of torch_dtypes,
+    e.g. bfloat16 for raw 16-bit words), C-contiguous, writable and holding
+    exactly count elements.  Never a copy (a copy would receive the results
+    and the caller's array would be left unchanged): anything else raises."""
+    want = np.dtype(dtype)
+    if hasattr(a, "data_ptr"):
+        got = str(a.dtype).replace("torch.", "")
+        if got != want.name and got not in torch_dtypes:
+            raise TypeError(f"out tensor dtype {a.dtype}, expected {want.name}")
+        if not a.is_contiguous():
+            raise ValueError("out tensor must be contiguous")
+        if a.numel() != count:
+            raise ValueError(f"out tensor holds {a.numel()} values, expected {count}")
+        _torch_ready(a)
+        return C.c_void_p(a.data_ptr())
+    if not isinstance(a, np.ndarray):
+        raise TypeError("out must be a numpy array or a torch tensor")
+    if a.dtype != want:
+        raise TypeError(f"out array dtype {a.dtype}, expected {want.name}")
+    if not a.flags["C_CONTIGUOUS"] or not a.flags["WRITEABLE"]:
+        raise ValueError("out array must be C-contiguous and writable")
+    if a.size != count:
+        raise ValueError(f"out array holds {a.size} values, expected {count}")
+    return a.ctypes.data_as(C.c_void_p)
 
 
 class Engine:
@@ -87,28 +127,29 @@ class Engine:
         return call("cvt_num_sites", self._h)
 
     def sites(self, out=None):
+        """(num_sites, 2) float32 sites; out: an array or tensor of exactly
+        num_sites * 2 float32 values to write instead."""
         n = self.num_sites
         if out is None:
             out = np.empty((n, 2), np.float32)
-        p, keep = _ptr(out)
-        call("cvt_get_sites", self._h, p)
+        call("cvt_get_sites", self._h, _out_ptr(out, np.float32, 2 * n))
         return out
 
     def voronoi(self):
         call("cvt_voronoi", self._h)
 
     def labels(self, out=None):
+        """(height, width) int32 labels; out: an array or tensor of exactly
+        width * height int32 values to write instead."""
         if out is None:
             out = np.empty((self.height, self.width), np.int32)
-        p, keep = _ptr(out)
-        call("cvt_get_labels", self._h, p)
+        call("cvt_get_labels", self._h, _out_ptr(out, np.int32, self.width * self.height))
         return out
 
     def moments(self):
         n = self.num_sites
         out = np.zeros((max(n, 1), 4), np.float64)
-        p, keep = _ptr(out)
-        call("cvt_moments", self._h, p)
+        call("cvt_moments", self._h, _out_ptr(out, np.float64, out.size))
         return out[:n]
 
     def lloyd(self, iterations: int, omega: float = 1.0):
@@ -117,8 +158,7 @@ class Engine:
     def history(self):
         n = call("cvt_history_length", self._h)
         out = np.zeros((max(n, 1), 2), np.float64)
-        p, keep = _ptr(out)
-        call("cvt_get_history", self._h, p, n)
+        call("cvt_get_history", self._h, _out_ptr(out, np.float64, out.size), n)
         return out[:n]
 
     def clear_history(self):
@@ -135,8 +175,7 @@ class Engine:
         one dense JFA pass (passes 1 .. P-1, mean), jfa_first the first pass
         (from the seed map)."""
         ms = np.zeros(8, np.float64)
-        p, keep = _ptr(ms)
-        call("cvt_profile", self._h, int(iterations), float(omega), p)
+        call("cvt_profile", self._h, int(iterations), float(omega), _out_ptr(ms, np.float64, 8))
         return {"jfa_pass": ms[0], "jfa_first": ms[2], "moments": ms[1], "update": ms[3],
                 "step": ms[5], "jfa_passes": int(ms[6]), "tile": int(ms[7])}
 

@@ -17,7 +17,7 @@ from __future__ import annotations
 import numpy as np
 
 from ._lib import call
-from .engine import _ptr
+from .engine import _out_ptr, _ptr
 
 
 def locate_points(uv, tri, q, p3=None, out=None):
@@ -46,12 +46,9 @@ def locate_points(uv, tri, q, p3=None, out=None):
             raise ValueError("out must be (face, w) or, with p3, (face, w, xyz)")
         face, w = out[0], out[1]
         out3 = out[2] if p3 is not None else None
-        sizes = [(face, nq), (w, 3 * nq)] + ([(out3, 3 * nq)] if out3 is not None else [])
-        if any(int(np.prod(a.shape)) != n for a, n in sizes):
-            raise ValueError("out arrays must hold nq, 3 nq (and 3 nq) values")
-    pf, kf = _ptr(face, np.int32)
-    pw, kw = _ptr(w, np.float64)
-    po, ko = _ptr(out3, np.float64)
+    pf = _out_ptr(face, np.int32, nq)
+    pw = _out_ptr(w, np.float64, 3 * nq)
+    po = None if out3 is None else _out_ptr(out3, np.float64, 3 * nq)
     call("cvt_locate_points", uvp, p3p, nv, trp, nf, qp, nq, pf, pw, po)
     return (face, w) if p3 is None else (face, w, out3)
 

@@ -8,7 +8,7 @@ from pathlib import Path
 import numpy as np
 
 from ._lib import call, lib
-from .engine import _ptr
+from .engine import _out_ptr, _ptr
 
 IN_C = 32
 WEIGHTS = Path(__file__).resolve().parent.parent / "weights" / "warmstart_r256.bin"
@@ -46,8 +46,7 @@ class UNet:
 
     def params(self, out=None):
         out = np.empty(self.num_params, np.float32) if out is None else out
-        p, k = _ptr(out)
-        call("cvt_unet_get_params", self._h, p)
+        call("cvt_unet_get_params", self._h, _out_ptr(out, np.float32, self.num_params))
         return out
 
     def set_params(self, params):
@@ -56,8 +55,7 @@ class UNet:
 
     def grads(self, out=None):
         out = np.empty(self.num_params, np.float32) if out is None else out
-        p, k = _ptr(out)
-        call("cvt_unet_get_grads", self._h, p)
+        call("cvt_unet_get_grads", self._h, _out_ptr(out, np.float32, self.num_params))
         return out
 
     def prepare(self, slot, density, xy, dst=None):
@@ -65,7 +63,8 @@ class UNet:
         n = int(density.shape[0])
         S = int(xy.shape[0]) if not hasattr(xy, "numel") else xy.numel() // 2
         x, k2 = _ptr(xy, np.float32)
-        o, k3 = _ptr(dst)
+        o = None if dst is None else _out_ptr(dst, np.uint16, self.R * self.R * 32,
+                                                ('bfloat16', 'int16'))
         call("cvt_unet_prepare", self._h, int(slot), d, n, x, S, o)
 
     def set_input(self, x, nb):
@@ -77,8 +76,8 @@ class UNet:
 
     def field(self, nb, out=None):
         out = np.empty((nb, self.R, self.R, 2), np.float32) if out is None else out
-        p, k = _ptr(out)
-        call("cvt_unet_get_field", self._h, p, int(nb))
+        call("cvt_unet_get_field", self._h, _out_ptr(out, np.float32, nb * self.R * self.R * 2),
+             int(nb))
         return out
 
     def loss_backward(self, nb, uv, target, S, sync=True):

@@ -12,7 +12,7 @@ from __future__ import annotations
 import numpy as np
 
 from ._lib import call
-from .engine import _ptr
+from .engine import _out_ptr, _ptr
 
 
 def vertex_weights(uv, p3, tri, out=None):
@@ -35,8 +35,5 @@ def vertex_weights(uv, p3, tri, out=None):
     nf = int(np.prod(k2.shape)) // 3
     if out is None:
         out = np.empty(nv, np.float64)
-    if int(np.prod(out.shape)) != nv:
-        raise ValueError("out must hold one value per vertex")
-    po, ko = _ptr(out, np.float64)
-    call("cvt_vertex_weights", uvp, p3p, nv, trp, nf, po)
+    call("cvt_vertex_weights", uvp, p3p, nv, trp, nf, _out_ptr(out, np.float64, nv))
     return out

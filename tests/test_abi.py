@@ -122,3 +122,77 @@ def test_c_host_compiles_links_and_runs(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.startswith("version ")
+
+
+# ------------------------------------------------- reference site initialiser --
+def test_ref_init_sites_kat_and_oracle(orc):
+    """cvt_ref_init_sites runs on the host: the zero-state KATs (first site
+    (0, 437) at 512 and (0, 1748) at 2048 for a uniform density) and the
+    oracle's putConstrains + randomPoints bit for bit, with constraints, a
+    masked-out disk and a continued generator state."""
+    from cvtamd import compat
+    from cvtamd import workloads as wl
+    for n, first in ((512, (0, 437)), (2048, (0, 1748))):
+        m, d = compat.ref_init_sites(np.ones((n, n), np.float32), 1)
+        ys, xs = np.nonzero(m[:, :, 0] != -32768)
+        assert list(zip(xs.tolist(), ys.tolist())) == [first] and d == 3
+        assert m[first[1], first[0]].tolist() == list(first)
+    for n, num, seed in ((64, 40, 1), (300, 900, 2), (512, 1000, 3)):
+        g = wl.rng(seed)
+        rho = wl.gaussian_mixture(n, g, 5, 0.05, wl.disk_mask(n))
+        mask = np.zeros((n, n), np.uint8)
+        ys, xs = np.nonzero(rho > 0)
+        pick = g.integers(0, len(xs), 7)
+        mask[ys[pick], xs[pick]] = 1
+        ref = orc.ref_put_constraints(mask)
+        kiss = orc.kiss(0)
+        orc.ref_random_points(ref, rho, num, kiss)
+        st = np.zeros(4, np.uint32)
+        got, _ = compat.ref_init_sites(rho, num, mask, rng_state=st)
+        assert np.array_equal(got, ref)
+        assert st.tolist() == [kiss.z, kiss.w, kiss.jsr, kiss.jcong]
+        # a second call continues the sequence, as the reference's globals do
+        orc.ref_random_points(ref, rho, 10, kiss)
+        again, _ = compat.ref_init_sites(rho, num + 10, mask)
+        assert np.array_equal(again, ref)
+
+
+def test_ref_init_sites_refuses_endless_loops():
+    import cvtamd
+    from cvtamd import compat
+    n = 16
+    rho = np.zeros((n, n), np.float32)
+    rho[3, 4] = 1.0
+    rho[5, 6] = 2.0
+    m, _ = compat.ref_init_sites(rho, 2)
+    assert sorted(zip(*np.nonzero(m[:, :, 0] != -32768))) == [(3, 4), (5, 6)]
+    with pytest.raises(cvtamd.CvtError):          # three sites, two pixels
+        compat.ref_init_sites(rho, 3)
+    mask = np.zeros((n, n), np.uint8)
+    mask[3, 4] = 1
+    with pytest.raises(cvtamd.CvtError):          # the masked pixel is taken
+        compat.ref_init_sites(rho, 2, mask)
+    with pytest.raises(cvtamd.CvtError):
+        compat.ref_init_sites(np.full((n, n), np.nan, np.float32), 1)
+    m, d = compat.ref_init_sites(np.zeros((n, n), np.float32), 0)
+    assert (m == -32768).all() and d == 0
+
+
+def test_cpp_caller_builds_against_shim_header(tmp_path):
+    """A C++ host that includes include/gcvt_compat.h (the reference's two
+    signatures, C++ linkage) and include/cvt.h compiles with g++, links the
+    mangled gCVT and discretization_d from libcvt_hip.so, and runs the host
+    part of centroidalVoronoi (the site initialiser) without a GPU."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not found")
+    lib = ROOT / "surface-remesher_amd" / "lib"
+    exe = tmp_path / "gcvt_caller"
+    subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", str(ROOT / "include"),
+                    str(ROOT / "tests" / "c" / "gcvt_caller.cpp"), "-L", str(lib), "-lcvt_hip",
+                    f"-Wl,-rpath,{lib}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe), "host"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "first site 0 437" in out.stdout

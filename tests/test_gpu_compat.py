@@ -143,28 +143,69 @@ def test_gcvt_close_to_exact_voronoi_semantics(gpu, orc):
     assert abs(ns - ns_exact) <= 0.02 * ns_exact
 
 
-def test_gcvt_multires_depth2(gpu, orc):
+def coarse_start(orc, rho, depth, num, seed):
+    """A full-size input map whose first nc^2 pairs hold an nc x nc map of
+    reference-initialised sites (nc the coarsest side): the layout gCVT reads
+    for its coarsest level."""
+    n = rho.shape[0]
+    L = orc.compat_levels(n, depth)
+    nc = n >> (L - 1)
+    f = n // nc
+    coarse = rho.reshape(nc, f, nc, f).mean(axis=(1, 3)).astype(np.float32)
+    cvor = reference_init(orc, coarse, np.zeros((nc, nc), np.uint8), num)
+    vor = np.full((n, n, 2), -32768, np.int16)
+    vor.reshape(-1, 2)[: nc * nc] = cvor.reshape(-1, 2)
+    return vor
+
+
+@pytest.mark.parametrize("n,depth,num,seed", [(512, 2, 300, 4), (1024, 3, 500, 5), (512, 4, 200, 6)])
+def test_gcvt_multires_matches_oracle(gpu, orc, n, depth, num, seed):
     from cvtamd import compat
     from cvtamd import workloads as wl
+    g = wl.rng(seed)
+    rho = wl.gaussian_mixture(n, g, 6, 0.05, wl.disk_mask(n))
+    vor = coarse_start(orc, rho, depth, num, seed)
+    # fixed pixels: every level reads this mask with its own row stride
+    mask = (g.random((n, n)) < 0.002).astype(np.uint8)
+    ref = vor.copy()
+    it_r, elog, ns_r = orc.compat_gcvt(rho, ref, mask, depth, 300)
+    got = vor.copy()
+    it, ns = compat.gcvt(got, rho, mask, depth, 300)
+    print(f"n={n} depth={depth}: {it} iterations, {ns} sites (oracle {it_r}, {ns_r})")
+    assert (it, ns) == (it_r, ns_r)
+    assert np.array_equal(got, ref)
+
+
+def test_gcvt_schedule_hand_traced(gpu):
+    # one site at (8, 4) on a 9 x 9 grid with rho = 1/64 on columns 0..2: 30
+    # iterations, omega = 1.25 after the second decade brings it to (1, 4)
+    # (the derivation is in tests/test_oracle.py)
+    from cvtamd import compat
+    n = 9
+    rho = np.zeros((n, n), np.float32)
+    rho[:, :3] = 1.0 / 64.0
+    vor = np.full((n, n, 2), -32768, np.int16)
+    vor[4, 8] = (8, 4)
+    it, ns = compat.gcvt(vor, rho, None, 1, 1000)
+    assert (it, ns) == (30, 1)
+    assert (vor[:, :, 0] == 1).all() and (vor[:, :, 1] == 4).all()
+
+
+def test_gcvt_multires_carries_schedule(gpu):
+    # 512^2, depth 2: a coarse site at (127, 127); omega = 1 and lastE carried
+    # from the coarse level give 30 iterations and the site (256, 256); with
+    # maxIter 15 the fine level still runs one iteration: 16, site (255, 255)
+    from cvtamd import compat
     n = 512
-    rho = wl.gaussian_mixture(n, wl.rng(4), 6, 0.05, wl.disk_mask(n))
-    mask = np.zeros((n, n), np.uint8)
-    vor = reference_init(orc, rho, mask, 800)
-    it, ns = compat.gcvt(vor, rho, mask, 2, 300)
-    assert 1 <= it <= 300 and 0 < ns <= 800
-    xy, _ = orc.compat_decode(vor)
-    assert len(xy) == ns
-    px = np.floor(xy).astype(int)
-    assert (rho[px[:, 1], px[:, 0]] > 0).all()
-    # the map labels every pixel with a site at the JFA(+1) distance of the
-    # decoded sites (labels may differ only on equidistant pixels: the
-    # engine's site order is its merge order, not row-major)
-    lab = orc.jfa(n, n, xy)
-    cx, cy = np.floor(xy[lab, 0]), np.floor(xy[lab, 1])
-    px, py = np.arange(n)[None, :], np.arange(n)[:, None]
-    d_ref = (cx - px) ** 2 + (cy - py) ** 2
-    d_got = (vor[:, :, 0] - px) ** 2 + (vor[:, :, 1] - py) ** 2
-    assert np.mean(d_got != d_ref) < 1e-3
+    rho = np.ones((n, n), np.float32)
+    vor = np.full((n, n, 2), -32768, np.int16)
+    vor.reshape(-1, 2)[127 * 256 + 127] = (127, 127)
+    a = vor.copy()
+    assert compat.gcvt(a, rho, None, 2, 1000) == (30, 1)
+    assert (a == 256).all()
+    b = vor.copy()
+    assert compat.gcvt(b, rho, None, 2, 15) == (16, 1)
+    assert (b == 255).all()
 
 
 def test_gcvt_shim_and_max_iter(gpu, orc):

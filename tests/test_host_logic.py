@@ -80,3 +80,52 @@ def test_pointer_helper_checks_tensor_dtypes():
     with pytest.raises(ValueError):
         _ptr(torch.zeros((4, 4), dtype=torch.float64).t(), np.float64)
     assert _ptr(None, np.float64) == (None, None)
+
+
+def test_out_pointer_never_copies():
+    """Outputs are written in place: an out array of the wrong dtype, not
+    C-contiguous, read-only or of the wrong size is refused (a converted copy
+    would take the results and leave the caller's array unchanged)."""
+    import torch
+    from cvtamd.engine import _out_ptr
+    a = np.zeros((3, 4), np.float32)
+    assert _out_ptr(a, np.float32, 12).value == a.ctypes.data
+    with pytest.raises(TypeError):
+        _out_ptr(np.zeros(12, np.float64), np.float32, 12)
+    with pytest.raises(ValueError):
+        _out_ptr(np.zeros((4, 6), np.float32)[:, ::2], np.float32, 12)
+    with pytest.raises(ValueError):
+        _out_ptr(np.zeros(11, np.float32), np.float32, 12)
+    ro = np.zeros(12, np.float32)
+    ro.setflags(write=False)
+    with pytest.raises(ValueError):
+        _out_ptr(ro, np.float32, 12)
+    with pytest.raises(TypeError):
+        _out_ptr([0.0] * 12, np.float32, 12)
+    t = torch.zeros(12, dtype=torch.int32)
+    assert _out_ptr(t, np.int32, 12).value == t.data_ptr()
+    with pytest.raises(TypeError):
+        _out_ptr(torch.zeros(12, dtype=torch.int64), np.int32, 12)
+    with pytest.raises(ValueError):
+        _out_ptr(torch.zeros((4, 4), dtype=torch.int32).t(), np.int32, 16)
+    with pytest.raises(ValueError):
+        _out_ptr(torch.zeros(13, dtype=torch.int32), np.int32, 12)
+    b = torch.zeros(8, dtype=torch.bfloat16)
+    assert _out_ptr(b, np.uint16, 8, ("bfloat16",)).value == b.data_ptr()
+
+
+def test_wrappers_refuse_bad_out_arrays():
+    """The binding's out= arguments go through the strict check before any
+    library call (no GPU needed to reach it)."""
+    from cvtamd import compat, weighting
+    from cvtamd.recover import locate_points
+    pts = np.array([[0, 0], [1, 0], [0, 1]], np.float64)
+    tri = np.array([[0, 1, 2]], np.int32)
+    with pytest.raises(TypeError):
+        compat.rasterize(pts, np.ones(3), tri, 0.1, 8, out=np.zeros((8, 8), np.float64))
+    with pytest.raises(ValueError):
+        compat.rasterize(pts, np.ones(3), tri, 0.1, 8, out=np.zeros((8, 7), np.float32))
+    with pytest.raises(ValueError):
+        weighting.vertex_weights(pts, np.zeros((3, 3)), tri, out=np.zeros(6)[::2])
+    with pytest.raises(TypeError):
+        locate_points(pts, tri, pts, out=(np.zeros(3, np.int64), np.zeros((3, 3))))

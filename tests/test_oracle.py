@@ -370,6 +370,130 @@ def test_compat_lloyd_invariants(orc):
     assert (vor[:, :, 0] >= 0).all()
 
 
+def sites_map(n, pts, stride=None):
+    """A reference input map: MARKER everywhere, (x, y) at each site; with
+    stride m < n the sites are written as an m x m map into the first m^2
+    pairs (the layout gCVT reads for its coarsest level)."""
+    vor = np.full((n, n, 2), -32768, np.int16)
+    flat = vor.reshape(-1, 2)
+    m = stride or n
+    for x, y in pts:
+        flat[y * m + x] = (x, y)
+    return vor
+
+
+def test_compat_levels(orc):
+    # source/gcvt.cu:1091: levels stop before a side below 256
+    assert orc.compat_levels(512, 1) == 1
+    assert orc.compat_levels(512, 2) == 2
+    assert orc.compat_levels(512, 3) == 2
+    assert orc.compat_levels(1024, 3) == 3
+    assert orc.compat_levels(2048, 9) == 4
+    assert orc.compat_levels(256, 2) == 1
+    assert orc.compat_levels(200, 3) == 1
+
+
+def test_compat_lattice_fixed_point_stops_after_20(orc):
+    # 9 x 9 blocks of 7 x 7 pixels, a site at each block centre: every cell is
+    # its block and every centroid its site, so nothing moves; E(0) = E(10) =
+    # a^2 (a^2 - 1) / 6 per block * 81 / 63^2 = (a^2 - 1) / 6 = 8 exactly.
+    # Check 10: dE = 1e18 - 8 (go on); check 20: dE = 0 < 1e-4: stop.
+    n, a = 63, 7
+    pts = [(a * i + 3, a * j + 3) for j in range(9) for i in range(9)]
+    rho = np.ones((n, n), np.float32)
+    vor = sites_map(n, pts)
+    it, elog, S = orc.compat_gcvt(rho, vor, None, 1, 1000)
+    assert it == 20 and S == 81
+    assert np.allclose(elog, [8.0, 8.0], rtol=1e-12)
+    xy, _ = orc.compat_decode(vor)
+    assert sorted(map(tuple, np.floor(xy).astype(int).tolist())) == sorted(pts)
+    bx = np.arange(n) // a * a + 3
+    assert (vor[:, :, 0] == bx[None, :]).all() and (vor[:, :, 1] == bx[:, None]).all()
+
+
+def test_compat_omega_schedule_hand_traced(orc):
+    # 9 x 9 grid, rho = 1/64 on columns 0..2, one site at (8, 4): its cell is
+    # the grid, the centroid is (1, 4).  omega = 2 reflects x about 1:
+    # 8 -> trunc(-5.5) = -5 -> clamp 0, then 0 -> 2 -> 0 -> 2 ...  Samples
+    # (x at iterations 0, 10, 20 = 8, 2, 2):
+    #   E(x) = rho (9 sum_c (x - c)^2 + 3 * 60) / 81, c = 0..2
+    #   E(8) = 1521/5184, E(2) = 225/5184.
+    # Check 10: omega = min(2, 1 + 1e18) = 2.  Check 20: dE = 1296/5184 =
+    # 0.25, omega = 1.25, go on.  Iteration 20: 2 + 1.25 (1 - 2) + .5 = 1.25
+    # -> 1, the centroid: fixed.  Check 30: dE = E(2) - E(2) = 0: stop.
+    # (With omega still 2 the site would end at 2; an energy sampled after
+    # the update, or a reversed dE, would change the count or the samples.)
+    n = 9
+    rho = np.zeros((n, n), np.float32)
+    rho[:, :3] = 1.0 / 64.0
+    vor = sites_map(n, [(8, 4)])
+    it, elog, S = orc.compat_gcvt(rho, vor, None, 1, 1000)
+    assert it == 30 and S == 1
+    assert np.allclose(elog, np.array([1521, 225, 225]) / 5184.0, rtol=1e-12, atol=0)
+    assert (vor[:, :, 0] == 1).all() and (vor[:, :, 1] == 4).all()
+
+
+def _sq_sum(c, n):
+    return float(sum((c - x) ** 2 for x in range(n)))
+
+
+def test_compat_multires_carries_omega_and_last_energy(orc):
+    # 512^2, depth 2, rho = 1: one site at coarse (127, 127) in the 256 x 256
+    # map read from the first 256^2 pairs.  Coarse: centroid 127.5, omega = 2
+    # swaps 127 <-> 128, so E(0) = E(10) (= 4 * 2 * 256 * S1 / 256^2, S1 =
+    # sum (127 - x)^2 over 0..255); check 20: dE = 0 < 0.3 stops the level
+    # with omega = min(2, 1 + 0) = 1 and lastE = E(0) kept.  Zoom: (254, 254).
+    # Fine, omega = 1: 254 + 1.5 + .5 -> 256, the centroid's pixel (255.5 -
+    # 256 = -0.5 keeps it).  Check 30: dE = E(0) - E_fine(20) < 0: stop.
+    # Omega reset to 2 would leave the site swapping 254 <-> 257; lastE reset
+    # to 1e18 would run past 30.
+    n = 512
+    rho = np.ones((n, n), np.float32)
+    vor = sites_map(n, [(127, 127)], stride=256)
+    it, elog, S = orc.compat_gcvt(rho, vor, None, 2, 1000)
+    ec = 4 * 2 * 256 * _sq_sum(127, 256) / 256 ** 2
+    ef = 2 * 512 * _sq_sum(254, 512) / 512 ** 2
+    assert it == 30 and S == 1
+    assert np.allclose(elog, [ec, ec, ef], rtol=1e-12)
+    assert ef > ec
+    assert (vor == 256).all()
+
+
+def test_compat_multires_runs_one_iteration_per_level(orc):
+    # the same start with maxIter 15: the coarse level runs iterations 0..14
+    # (check 10 goes on), the fine level's do-while still runs one: 16 in
+    # all.  Coarse iteration 14 leaves the site at 128 -> 256; the fine step
+    # with omega = 2: 256 + 2 (-0.5) + .5 = 255.5 -> 255.
+    n = 512
+    rho = np.ones((n, n), np.float32)
+    vor = sites_map(n, [(127, 127)], stride=256)
+    it, elog, S = orc.compat_gcvt(rho, vor, None, 2, 15)
+    assert it == 16 and S == 1
+    assert (vor == 255).all()
+
+
+def test_compat_multires_coarse_mask_is_strided(orc):
+    # a coarse level reads mask with its own row stride: a site at coarse
+    # (3, 1) is fixed by mask byte 1 * 256 + 3, which is fine pixel (259, 0)
+    n = 512
+    rho = np.ones((n, n), np.float32)
+    mask = np.zeros((n, n), np.uint8)
+    mask.reshape(-1)[1 * 256 + 3] = 1
+    vor = sites_map(n, [(3, 1)], stride=256)
+    it, elog, S = orc.compat_gcvt(rho, vor, mask, 2, 5)
+    # five coarse iterations with the site pinned, then one fine iteration
+    # from (6, 2), which is not masked at the fine level: omega = 2 moves it
+    assert it == 6 and S == 1
+    xy, _ = orc.compat_decode(vor)
+    fx, fy = np.floor(xy[0]).astype(int)
+    # fine step from (6, 2): trunc(6 + 2 (255.5 - 6) + .5) = 505 -> (505, 509)
+    assert (fx, fy) == (505, 509)
+    # without the mask the coarse level would move the site
+    vor2 = sites_map(n, [(3, 1)], stride=256)
+    orc.compat_gcvt(rho, vor2, None, 2, 5)
+    assert not (vor2 == vor).all()
+
+
 def test_extract_points_order_and_mask(orc):
     n = 4
     vor = np.full((n, n, 2), -32768, np.int16)
