@@ -105,6 +105,21 @@ int cvt_moments(cvt_ctx *ctx, double *moments);
  * then s <- s + omega (c - s) for every non-fixed site with W > 0.  The
  * per-iteration energy and gradient norm are recorded (cvt_get_history). */
 int cvt_lloyd(cvt_ctx *ctx, int iterations, float omega);
+/* Lloyd steps as cvt_lloyd, at most max_iter, that stop themselves on the
+ * device: after the first iteration whose energy E is <= target, or whose
+ * energy fell by less than rel_tol * E over the last `window` iterations of
+ * this call (window 0: no such test; target -inf: no target).  The update's
+ * last block tests the rule on the device history and raises a stop flag
+ * that makes every kernel of the later iterations return at entry.  The
+ * host enqueues chunks of 8 iterations, each followed by a copy of the flag,
+ * and waits only for the chunk before the one still queued, so the device
+ * never idles and at most two chunks run past the stop (as no-ops).
+ * *iterations (optional) receives the number of iterations run; the history
+ * holds their energies as with cvt_lloyd.  The reference tests its stop rule
+ * on the host every 10 iterations after a blocking copy of the energy
+ * (source/gcvt.cu:1123-1140). */
+int cvt_lloyd_until(cvt_ctx *ctx, float omega, double rel_tol, int window, double target,
+                    int max_iter, int *iterations);
 /* Number of recorded iterations (>= 0) or a negative status. */
 int cvt_history_length(cvt_ctx *ctx);
 /* The first n recorded (energy, gradient norm) pairs, 2n doubles.  Energy
@@ -119,10 +134,11 @@ int cvt_energy(cvt_ctx *ctx, double *energy, double *gradnorm);
 int cvt_synchronize(cvt_ctx *ctx);
 /* Runs `iterations` (1..1000) Lloyd steps eagerly with HIP events around
  * each phase and returns mean durations in ms: ms[0] one dense JFA pass (the
- * back-to-back sequence of passes 1 .. P-1 over P - 1), ms[1] the moment
- * kernel, ms[2] the first JFA pass (from the seed map), ms[3] update +
- * finalize, ms[4] 0, ms[5] the whole step, ms[6] the JFA pass count P, ms[7]
- * the moment tile.
+ * back-to-back sequence of passes 1 .. P-1 over P - 1; with the persistent
+ * JFA, its single launch over P - 1), ms[1] the moment kernel, ms[2] the
+ * first JFA pass (from the seed map; 0 with the persistent JFA, which holds
+ * it), ms[3] update + finalize, ms[4] the whole JFA, ms[5] the whole step,
+ * ms[6] the JFA pass count P, ms[7] the moment tile.
  * (No reference counterpart: the reference times stages with clock(),
  * source/main.cpp:181-242.) */
 int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms);

@@ -17,7 +17,8 @@
  *     has its centre at (x + 0.5, y + 0.5);
  *   - labels are int32 site indices, ORC_INVALID = 0x7fffffff = no site;
  *   - the nearest-site order is (d, index) lexicographic, with
- *     d = fmaf(dx, dx, dy*dy), dx = sx - (x + 0.5f), dy = sy - (y + 0.5f).
+ *     d = fmaf(dy, dy, dx*dx), dx = sx - (x + 0.5f), dy = sy - (y + 0.5f)
+ *     (dx*dx is rounded first, then dy*dy + dx*dx is rounded once).
  *
  * Pinning of each function (what checks it in tests/test_oracle.py):
  *   orc_jfa            -- bit-compared with orc_voronoi_brute on small grids
@@ -74,7 +75,7 @@ static inline float orc_dist(float sx, float sy, int x, int y)
 {
     float dx = sx - ((float)x + 0.5f);
     float dy = sy - ((float)y + 0.5f);
-    return fmaf(dx, dx, dy * dy);
+    return fmaf(dy, dy, dx * dx);
 }
 
 /* (d, l) < (bd, bl) lexicographically */

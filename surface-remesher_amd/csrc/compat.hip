@@ -268,7 +268,7 @@ extern "C" int cvt_compat_gcvt(int16_t *voronoi, const float *density, const uin
         if (rc) goto done;
         CCHK(hipMalloc(&dcoords, sizeof(short2) * np), "alloc");
         hipLaunchKernelGGL(k_labels_to_coords, dim3(1024), dim3(256), 0, c->stream,
-                           c->lab[c->final_buf], c->sites, c->none(), np, dcoords);
+                           c->labels_map(), c->sites, c->none(), np, dcoords);
         CCHK(hipMemcpyAsync(voronoi, dcoords, sizeof(short2) * np, hipMemcpyDefault, c->stream),
              "download");
         CCHK(hipStreamSynchronize(c->stream), "final pass");

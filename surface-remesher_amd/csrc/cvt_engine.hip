@@ -61,7 +61,7 @@ static int choose_tile(int W, int H, int S)
     return t;
 }
 
-int Ctx::enqueue_moments() { return launch_moments_on(this, lab[final_buf]); }
+int Ctx::enqueue_moments() { return launch_moments_on(this, labels_map()); }
 
 static int launch_moments_on(Ctx *c, const uint32_t *lab)
 {
@@ -120,6 +120,13 @@ int Ctx::check_flags()
     if (hipMemcpyAsync(f, dflags, sizeof f, hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
         return fail(CVT_ERR_HIP, "flag readback: %s", hipGetErrorString(hipGetLastError()));
+    if (f[FLAG_FLOW]) {
+        int z = 0;
+        hipMemcpyAsync(dflags + FLAG_FLOW, &z, sizeof z, hipMemcpyHostToDevice, stream);
+        hipStreamSynchronize(stream);
+        return fail(CVT_ERR_HIP, "a wait of the persistent JFA ran out of spins; its labels are "
+                                 "not valid");
+    }
     if (f[FLAG_FULL]) {
         int z = 0;
         hipMemcpyAsync(dflags + FLAG_FULL, &z, sizeof z, hipMemcpyHostToDevice, stream);
@@ -154,6 +161,29 @@ int Ctx::enqueue_jfa(hipEvent_t after_first)
 {
     const int np = jfa_num_passes(W, H);
     int step = jfa_first_step(W, H);
+    if (flow) {
+        // the whole JFA in one persistent launch
+        FlowArgs a;
+        a.maps = maps;
+        a.seed = seed;
+        a.sites = sites;
+        a.flags = flow_flags;
+        a.ctl = flow_ctl;
+        a.dflags = dflags;
+        a.stop = dstate + ST_STOP;
+        a.W = W;
+        a.H = H;
+        a.S = S;
+        a.np = np;
+        a.step0 = step;
+        a.table_f4 = jplan.table_f4;
+        a.none = none();
+        flow_phases(a);
+        launch_jfa_flow(stream, jplan, a);
+        if (after_first) hipEventRecord(after_first, stream);
+        seeds_valid = false;
+        return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "jfa launch failed");
+    }
     const uint32_t *in = seed;
     int cur = 0;
     // Pass 0 reads only the seed map: with three passes or more it scatters
@@ -166,7 +196,7 @@ int Ctx::enqueue_jfa(hipEvent_t after_first)
     const int step0 = step;
     for (int k = 0; k < np; ++k) {
         if (k == 0 && scatter) {
-            launch_jfa_seed_pass(stream, seed, first, sites, S, W, H, step);
+            launch_jfa_seed_pass(stream, seed, first, sites, S, W, H, step, dstate + ST_STOP);
             if (after_first) hipEventRecord(after_first, stream);
             in = first;
             if (step > 1) step >>= 1;
@@ -175,7 +205,7 @@ int Ctx::enqueue_jfa(hipEvent_t after_first)
         uint32_t *out = lab[cur];
         uint32_t *clear = (k == 1) ? seed : (k == 2 && scatter) ? first : nullptr;
         launch_jfa_pass(stream, jplan, in, out, sites, W, H, step, clear, none(),
-                        clear ? S : 0, k == 2 ? step0 : 0);
+                        clear ? S : 0, k == 2 ? step0 : 0, dstate + ST_STOP);
         if (k == 0 && after_first) hipEventRecord(after_first, stream);
         in = out;
         cur ^= 1;
@@ -198,6 +228,7 @@ int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, doub
     fa.e_scale = ldexp(1.0, rho_exp) / a;
     fa.g_scale = 2.0 * ldexp(1.0, rho_exp) / a;
     fa.out_eg = out_eg;
+    fa.until = until;
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, (long)capS, ovf,
                        ovf_head, dflags, dstate, omega, domega, mode, ldexp(1.0, rho_exp), mom_out,
                        blockpart, seed, W, H, ca, fa);
@@ -238,6 +269,53 @@ int Ctx::build_graph(float omega)
     return CVT_OK;
 }
 
+int Ctx::lloyd_until(int max_iter, float omega, double rel_tol, int window, double target,
+                     int *ran_out)
+{
+    if (!have_density) return fail(CVT_ERR_STATE, "cvt_lloyd_until before cvt_set_density");
+    if (!until_host) {
+        if (hipHostMalloc((void **)&until_host, 2 * sizeof(int)) != hipSuccess ||
+            hipEventCreateWithFlags(&until_ev[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&until_ev[1], hipEventDisableTiming) != hipSuccess)
+            return fail(CVT_ERR_HIP, "stop-flag buffers");
+    }
+    hipLaunchKernelGGL(k_until, dim3(1), dim3(64), 0, stream, dstate, until, 1, rel_tol, target,
+                       window, ran);
+    // Chunks of iterations are enqueued with a copy of the stop flag behind
+    // each; before enqueuing chunk j + 1 the host waits for chunk j - 1 only
+    // (chunk j is still queued, so the device never idles) and stops
+    // enqueuing once a copy reads set.  Iterations enqueued after the stop
+    // return at entry.
+    constexpr int kChunk = 8;
+    int rc = CVT_OK, done = 0;
+    for (int j = 0; done < max_iter; ++j) {
+        const int n = std::min(kChunk, max_iter - done);
+        if ((rc = lloyd(n, omega))) break;
+        done += n;
+        if (hipMemcpyAsync(until_host + (j & 1), dstate + ST_STOP, sizeof(int),
+                           hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipEventRecord(until_ev[j & 1], stream) != hipSuccess) {
+            rc = fail(CVT_ERR_HIP, "stop-flag copy");
+            break;
+        }
+        if (j >= 1) {
+            if (hipEventSynchronize(until_ev[(j - 1) & 1]) != hipSuccess) {
+                rc = fail(CVT_ERR_HIP, "stop-flag wait");
+                break;
+            }
+            if (until_host[(j - 1) & 1]) break;
+        }
+    }
+    hipLaunchKernelGGL(k_until, dim3(1), dim3(64), 0, stream, dstate, until, 0, rel_tol, target,
+                       window, ran);
+    int n = 0;
+    if (hipMemcpyAsync(&n, ran, sizeof n, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return rc ? rc : fail(CVT_ERR_HIP, "iteration count readback");
+    if (ran_out) *ran_out = n;
+    return rc ? rc : check_flags();
+}
+
 int Ctx::lloyd(int iterations, float omega)
 {
     if (!have_density) return fail(CVT_ERR_STATE, "cvt_lloyd before cvt_set_density");
@@ -260,10 +338,16 @@ int Ctx::lloyd(int iterations, float omega)
 
 Ctx::~Ctx()
 {
+    if (until_host) {
+        hipHostFree(until_host);
+        hipEventDestroy(until_ev[0]);
+        hipEventDestroy(until_ev[1]);
+    }
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
     void *bufs[] = {rho, seed, lab[0], lab[1], first, sites, fixed, slots, ovf, ovf_head,
-                    blockpart, hist, dstate, dflags, scratch, eg};
+                    blockpart, hist, dstate, dflags, scratch, eg, maps, flow_flags, flow_ctl,
+                    until, ran};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (own_stream && own) hipStreamDestroy(own_stream);
@@ -299,6 +383,8 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     c->W = W; c->H = H; c->maxS = max_sites; c->S = 0;
     c->capS = std::max(max_sites, 1);
     c->jplan = plan_jfa(W, H, c->capS);
+    c->flow = flow_eligible(c->jplan, W, H);
+    c->npass = jfa_num_passes(W, H);
     hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -311,10 +397,16 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     c->hist_cap = 1 << 16;
     const int nb = (int)std::max(1L, (ms + 255) / 256);
     bool ok = hipMalloc(&c->rho, sizeof(float) * n) == hipSuccess &&
-              hipMalloc(&c->seed, sizeof(uint32_t) * n) == hipSuccess &&
-              hipMalloc(&c->lab[0], sizeof(uint32_t) * n) == hipSuccess &&
-              hipMalloc(&c->lab[1], sizeof(uint32_t) * n) == hipSuccess &&
-              hipMalloc(&c->first, sizeof(uint32_t) * n) == hipSuccess &&
+              hipMalloc(&c->seed, sizeof(uint32_t) * n) == hipSuccess;
+    if (ok && c->flow)
+        ok = hipMalloc(&c->maps, sizeof(uint32_t) * n * c->npass) == hipSuccess &&
+             hipMalloc(&c->flow_flags, sizeof(uint32_t) * (size_t)H * c->npass) == hipSuccess &&
+             hipMalloc(&c->flow_ctl, sizeof(uint32_t) * CTL_WORDS) == hipSuccess;
+    else if (ok)
+        ok = hipMalloc(&c->lab[0], sizeof(uint32_t) * n) == hipSuccess &&
+             hipMalloc(&c->lab[1], sizeof(uint32_t) * n) == hipSuccess &&
+             hipMalloc(&c->first, sizeof(uint32_t) * n) == hipSuccess;
+    ok = ok &&
               hipMalloc(&c->sites, sizeof(float2) * (c->capS + 2)) == hipSuccess &&
               hipMalloc(&c->fixed, ms) == hipSuccess &&
               hipMalloc(&c->slots, sizeof(Slot) * 9 * ms) == hipSuccess &&
@@ -324,16 +416,29 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
               hipMalloc(&c->dstate, sizeof(int) * NUM_STATE) == hipSuccess &&
               hipMalloc(&c->dflags, sizeof(int) * NUM_FLAGS) == hipSuccess &&
               hipMalloc(&c->scratch, sizeof(float) * 4096) == hipSuccess &&
-              hipMalloc(&c->eg, sizeof(double) * 4) == hipSuccess;
+              hipMalloc(&c->eg, sizeof(double) * 4) == hipSuccess &&
+              hipMalloc(&c->until, sizeof(double) * 2) == hipSuccess &&
+              hipMalloc(&c->ran, sizeof(int)) == hipSuccess;
     if (!ok) {
         delete c;
         return fail(CVT_ERR_NOMEM, "device allocation for a %dx%d grid, %d sites", W, H, max_sites);
     }
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->seed, c->none(), n);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[0], c->none(), n);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[1], c->none(), n);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->first, c->none(), n);
+    if (c->flow) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->maps, c->none(),
+                           n * c->npass);
+        // flags 0 and epoch 1: no row is complete before the first launch
+        hipMemsetAsync(c->flow_flags, 0, sizeof(uint32_t) * (size_t)H * c->npass, s);
+        uint32_t ctl[CTL_WORDS] = {0};
+        ctl[CTL_EPOCH] = 1;
+        hipMemcpyAsync(c->flow_ctl, ctl, sizeof ctl, hipMemcpyHostToDevice, s);
+        hipStreamSynchronize(s);
+    } else {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[0], c->none(), n);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[1], c->none(), n);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->first, c->none(), n);
+    }
     hipMemsetAsync(c->rho, 0, sizeof(float) * n, s);
     hipMemsetAsync(c->fixed, 0, ms, s);
     hipMemsetAsync(c->slots, 0, sizeof(Slot) * 9 * ms, s);
@@ -497,8 +602,8 @@ int cvt_get_labels(cvt_ctx *ctx, int32_t *labels)
     if (!labels) return fail(CVT_ERR_ARG, "null output");
     if (!c->labels_fresh) return fail(CVT_ERR_STATE, "no label map yet (call cvt_voronoi)");
     const long n = (long)c->W * c->H;
-    int32_t *tmp = reinterpret_cast<int32_t *>(c->lab[c->final_buf ^ 1]);
-    hipLaunchKernelGGL(k_export_labels, dim3(1024), dim3(256), 0, c->stream, c->lab[c->final_buf],
+    int32_t *tmp = reinterpret_cast<int32_t *>(c->spare_map());
+    hipLaunchKernelGGL(k_export_labels, dim3(1024), dim3(256), 0, c->stream, c->labels_map(),
                        tmp, n, c->none());
     HIPCHK(hipMemcpyAsync(labels, tmp, sizeof(int32_t) * n, hipMemcpyDefault, c->stream),
            "label download");
@@ -515,7 +620,7 @@ int cvt_moments(cvt_ctx *ctx, double *m)
     if (!m) return fail(CVT_ERR_ARG, "null output");
     double *dm = nullptr;
     HIPCHK(hipMalloc(&dm, sizeof(double) * 4 * std::max(c->S, 1)), "moment buffer");
-    int rc = launch_moments_on(c, c->lab[c->final_buf]);
+    int rc = launch_moments_on(c, c->labels_map());
     if (!rc) rc = c->enqueue_update(0, 1.0f, dm, 0, nullptr);
     if (!rc && c->S > 0 &&
         hipMemcpyAsync(m, dm, sizeof(double) * 4 * c->S, hipMemcpyDefault, c->stream) != hipSuccess)
@@ -533,6 +638,20 @@ int cvt_lloyd(cvt_ctx *ctx, int iterations, float omega)
     if (iterations < 0) return fail(CVT_ERR_ARG, "iterations %d", iterations);
     if (!(omega > 0.0f && omega < 2.0f + 1e-6f)) return fail(CVT_ERR_ARG, "omega %g", omega);
     return c->lloyd(iterations, omega);
+}
+
+int cvt_lloyd_until(cvt_ctx *ctx, float omega, double rel_tol, int window, double target,
+                    int max_iter, int *iterations)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (max_iter < 0) return fail(CVT_ERR_ARG, "max_iter %d", max_iter);
+    if (window < 0 || window >= c->hist_cap) return fail(CVT_ERR_ARG, "window %d", window);
+    if (!(omega > 0.0f && omega < 2.0f + 1e-6f)) return fail(CVT_ERR_ARG, "omega %g", omega);
+    if (!(rel_tol >= 0.0) || target != target)
+        return fail(CVT_ERR_ARG, "rel_tol %g, target %g", rel_tol, target);
+    if (iterations) *iterations = 0;
+    return c->lloyd_until(max_iter, omega, rel_tol, window, target, iterations);
 }
 
 int cvt_history_length(cvt_ctx *ctx)
@@ -576,11 +695,11 @@ int cvt_energy(cvt_ctx *ctx, double *energy, double *gradnorm)
     if (!c->have_density) return fail(CVT_ERR_STATE, "cvt_energy before cvt_set_density");
     int rc = cvt_voronoi(ctx);
     if (rc) return rc;
-    if ((rc = launch_moments_on(c, c->lab[c->final_buf]))) return rc;
+    if ((rc = launch_moments_on(c, c->labels_map()))) return rc;
     if ((rc = c->enqueue_update(0, 1.0f, nullptr, 0, c->eg))) return rc;
     const long n = (long)c->W * c->H;
     const int nb = 1024;
-    hipLaunchKernelGGL(k_energy64, dim3(nb), dim3(256), 0, c->stream, c->lab[c->final_buf], c->rho,
+    hipLaunchKernelGGL(k_energy64, dim3(nb), dim3(256), 0, c->stream, c->labels_map(), c->rho,
                        c->sites, c->S, c->W, c->H, c->blockpart);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->blockpart, nb, c->dstate,
                        c->dflags, c->hist, c->hist_cap, 0, 1.0 / (double)n, 0.0, c->eg + 2);
@@ -634,11 +753,17 @@ int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms)
     for (auto &e : ev) hipEventDestroy(e);
     if (rc) return rc;
     c->labels_fresh = true;
-    ms[0] = np > 1 ? acc[1] / ((double)iterations * (np - 1)) : acc[0] / iterations;
+    if (c->flow) {
+        // one launch: its time per dense pass, and no separate first pass
+        ms[0] = (acc[0] + acc[1]) / ((double)iterations * (np - 1));
+        ms[2] = 0.0;
+    } else {
+        ms[0] = np > 1 ? acc[1] / ((double)iterations * (np - 1)) : acc[0] / iterations;
+        ms[2] = np > 1 ? acc[0] / iterations : 0.0;
+    }
     ms[1] = acc[2] / iterations;
-    ms[2] = np > 1 ? acc[0] / iterations : 0.0;
     ms[3] = acc[3] / iterations;
-    ms[4] = 0.0;
+    ms[4] = (acc[0] + acc[1]) / iterations;
     ms[5] = acc[4] / iterations;
     ms[6] = np;
     ms[7] = c->tile;

@@ -45,7 +45,7 @@ __device__ __forceinline__ float cvt_dist(float2 s, float px, float py)
 {
     const float dx = s.x - px;
     const float dy = s.y - py;
-    return __builtin_fmaf(dx, dx, dy * dy);
+    return __builtin_fmaf(dy, dy, dx * dx);
 }
 
 // ---------------------------------------------------------------------------
@@ -430,6 +430,7 @@ static __global__ __launch_bounds__(256) void k_moments(
     int *__restrict__ ovf_head, int *__restrict__ dflags, int *__restrict__ dstate)
 {
     __shared__ TileLds<TileHash<T>::HC> L;
+    if (dstate[ST_STOP]) return;   // a stop rule held (cvt_lloyd_until)
     const int tile = blockIdx.x;
     const int tX = tile % tilesX, tY = tile / tilesX;
     tile_lds_init(L);
@@ -517,6 +518,7 @@ static __global__ __launch_bounds__(256) void k_update(
 {
     __shared__ double red_e[256], red_g[256];
     __shared__ int last;
+    if (dstate[ST_STOP]) return;   // a stop rule held (cvt_lloyd_until)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int live = (mode & UPD_DEV_S) ? dstate[ST_NSITES] : S;
     const float omega = domega ? *domega : omega_arg;
@@ -623,10 +625,41 @@ static __global__ __launch_bounds__(256) void k_update(
             const int h = dstate[ST_HIST];
             if (h < fa.hist_cap) { fa.hist[2 * h] = E; fa.hist[2 * h + 1] = G; }
             dstate[ST_HIST] = h + 1;
+            // stop rule of cvt_lloyd_until: this iteration's energy is at or
+            // below the target, or fell by less than rel_tol * E over the
+            // last `window` iterations of the call
+            if (dstate[ST_UACTIVE] && fa.until) {
+                const int k = h - dstate[ST_UBASE], win = dstate[ST_UWIN];
+                bool stop = E <= fa.until[1];
+                if (win > 0 && k >= win && h < fa.hist_cap)
+                    stop = stop || (fa.hist[2 * (h - win)] - E) < fa.until[0] * E;
+                if (stop) dstate[ST_STOP] = 1;
+            }
         }
         if (fa.out_eg) { fa.out_eg[0] = E; fa.out_eg[1] = G; }
         dstate[ST_EPOCH] = dstate[ST_EPOCH] + 1;
         dstate[ST_DONE] = 0;
+    }
+}
+
+// cvt_lloyd_until: arm the stop rule (relative decrease over `window`
+// iterations below rel_tol, or energy at or below target) from the current
+// history length, or disarm it and report the iterations the call ran
+static __global__ void k_until(int *__restrict__ dstate, double *__restrict__ until, int arm,
+                               double rel_tol, double target, int window, int *__restrict__ ran)
+{
+    if (threadIdx.x != 0) return;
+    if (arm) {
+        until[0] = rel_tol;
+        until[1] = target;
+        dstate[ST_UBASE] = dstate[ST_HIST];
+        dstate[ST_UWIN] = window;
+        dstate[ST_STOP] = 0;
+        dstate[ST_UACTIVE] = 1;
+    } else {
+        *ran = dstate[ST_HIST] - dstate[ST_UBASE];
+        dstate[ST_UACTIVE] = 0;
+        dstate[ST_STOP] = 0;
     }
 }
 

@@ -30,11 +30,19 @@ struct __align__(8) OvfRec {
 };
 
 // device flags
-enum { FLAG_FULL = 1, FLAG_BAD = 2, NUM_FLAGS = 4 };
+// (FLAG_FLOW: a wait of the persistent JFA ran out of spins; the labels of
+// that JFA are not valid)
+enum { FLAG_FULL = 1, FLAG_BAD = 2, FLAG_FLOW = 3, NUM_FLAGS = 4 };
 // device state words
 // (ST_DONE counts the finished blocks of a site-update launch; the last block
-// to finish resets it)
-enum { ST_EPOCH = 0, ST_HIST = 1, ST_OVFN = 2, ST_NSITES = 3, ST_DONE = 4, NUM_STATE = 5 };
+// to finish resets it.  ST_STOP, when set, makes every kernel of a Lloyd
+// iteration return at entry: cvt_lloyd_until arms the stop rule (ST_UACTIVE,
+// the history index it started at ST_UBASE, the window ST_UWIN) and the
+// update's last block raises ST_STOP when the rule holds.)
+enum {
+    ST_EPOCH = 0, ST_HIST = 1, ST_OVFN = 2, ST_NSITES = 3, ST_DONE = 4,
+    ST_STOP = 5, ST_UBASE = 6, ST_UWIN = 7, ST_UACTIVE = 8, NUM_STATE = 9
+};
 
 // what the last block of a site update does with the energy partials
 struct FinalArgs {
@@ -43,6 +51,7 @@ struct FinalArgs {
     int record;            // append to the history
     double e_scale, g_scale;
     double *out_eg;        // optional (E, |grad E|) copy
+    const double *until;   // stop rule (relative decrease, target energy), read when ST_UACTIVE
 };
 
 

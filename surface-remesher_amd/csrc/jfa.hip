@@ -10,7 +10,11 @@
 // (inf, inf) at index capS.  The gather then needs no address arithmetic, and
 // the order of byte offsets is the order of indices.
 //
-// Candidate order: (d, index) lexicographic with d = fmaf(dx, dx, dy*dy) >= 0.
+// Candidate order: (d, index) lexicographic with d = fmaf(dy, dy, dx*dx) >= 0
+// (dx*dx rounded, then dy*dy + dx*dx rounded once).  A label read from a tap
+// row serves the pixels of up to three output rows in the same column, so
+// dx*dx is computed once per label and each use costs one subtraction (dy),
+// one fma and one min.
 // The pair is packed into one 64-bit word, hi = bits(d), lo = label, and read
 // as a double: for d >= 0 (including +inf) the hi word is below 0x7ff00000, so
 // the double is finite, not NaN, and double order equals (d, label) order
@@ -23,6 +27,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cvt_types.h"
 #include "jfa.h"
 
 #include <algorithm>
@@ -63,8 +68,10 @@ __global__ __launch_bounds__(256) void k_jfa_pass(const uint32_t *__restrict__ i
                                                   uint32_t *__restrict__ out,
                                                   const float2 *__restrict__ sites, int W,
                                                   int H, int tilesX, int ntiles, int step,
-                                                  uint32_t *__restrict__ clear, uint32_t none)
+                                                  uint32_t *__restrict__ clear, uint32_t none,
+                                                  const int *__restrict__ stop)
 {
+    if (stop && *stop) return;
     // XCD-aware tile order (wave-uniform values live in scalar registers)
     const int b = blockIdx.x;
     const int q = ntiles >> 3, r = ntiles & 7;
@@ -118,7 +125,7 @@ __global__ __launch_bounds__(256) void k_jfa_pass(const uint32_t *__restrict__ i
             const float2 s = *reinterpret_cast<const float2 *>(sb + l[t]);
             const float dx = s.x - px;
             const float dy = s.y - py;
-            const float d = __builtin_fmaf(dx, dx, dy * dy);
+            const float d = __builtin_fmaf(dy, dy, dx * dx);
             const double key = jfa_key(d, l[t]);
             best = (t == 0) ? key : __builtin_fmin(best, key);
         }
@@ -330,7 +337,7 @@ __device__ __forceinline__ double cand(float2 s, uint32_t l, float px, float py)
 {
     const float dx = s.x - px;
     const float dy = s.y - py;
-    return jfa_key(__builtin_fmaf(dx, dx, dy * dy), l);
+    return jfa_key(__builtin_fmaf(dy, dy, dx * dx), l);
 }
 
 template <int MODE, int TAB>
@@ -401,13 +408,12 @@ __device__ __forceinline__ void eval_item(const Item<MODE> &it, const MapGeo &m,
                 P[6] = rok ? it.r[r].x : c.z;
                 P[7] = rok ? it.r[r].y : c.w;
             }
-            float sx[NP], dyy[NP];
+            float sx[NP], dyr[NP];
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
                 const float2 s = tab(P[k]);
-                const float dy = s.y - py;
                 sx[k] = s.x;
-                dyy[k] = dy * dy;
+                dyr[k] = s.y - py;
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -416,7 +422,8 @@ __device__ __forceinline__ void eval_item(const Item<MODE> &it, const MapGeo &m,
                 for (int t = 0; t < 3; ++t) {
                     const int k = j + t * MODE;      // left, centre, right
                     const float dx = sx[k] - px;
-                    const double key = jfa_key(__builtin_fmaf(dx, dx, dyy[k]), P[k]);
+                    const double key =
+                        jfa_key(__builtin_fmaf(dyr[k], dyr[k], dx * dx), P[k]);
                     best[j] = (r == 0 && t == 0) ? key : __builtin_fmin(best[j], key);
                 }
             }
@@ -530,50 +537,181 @@ __device__ __forceinline__ void run_of(const RunGeo &g, int u, int &y, int &seg)
     seg = __builtin_amdgcn_readfirstlane(seg);
 }
 
-// Runs [u0, u1) of one map by all waves of the block, dealt round robin.  Six
-// tap-row buffers rotate through a run: item j (centre y + jk) reads buffers
-// j, j+1, j+2 while the row item j+1 adds is loaded into buffer j+3, and the
-// last item's slot loads the next run's first three rows into buffers 0-2.
-// Rows past the grid end their chain: their items are skipped and their loads
-// not issued.  A wave's first loads are issued before the site table is
-// staged, so the two latencies overlap.
+// One tap row of a run, evaluated as it streams in.  A run's output rows are
+// y + j*k (j = 0..3) and its tap rows y + (i - 1)*k (i = 0..5); tap row I is
+// the down tap of output row I - 2, the centre of I - 1 and the up tap of I.
+// Each label of the row is looked up once, its dx*dx computed once per pixel
+// column, and then folded into the (up to three) output rows that use it:
+// dy, fma and min per use.  Off-grid tap rows are never loaded (their
+// redirected candidates duplicate the centre row's); a column quad that is
+// off the grid for every lane of the wave is skipped for the same reason,
+// and one that is off for some lanes holds the centre quad's labels there.
+template <int MODE, int TAB, int I>
+__device__ __forceinline__ void run_row(const RowData<MODE> &R, const SiteTab<TAB> &tab,
+                                        const float (&px)[4], const float (&py)[4],
+                                        double (&best)[4][4], bool lok, bool rok, bool left,
+                                        bool right)
+{
+    constexpr int j0 = I - 2 < 0 ? 0 : I - 2;
+    constexpr int j1 = I > 3 ? 3 : I;
+    if constexpr (MODE == STEP_ANY) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (q == 0 && !left) continue;
+            if (q == 2 && !right) continue;
+            const u32x4 &Q = q == 0 ? R.l : q == 1 ? R.c : R.r;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const uint32_t l = q_el(Q, jj);
+                const float2 st = tab(l);
+                const float dx = st.x - px[jj];
+                const float dx2 = dx * dx;
+                // one key register per label: lo = label, hi = each use's d
+                u32x2 kv;
+                kv.x = l;
+#pragma unroll
+                for (int j = j0; j <= j1; ++j) {
+                    const float dy = st.y - py[j];
+                    kv.y = __float_as_uint(__builtin_fmaf(dy, dy, dx2));
+                    best[j][jj] = __builtin_fmin(best[j][jj], __builtin_bit_cast(double, kv));
+                }
+            }
+        }
+    } else {
+        // tap positions x - MODE .. x + 3 + MODE of the row; pixel jj takes
+        // positions jj, jj + MODE, jj + 2 * MODE (left, centre, right)
+        constexpr int NP = (MODE == STEP_1) ? 6 : 8;
+        const u32x4 &c = R.c;
+        uint32_t P[NP];
+        if constexpr (MODE == STEP_1) {
+            P[0] = lok ? R.l : c.x;
+            P[1] = c.x; P[2] = c.y; P[3] = c.z; P[4] = c.w;
+            P[5] = rok ? R.r : c.w;
+        } else {
+            P[0] = lok ? R.l.x : c.x;
+            P[1] = lok ? R.l.y : c.y;
+            P[2] = c.x; P[3] = c.y; P[4] = c.z; P[5] = c.w;
+            P[6] = rok ? R.r.x : c.z;
+            P[7] = rok ? R.r.y : c.w;
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const float2 st = tab(P[k]);
+            float dy[4];
+#pragma unroll
+            for (int j = j0; j <= j1; ++j) dy[j] = st.y - py[j];
+            u32x2 kv;
+            kv.x = P[k];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const int jj = k - t * MODE;
+                if (jj < 0 || jj > 3) continue;
+                const float dx = st.x - px[jj];
+                const float dx2 = dx * dx;
+#pragma unroll
+                for (int j = j0; j <= j1; ++j) {
+                    kv.y = __float_as_uint(__builtin_fmaf(dy[j], dy[j], dx2));
+                    best[j][jj] = __builtin_fmin(best[j][jj], __builtin_bit_cast(double, kv));
+                }
+            }
+        }
+    }
+}
+
+// the largest key: +inf distance with an all-ones label, above every
+// candidate (the sentinel's key is (+inf, none) with none < 0xffffffff)
+__device__ __forceinline__ double key_max() { return __hiloint2double(0x7f800000, -1); }
+
+// Runs u, u + stride, ... below u_end of one map, by one wave.  The wave
+// streams a run's six tap rows through three row buffers, two rows ahead of
+// the one being evaluated; during the last two rows it loads the next run's
+// first two.  Rows past the grid end their chain: their outputs are not
+// stored and their loads not issued.  With `stage` the site table is staged
+// after the wave's first loads are issued, so the two latencies overlap.
+template <int MODE, int TAB>
+__device__ __forceinline__ void wave_runs(const MapGeo &m, const SiteTab<TAB> &tab,
+                                          const RunGeo &g, int u, int u_end, int stride,
+                                          bool stage)
+{
+    const int lane4 = 4 * (threadIdx.x & 63);
+    const int k = m.step, H = m.H, W = m.W;
+    RowData<MODE> R0, R1, R2;
+    int y = 0, seg = 0;
+    if (u < u_end) {
+        run_of(g, u, y, seg);
+        if (y - k >= 0) load_row<MODE>(R0, m, y - k, seg, lane4);
+        if (y < H) load_row<MODE>(R1, m, y, seg, lane4);
+    }
+    if (stage) tab.stage();
+    while (u < u_end) {
+        const int un = u + stride;
+        int yn = 0, sn = 0;
+        if (un < u_end) run_of(g, un, yn, sn);
+        const int xb = seg * 256 + lane4;
+        const int xc = min(xb, W - 4);
+        float px[4], py[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) px[jj] = (float)(xc + jj) + 0.5f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            py[j] = __int_as_float(
+                __builtin_amdgcn_readfirstlane(__float_as_int((float)(y + j * k) + 0.5f)));
+        const bool lok = xc > 0, rok = xc + 4 < W;
+        bool left = true, right = true;
+        if constexpr (MODE == STEP_ANY) {
+            left = __ballot(xc - k >= 0) != 0ull;
+            right = __ballot(xc + 3 + k < W) != 0ull;
+        }
+        double best[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) best[j][jj] = key_max();
+
+        if (y + k < H) load_row<MODE>(R2, m, y + k, seg, lane4);
+        if (y - k >= 0) run_row<MODE, TAB, 0>(R0, tab, px, py, best, lok, rok, left, right);
+        if (y + 2 * k < H) load_row<MODE>(R0, m, y + 2 * k, seg, lane4);
+        if (y < H) run_row<MODE, TAB, 1>(R1, tab, px, py, best, lok, rok, left, right);
+        if (y + 3 * k < H) load_row<MODE>(R1, m, y + 3 * k, seg, lane4);
+        if (y + k < H) run_row<MODE, TAB, 2>(R2, tab, px, py, best, lok, rok, left, right);
+        if (y + 4 * k < H) load_row<MODE>(R2, m, y + 4 * k, seg, lane4);
+        if (y + 2 * k < H) run_row<MODE, TAB, 3>(R0, tab, px, py, best, lok, rok, left, right);
+        if (un < u_end && yn - k >= 0) load_row<MODE>(R0, m, yn - k, sn, lane4);
+        if (y + 3 * k < H) run_row<MODE, TAB, 4>(R1, tab, px, py, best, lok, rok, left, right);
+        if (un < u_end && yn < H) load_row<MODE>(R1, m, yn, sn, lane4);
+        if (y + 4 * k < H) run_row<MODE, TAB, 5>(R2, tab, px, py, best, lok, rok, left, right);
+
+        if (xb < W) {
+            const uint32_t o1 = 4u * (uint32_t)xc;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int yj = y + j * k;
+                if (yj >= H) break;
+                const uint32_t s1 = row_off(m, yj);
+                const u32x4 r4 = {(uint32_t)__double2loint(best[j][0]),
+                                  (uint32_t)__double2loint(best[j][1]),
+                                  (uint32_t)__double2loint(best[j][2]),
+                                  (uint32_t)__double2loint(best[j][3])};
+                __builtin_amdgcn_raw_buffer_store_b128(r4, m.rout, o1, s1, CVT_MAP_STORE_POLICY);
+                if (m.clear) {
+                    const u32x4 n4 = {m.none, m.none, m.none, m.none};
+                    __builtin_amdgcn_raw_buffer_store_b128(n4, m.rclr, o1, s1, CVT_MAP_STORE_POLICY);
+                }
+            }
+        }
+        u = un;
+        y = yn;
+        seg = sn;
+    }
+}
+
+// Runs [u0, u1) of one map by all waves of the block, dealt round robin.
 template <int MODE, int TAB>
 __device__ __forceinline__ void quad_runs(const MapGeo &m, const SiteTab<TAB> &tab, int u0, int u1)
 {
-    const RunGeo g = run_geo(m);
     const int nwaves = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane4 = 4 * (threadIdx.x & 63);
-    const int k = m.step, H = m.H;
-    RowData<MODE> B0, B1, B2, B3, B4, B5;
-    int u = u0 + wave, y = 0, seg = 0;
-    if (u < u1) {
-        run_of(g, u, y, seg);
-        load_row<MODE>(B0, m, up_row(m, y), seg, lane4);
-        load_row<MODE>(B1, m, y, seg, lane4);
-        load_row<MODE>(B2, m, down_row(m, y), seg, lane4);
-    }
-    tab.stage();
-    for (; u < u1; u += nwaves) {
-        // item 0 (a chain shorter than the longest can leave a run empty)
-        if (y + k < H) load_row<MODE>(B3, m, down_row(m, y + k), seg, lane4);
-        if (y < H) eval_item<MODE, TAB>(item_of<MODE>(B0, B1, B2), m, tab, y, seg, lane4);
-        // item 1
-        if (y + 2 * k < H) load_row<MODE>(B4, m, down_row(m, y + 2 * k), seg, lane4);
-        if (y + k < H) eval_item<MODE, TAB>(item_of<MODE>(B1, B2, B3), m, tab, y + k, seg, lane4);
-        // item 2
-        if (y + 3 * k < H) load_row<MODE>(B5, m, down_row(m, y + 3 * k), seg, lane4);
-        if (y + 2 * k < H) eval_item<MODE, TAB>(item_of<MODE>(B2, B3, B4), m, tab, y + 2 * k, seg, lane4);
-        // item 3, with the next run's first rows
-        const int yc = y + 3 * k, sc = seg;
-        if (u + nwaves < u1) {
-            run_of(g, u + nwaves, y, seg);
-            load_row<MODE>(B0, m, up_row(m, y), seg, lane4);
-            load_row<MODE>(B1, m, y, seg, lane4);
-            load_row<MODE>(B2, m, down_row(m, y), seg, lane4);
-        }
-        if (yc < H) eval_item<MODE, TAB>(item_of<MODE>(B3, B4, B5), m, tab, yc, sc, lane4);
-    }
+    wave_runs<MODE, TAB>(m, tab, run_geo(m), u0 + wave, u1, nwaves, true);
 }
 
 // Single map, site table staged at LDS byte 0 in a static array of ENTRIES
@@ -613,8 +751,9 @@ template <int MODE, int ENTRIES, bool RUNS>
 __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa_quad(
     const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
     const float2 *__restrict__ sites, int table_f4, int W, int H, int step,
-    uint32_t *__restrict__ clear, uint32_t none, int nclear, int rstep)
+    uint32_t *__restrict__ clear, uint32_t none, int nclear, int rstep, const int *__restrict__ stop)
 {
+    if (stop && *stop) return;
     if (clear && nclear > 0) {
         const __amdgpu_buffer_rsrc_t rc = buf_rsrc(clear, 4u * (uint32_t)W * (uint32_t)H);
         const int R = rstep > 0 ? 9 : 1;
@@ -659,6 +798,244 @@ __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa
         tab.base = 0;
         tab.n4 = 0;
         quad_layout<MODE, TAB_GLOBAL>(m, tab, RUNS);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent JFA (declared in jfa.h).  A workgroup holds TEAMS teams of four
+// waves; a team takes one ticket at a time (the next one is fetched while the
+// current item runs), waits until what the item reads is complete, runs it
+// and publishes its completion.  Completion is published after every wave of
+// the team has waited for its own map stores, which are device-scope
+// write-through stores: a flag that reads complete therefore implies that the
+// rows it covers are visible to every XCD.
+// ---------------------------------------------------------------------------
+#define FLOW_SPIN_LIMIT (1 << 20)
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t add_agent(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0): every vector memory access of the wave, the map stores
+// included, has completed
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0xF70); }
+
+// Spin (wave-uniform) until every lane's flag reads `want`; lanes with
+// need == false take part without a load.  A wait that runs out of spins
+// raises FLAG_FLOW and returns; once the flag is up every later wait returns
+// at its next check, so a launch whose flags never complete still ends
+// promptly (with labels the host then reports as invalid).
+__device__ __forceinline__ void flow_wait(const uint32_t *p, bool need, uint32_t want, int *dflags)
+{
+    for (int spin = 1;; ++spin) {
+        const uint32_t v = need ? ld_agent(p) : want;
+        if (__ballot(v != want) == 0ull) return;
+        if ((spin & 255) == 0 &&
+            (spin >= FLOW_SPIN_LIMIT ||
+             __hip_atomic_load(dflags + FLAG_FLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            if ((threadIdx.x & 63) == 0) atomicOr(dflags + FLAG_FLOW, 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// the receiver of (site i, offset r) and its value, exactly as k_jfa_seeds
+template <int TAB>
+__device__ __forceinline__ void flow_scatter_one(const FlowArgs &a, const SiteTab<TAB> &tab,
+                                                 __amdgpu_buffer_rsrc_t rseed,
+                                                 __amdgpu_buffer_rsrc_t rout, int pr)
+{
+    const int i = pr / 9, r = pr - 9 * (pr / 9);
+    const float2 s = a.sites[i];
+    const int qx = seed_pixel(s.x, a.W), qy = seed_pixel(s.y, a.H);
+    const int step = a.step0;
+    const int px = qx - (r % 3 - 1) * step, py = qy - (r / 3 - 1) * step;
+    if (px < 0 || px >= a.W || py < 0 || py >= a.H) return;
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    double best = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int ox = (k % 3 - 1) * step, oy = (k / 3 - 1) * step;
+        const int tx = (px + ox >= 0 && px + ox < a.W) ? px + ox : px;
+        const int ty = (py + oy >= 0 && py + oy < a.H) ? py + oy : py;
+        const uint32_t l =
+            __builtin_amdgcn_raw_buffer_load_b32(rseed, 4u * (uint32_t)(ty * a.W + tx), 0, 0);
+        const double key = cand(tab(l), l, fx, fy);
+        best = (k == 0) ? key : __builtin_fmin(best, key);
+    }
+    __builtin_amdgcn_raw_buffer_store_b32((uint32_t)__double2loint(best), rout,
+                                          4u * (uint32_t)(py * a.W + px), 0,
+                                          CVT_MAP_STORE_POLICY);
+}
+
+// pixel of site i's seed shifted by d * step (d = -1, 0, 1 per axis), or -1
+// off the grid
+__device__ __forceinline__ int flow_receiver(const FlowArgs &a, int i, int r, int step)
+{
+    const float2 s = a.sites[i];
+    const int x = seed_pixel(s.x, a.W) + (r % 3 - 1) * step;
+    const int y = seed_pixel(s.y, a.H) + (r / 3 - 1) * step;
+    return (x >= 0 && x < a.W && y >= 0 && y < a.H) ? y * a.W + x : -1;
+}
+
+__device__ __forceinline__ int flow_mode(int step)
+{
+    return step == 1 ? STEP_1 : step == 2 ? STEP_2 : STEP_ANY;
+}
+
+template <int ENTRIES, int TEAMS>
+__global__ __launch_bounds__(256 * TEAMS, TEAMS == 1 ? 4 : 1) void k_jfa_flow(const FlowArgs a)
+{
+    constexpr int TAB = ENTRIES > 0 ? TAB_LDS0 : TAB_SPLIT;
+    constexpr int NE = ENTRIES > 0 ? ENTRIES : -ENTRIES;
+    __shared__ float4 lds_tab[NE / 2];
+    __shared__ uint32_t t_slot[TEAMS][2];
+    __shared__ uint32_t t_bar[TEAMS];
+    // a stop rule held (cvt_lloyd_until): every workgroup returns before it
+    // touches the control words
+    if (a.stop && *a.stop) return;
+
+    SiteTab<TAB> tab;
+    tab.lds = (lds_cchar *)lds_tab;
+    tab.src = reinterpret_cast<const f32x4 *>(a.sites);
+    tab.base = 0;
+    if constexpr (ENTRIES > 0) {
+        tab.n4 = a.table_f4;
+    } else {
+        // entries 0 .. NE - 3 staged, then a zero pair; the rest gathered
+        constexpr int kept = NE - 2;
+        tab.n4 = kept / 2;
+        tab.g = buf_rsrc(a.sites + kept, 8u * (uint32_t)(2 * a.table_f4 - kept));
+        tab.split = 8u * (uint32_t)kept;
+    }
+    if (threadIdx.x < TEAMS) t_bar[threadIdx.x] = 0;
+    tab.stage();   // ends with a workgroup barrier
+
+    const int team = (int)(threadIdx.x >> 8);
+    const int wt = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);
+    const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x & 255;
+    const bool leader = wt == 0 && lane == 0;
+    const uint32_t epoch = __builtin_amdgcn_readfirstlane(ld_agent(a.ctl + CTL_EPOCH));
+    const long N = (long)a.W * a.H;
+    const uint32_t map_bytes = 4u * (uint32_t)N;
+    uint32_t gen = 0;
+    auto team_sync = [&]() {
+        if constexpr (TEAMS == 1) {
+            __syncthreads();
+        } else {
+            gen += 4;
+            if (lane == 0)
+                __hip_atomic_fetch_add(&t_bar[team], 1u, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            while (__hip_atomic_load(&t_bar[team], __ATOMIC_ACQUIRE,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP) < gen)
+                __builtin_amdgcn_s_sleep(1);
+        }
+    };
+
+    if (leader) t_slot[team][0] = add_agent(a.ctl + CTL_TICKET, 1u);
+    team_sync();
+    int cur = 0;
+    for (;;) {
+        const int t = __builtin_amdgcn_readfirstlane((int)t_slot[team][cur]);
+        if (t >= a.total) break;
+        uint32_t tn = 0;
+        if (leader) tn = add_agent(a.ctl + CTL_TICKET, 1u);
+        int ph = 0;
+        while (ph + 1 < a.nph && t >= a.ph[ph + 1].start) ++ph;
+        const int kind = a.ph[ph].kind, pass = a.ph[ph].pass, step = a.ph[ph].step;
+        const int item = t - a.ph[ph].start;
+
+        // rows of a pass item: run group g0 + (lane >> 2) ... (four output rows
+        // per group); its six tap rows per group for the wait
+        int rg_lo = 0, rg_hi = 0, chains = 1;
+        if (kind == FLOW_SCATTER) {
+            const __amdgpu_buffer_rsrc_t rseed = buf_rsrc(a.seed, map_bytes);
+            const __amdgpu_buffer_rsrc_t rout = buf_rsrc(a.maps, map_bytes);
+#pragma unroll
+            for (int kk = 0; kk < CVT_FLOW_CHUNK / 256; ++kk) {
+                const int pr = item * CVT_FLOW_CHUNK + kk * 256 + tid;
+                if (pr < 9 * a.S) flow_scatter_one<TAB>(a, tab, rseed, rout, pr);
+            }
+        } else if (kind == FLOW_SEED_RESET || kind == FLOW_FIRST_RESET) {
+            const bool seedr = kind == FLOW_SEED_RESET;
+            const uint32_t need = a.ph[seedr ? 0 : 1].count;
+            flow_wait(a.ctl + (seedr ? CTL_SCATTER : CTL_PASS1), lane == 0, need, a.dflags);
+            const __amdgpu_buffer_rsrc_t rc = buf_rsrc(seedr ? a.seed : a.maps, map_bytes);
+            const int n = seedr ? a.S : 9 * a.S;
+#pragma unroll
+            for (int kk = 0; kk < CVT_FLOW_CHUNK / 256; ++kk) {
+                const int e = item * CVT_FLOW_CHUNK + kk * 256 + tid;
+                if (e >= n) continue;
+                const int q = seedr ? flow_receiver(a, e, 4, 0) : flow_receiver(a, e / 9, e % 9, a.step0);
+                if (q >= 0)
+                    __builtin_amdgcn_raw_buffer_store_b32(a.none, rc, 4u * (uint32_t)q, 0,
+                                                          CVT_MAP_STORE_POLICY);
+            }
+        } else {
+            MapGeo m = map_geo(a.maps + (long)(pass - 1) * N, a.maps + (long)pass * N, nullptr,
+                               a.W, a.H, step, a.none);
+            const RunGeo g = run_geo(m);
+            chains = g.chains;
+            rg_lo = item * a.upi;
+            rg_hi = min(rg_lo + a.upi, g.chains * g.nq);
+            if (pass == 1) {
+                const uint32_t need = a.ph[0].count;
+                flow_wait(a.ctl + CTL_SCATTER, lane == 0, need, a.dflags);
+            } else {
+                // lane l checks tap row (l % 6) of run group rg_lo + l / 6
+                const int rg = rg_lo + lane / 6, ti = lane % 6;
+                const int y0 = (rg % g.chains) + 4 * (rg / g.chains) * step;
+                const int row = y0 + (ti - 1) * step;
+                const bool need = lane < 6 * (rg_hi - rg_lo) && row >= 0 && row < a.H;
+                flow_wait(a.flags + (long)(pass - 1) * a.H + (need ? row : 0), need, epoch,
+                          a.dflags);
+            }
+            const int u0 = rg_lo * g.nseg, u1 = rg_hi * g.nseg;
+            switch (flow_mode(step)) {
+            case STEP_1: wave_runs<STEP_1, TAB>(m, tab, g, u0 + wt, u1, 4, false); break;
+            case STEP_2: wave_runs<STEP_2, TAB>(m, tab, g, u0 + wt, u1, 4, false); break;
+            default: wave_runs<STEP_ANY, TAB>(m, tab, g, u0 + wt, u1, 4, false); break;
+            }
+        }
+        wait_vm();
+        if (leader) t_slot[team][cur ^ 1] = tn;
+        team_sync();
+        if (wt == 0) {
+            if (kind == FLOW_SCATTER) {
+                if (lane == 0) add_agent(a.ctl + CTL_SCATTER, 1u);
+            } else if (kind == FLOW_PASS) {
+                // lane l publishes output row (l % 4) of run group rg_lo + l / 4
+                const int rg = rg_lo + lane / 4, j = lane % 4;
+                const int row = (rg % chains) + (4 * (rg / chains) + j) * step;
+                if (lane < 4 * (rg_hi - rg_lo) && row < a.H)
+                    st_agent(a.flags + (long)pass * a.H + row, epoch);
+                if (pass == 1 && lane == 0) add_agent(a.ctl + CTL_PASS1, 1u);
+            }
+        }
+        cur ^= 1;
+    }
+    // the last team out resets the counters and advances the epoch for the
+    // next launch (every team has read the epoch and taken its last ticket)
+    if (leader) {
+        const uint32_t nteams = gridDim.x * TEAMS;
+        if (add_agent(a.ctl + CTL_EXIT, 1u) == nteams - 1) {
+            st_agent(a.ctl + CTL_TICKET, 0u);
+            st_agent(a.ctl + CTL_SCATTER, 0u);
+            st_agent(a.ctl + CTL_PASS1, 0u);
+            st_agent(a.ctl + CTL_EXIT, 0u);
+            st_agent(a.ctl + CTL_EPOCH, epoch + 1u);
+        }
     }
 }
 
@@ -793,29 +1170,29 @@ JfaPlan plan_jfa(int W, int H, int capS)
 template <int MODE, bool RUNS>
 static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
                         const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
-                        int nclear, int rstep)
+                        int nclear, int rstep, const int *stop)
 {
     const dim3 g(p.grid), b(p.threads);
     switch (p.lds_entries) {
     case 2048:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 2048, RUNS>), g, b, 0, st, in, out, sites, p.table_f4,
-                           W, H, step, clear, none, nclear, rstep);
+                           W, H, step, clear, none, nclear, rstep, stop);
         break;
     case 5120:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 5120, RUNS>), g, b, 0, st, in, out, sites, p.table_f4,
-                           W, H, step, clear, none, nclear, rstep);
+                           W, H, step, clear, none, nclear, rstep, stop);
         break;
     case 20480:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 20480, RUNS>), g, b, 0, st, in, out, sites,
-                           p.table_f4, W, H, step, clear, none, nclear, rstep);
+                           p.table_f4, W, H, step, clear, none, nclear, rstep, stop);
         break;
     case -20480:
         hipLaunchKernelGGL((k_jfa_quad<MODE, -20480, RUNS>), g, b, 0, st, in, out, sites,
-                           p.table_f4, W, H, step, clear, none, nclear, rstep);
+                           p.table_f4, W, H, step, clear, none, nclear, rstep, stop);
         break;
     default:
         hipLaunchKernelGGL((k_jfa_quad<MODE, 0, RUNS>), g, b, 0, st, in, out, sites, p.table_f4, W,
-                           H, step, clear, none, nclear, rstep);
+                           H, step, clear, none, nclear, rstep, stop);
         break;
     }
 }
@@ -842,29 +1219,29 @@ static bool use_runs(const JfaPlan &p, int W, int H, int step)
 
 void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
-                     int nclear, int rstep)
+                     int nclear, int rstep, const int *stop)
 {
     if (p.quad) {
         const bool runs = use_runs(p, W, H, step);
         if (step == 1 && runs)
-            launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
+            launch_quad<STEP_1, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep, stop);
         else if (step == 1)
-            launch_quad<STEP_1, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
+            launch_quad<STEP_1, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep, stop);
         // (step 2 with the all-global table keeps bands: six buffers of it
         // spill)
         else if (step == 2 && runs && p.lds_entries != 0)
-            launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
-        else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
+            launch_quad<STEP_2, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep, stop);
+        else if (step == 2) launch_quad<STEP_2, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep, stop);
         else if (runs)
-            launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
-        else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep);
+            launch_quad<STEP_ANY, true>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep, stop);
+        else launch_quad<STEP_ANY, false>(st, p, in, out, sites, W, H, step, clear, none, nclear, rstep, stop);
         return;
     }
     constexpr int R = CVT_JFA_ROWS;
     const int tilesX = (W + 63) / 64, tilesY = (H + 4 * R - 1) / (4 * R);
     const int ntiles = tilesX * tilesY;
     hipLaunchKernelGGL(k_jfa_pass<R>, dim3(ntiles), dim3(256), 0, st, in, out, sites, W, H, tilesX,
-                       ntiles, step, clear, none);
+                       ntiles, step, clear, none, stop);
 }
 
 
@@ -884,8 +1261,9 @@ void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint3
 __global__ __launch_bounds__(256) void k_jfa_seeds(const uint32_t *__restrict__ seed,
                                                    uint32_t *__restrict__ out,
                                                    const float2 *__restrict__ sites, int S, int W,
-                                                   int H, int step)
+                                                   int H, int step, const int *__restrict__ stop)
 {
+    if (stop && *stop) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / 9, r = t - 9 * (t / 9);
     if (i >= S) return;
@@ -909,11 +1287,80 @@ __global__ __launch_bounds__(256) void k_jfa_seeds(const uint32_t *__restrict__ 
 }
 
 void launch_jfa_seed_pass(hipStream_t st, const uint32_t *seed, uint32_t *out,
-                          const float2 *sites, int S, int W, int H, int step)
+                          const float2 *sites, int S, int W, int H, int step, const int *stop)
 {
     if (S > 0)
         hipLaunchKernelGGL(k_jfa_seeds, dim3((9 * S + 255) / 256), dim3(256), 0, st, seed, out,
-                           sites, S, W, H, step);
+                           sites, S, W, H, step, stop);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent JFA: plan and launch
+// ---------------------------------------------------------------------------
+// LDS table sizes of the persistent kernel, in float2 entries: the largest
+// leaves 64 bytes of the 160 KB for the teams' ticket slots and barriers
+static const int kFlowEntries[3] = {2048, 5120, 20472};
+
+static int flow_entries(const JfaPlan &p)
+{
+    const int entries = 2 * p.table_f4;
+    for (int e : kFlowEntries)
+        if (entries <= e) return e;
+    return -kFlowEntries[2];
+}
+
+bool flow_eligible(const JfaPlan &p, int W, int H)
+{
+    // three passes or more: the first step is at least 2
+    return p.quad && p.lds_entries != 0 && W % 32 == 0 && std::max(W, H) > 2;
+}
+
+void flow_phases(FlowArgs &a)
+{
+    const int nseg = (a.W + 255) / 256;
+    a.upi = (4 + nseg - 1) / nseg;
+    const int C = CVT_FLOW_CHUNK;
+    const int nsc = (9 * a.S + C - 1) / C;
+    int n = 0, t = 0;
+    auto add = [&](int kind, int pass, int step, int count) {
+        FlowPhase &f = a.ph[n++];
+        f.start = t;
+        f.count = count;
+        f.kind = kind;
+        f.pass = pass;
+        f.step = step;
+        t += count;
+    };
+    add(FLOW_SCATTER, 0, a.step0, nsc);
+    for (int p = 1; p < a.np; ++p) {
+        const int step = std::max(a.step0 >> p, 1);
+        const int chains = std::min(step, a.H);
+        const int rows = (a.H + step - 1) / step;
+        const int groups = chains * ((rows + 3) / 4);
+        add(FLOW_PASS, p, step, (groups + a.upi - 1) / a.upi);
+        if (p == 1) add(FLOW_SEED_RESET, 0, 0, (a.S + C - 1) / C);
+        if (p == 2) add(FLOW_FIRST_RESET, 0, a.step0, nsc);
+    }
+    a.nph = n;
+    a.total = t;
+}
+
+template <int E, int TEAMS>
+static void launch_flow_e(hipStream_t st, const FlowArgs &a, int grid)
+{
+    hipLaunchKernelGGL((k_jfa_flow<E, TEAMS>), dim3(grid), dim3(256 * TEAMS), 0, st, a);
+}
+
+void launch_jfa_flow(hipStream_t st, const JfaPlan &p, const FlowArgs &a)
+{
+    const int e = flow_entries(p);
+    const int cu = g_num_cu > 0 ? g_num_cu : 256;
+    switch (e) {
+    case 2048: launch_flow_e<2048, 4>(st, a, cu); break;
+    case 5120: launch_flow_e<5120, 4>(st, a, cu); break;
+    case 20472: launch_flow_e<20472, 4>(st, a, cu); break;
+    default: launch_flow_e<-20472, 4>(st, a, cu); break;
+    }
 }
 
 }  // namespace cvt

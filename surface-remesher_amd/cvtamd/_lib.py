@@ -51,6 +51,7 @@ SIGNATURES = {
     "cvt_get_labels": (_i, [_vp, _vp]),
     "cvt_moments": (_i, [_vp, _vp]),
     "cvt_lloyd": (_i, [_vp, _i, _f]),
+    "cvt_lloyd_until": (_i, [_vp, _f, _d, _i, _d, _i, C.POINTER(_i)]),
     "cvt_history_length": (_i, [_vp]),
     "cvt_get_history": (_i, [_vp, _vp, _i]),
     "cvt_clear_history": (_i, [_vp]),

@@ -155,6 +155,17 @@ class Engine:
     def lloyd(self, iterations: int, omega: float = 1.0):
         call("cvt_lloyd", self._h, int(iterations), float(omega))
 
+    def lloyd_until(self, max_iter: int, rel_tol: float = 0.0, window: int = 0,
+                    target: float = -np.inf, omega: float = 1.0) -> int:
+        """Lloyd steps that stop on the device (cvt_lloyd_until): after the
+        first iteration with energy <= target, or whose energy fell by less
+        than rel_tol * E over the last `window` iterations of the call.
+        Returns the number of iterations run."""
+        n = C.c_int(0)
+        call("cvt_lloyd_until", self._h, float(omega), float(rel_tol), int(window),
+             float(target), int(max_iter), C.byref(n))
+        return n.value
+
     def history(self):
         n = call("cvt_history_length", self._h)
         out = np.zeros((max(n, 1), 2), np.float64)
@@ -177,7 +188,7 @@ class Engine:
         ms = np.zeros(8, np.float64)
         call("cvt_profile", self._h, int(iterations), float(omega), _out_ptr(ms, np.float64, 8))
         return {"jfa_pass": ms[0], "jfa_first": ms[2], "moments": ms[1], "update": ms[3],
-                "step": ms[5], "jfa_passes": int(ms[6]), "tile": int(ms[7])}
+                "jfa": ms[4], "step": ms[5], "jfa_passes": int(ms[6]), "tile": int(ms[7])}
 
     def synchronize(self):
         call("cvt_synchronize", self._h)

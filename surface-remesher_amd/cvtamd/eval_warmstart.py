@@ -15,10 +15,11 @@ seeds the training never uses):
    the lower-energy run continues (plain Lloyd is the arbiter and the
    fallback) until the target.  Both trials are inside the learned time.
 
-Both runs check the energy history every CHUNK iterations (one graph launch
-per iteration, one history read per check), so they pay the same checking
-cost; the iteration counts are exact (from the per-iteration history).
-Inputs are resident on the device before either clock starts.
+Every run stops itself on the device (cvt_lloyd_until: the update kernel
+tests the rule after each iteration and the later iterations return at
+entry), so plain and learned runs pay the same, small checking cost and the
+iteration counts are exact.  Inputs are resident on the device before either
+clock starts.
 """
 from __future__ import annotations
 
@@ -39,29 +40,19 @@ CAP = 2000
 
 
 def converge(e, cap=CAP):
-    """Plain Lloyd until the 10-iteration relative decrease is below CONV; energies."""
+    """Plain Lloyd until the 10-iteration relative decrease is below CONV
+    (at most cap iterations); the energies."""
     e.clear_history()
-    while True:
-        e.lloyd(CHUNK)
-        h = e.history()[:, 0]
-        if len(h) >= 2 * CHUNK and (h[-CHUNK - 1] - h[-1]) / h[-1] < CONV:
-            return h
-        if len(h) >= cap:
-            return h
+    e.lloyd_until(cap, rel_tol=CONV, window=CHUNK)
+    return e.history()[:, 0]
 
 
 def run_to(e, target, cap, sync):
-    """Lloyd in chunks until an energy <= target; (iterations, reached)."""
-    done = 0
-    while done < cap:
-        e.lloyd(CHUNK)
-        done += CHUNK
-        h = e.history()[:, 0]               # synchronises
-        hit = np.nonzero(h <= target)[0]
-        if len(hit):
-            return int(hit[0]) + 1, True
-    sync()
-    return done, False
+    """Lloyd until an energy <= target (at most cap iterations);
+    (iterations, reached).  The call synchronises once, at its end."""
+    n = e.lloyd_until(cap, target=target)
+    h = e.history()[:, 0]
+    return n, bool(len(h) and h[-1] <= target)
 
 
 def time_to_tol(e, net, rho, xy0, cap=CAP, trial=TRIAL):

@@ -84,3 +84,37 @@ def test_batch_config4_shape_runs(gpu, orc):
             assert h.shape == (3, 2) and np.all(np.diff(h[:, 0]) < 0)
             _, e_ref, _ = orc.lloyd(w.density, w.sites, 3)
             assert np.allclose(h[:, 0], e_ref, rtol=1e-5)
+
+
+def test_batch_config4_full(gpu, orc):
+    """The whole batched workload: 64 patches (sides log-uniform in
+    [256, 2048]) with the tuned moment tiles.  After 3 iterations every patch
+    matches a single-patch context run with the same tile bit for bit (sites,
+    energy history, labels); energies match the oracle's Lloyd; the labels of
+    patches up to 768^2 match the oracle's JFA(+1) of their final sites."""
+    import cvtamd
+    from cvtamd import autotune
+    from cvtamd import workloads as wl
+    ws = wl.config4(4)
+    assert len(ws) == 64
+    iters = 3
+    sizes = [(w.n, w.n) for w in ws]
+    with cvtamd.Batch(sizes, [w.num_sites for w in ws]) as b:
+        for k, w in enumerate(ws):
+            b.set_density(k, w.density)
+            b.set_sites(k, w.sites)
+        tiles = autotune.apply_batch(b, sizes, [w.num_sites for w in ws])
+        b.lloyd(iters)
+        b.synchronize()
+        res = [(b.sites(k), b.history(k)) for k in range(len(ws))]
+        b.voronoi()
+        labs = [b.labels(k) for k in range(len(ws))]
+    for k, w in enumerate(ws):
+        s1, h1, l1 = run_single(w.n, w.density, w.sites, iters, tiles[k])
+        assert np.array_equal(res[k][0], s1), k
+        assert np.array_equal(res[k][1], h1), k
+        assert np.array_equal(labs[k], l1), k
+        _, e_ref, _ = orc.lloyd(w.density, w.sites, iters)
+        assert np.allclose(res[k][1][:, 0], e_ref, rtol=1e-5), k
+        if w.n <= 768:
+            assert np.array_equal(labs[k], orc.jfa(w.n, w.n, res[k][0])), k

@@ -59,6 +59,10 @@ def check_moments(m, ref, W_tol=1e-6):
     (260, 130, 5000, 12),     # 40 KB site table, width not a multiple of 256
     (512, 384, 8000, 10),     # 160 KB site table on a small grid
     (2560, 1700, 8000, 13),   # runs layout over chains of unequal length
+    (704, 608, 20470, 17),    # persistent JFA: the largest table its LDS holds
+    (704, 608, 20471, 18),    # persistent JFA: just too large, split table
+    (256, 320, 900, 19),      # persistent JFA, one segment per row: four run groups per item
+    (32, 40, 20, 20),         # persistent JFA on the narrowest grid it takes
 ])
 def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
     xy = rand_sites(seed, W, H, S)
