@@ -470,8 +470,8 @@ def main():
     # per-kernel durations (HIP events between launches, same stream)
     prof = eng.profile(args.profile_iters, 1.0)
     P = prof["jfa_passes"]
-    jfa_bytes = 8.0 * n * n                       # read + write int32 labels
-    jfa_gbs = jfa_bytes / (prof["jfa_pass"] * 1e-3) / 1e9
+    kernel, jfa_bytes, launch_ms = "k_jfa_quad", 8.0 * n * n, prof["jfa_pass"]
+    jfa_gbs = jfa_bytes / (launch_ms * 1e-3) / 1e9
     iter_bytes = n * n * (8.0 * P + 8.0)           # N^2 (8P + 8)
     iter_gbs = iter_bytes / (ms_step * 1e-3) / 1e9
     eng.close()
@@ -493,14 +493,14 @@ def main():
                    "moment_tile": prof["tile"], "seed": args.seed},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_jfa_quad",
+            "kernel": kernel,
             "achieved": round(jfa_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(jfa_gbs / HBM_PEAK_GBS, 4),
             "traffic": pmc_traffic(w.name),
             "algorithmic_bytes_per_launch": jfa_bytes,
-            "launch_ms": round(prof["jfa_pass"], 6),
+            "launch_ms": round(launch_ms, 6),
             "iteration_achieved": round(iter_gbs, 1),
             "iteration_frac": round(iter_gbs / HBM_PEAK_GBS, 4),
             "kernel_ms": {k: round(v, 6) for k, v in prof.items() if isinstance(v, float)},

@@ -18,7 +18,8 @@
  * (x + 0.5, y + 0.5).  Labels are int32 site indices, CVT_LABEL_NONE where no
  * site reaches.  The Voronoi map is JFA with steps s0, s0/2, ..., 1 plus one
  * extra step-1 pass (s0 = largest power of two below max(width, height)),
- * distance d = fmaf(dx, dx, dy*dy), ties to the lower index.
+ * distance d = fmaf(dy, dy, dx*dx) (dx*dx rounded first), ties to the
+ * lower index.
  *
  * Which reference interface each entry point replaces:
  *   cvt_create / cvt_destroy      gcvtInitialization / pbaCVDDeinitialization
@@ -35,6 +36,9 @@
  *                                 pbaCVDCalcEnergy + pbaCVDComputeCentroid +
  *                                 pbaCVDUpdateSites
  *                                 (source/gcvt.cu:1113-1121)
+ *   cvt_lloyd_until               the same loop with its stop rule
+ *                                 (source/gcvt.cu:1123-1140), tested on the
+ *                                 device
  *   cvt_energy                    pbaCVDCalcEnergy (source/gcvt.cu:1059-1083),
  *                                 in float64 and without its reduction bias
  *   cvt_compat_gcvt               gCVT (source/gcvt.cu:1087-1156)
@@ -134,10 +138,9 @@ int cvt_energy(cvt_ctx *ctx, double *energy, double *gradnorm);
 int cvt_synchronize(cvt_ctx *ctx);
 /* Runs `iterations` (1..1000) Lloyd steps eagerly with HIP events around
  * each phase and returns mean durations in ms: ms[0] one dense JFA pass (the
- * back-to-back sequence of passes 1 .. P-1 over P - 1; with the persistent
- * JFA, its single launch over P - 1), ms[1] the moment kernel, ms[2] the
- * first JFA pass (from the seed map; 0 with the persistent JFA, which holds
- * it), ms[3] update + finalize, ms[4] the whole JFA, ms[5] the whole step,
+ * back-to-back sequence of passes 1 .. P-1 over P - 1), ms[1] the moment
+ * kernel, ms[2] the first JFA pass (from the seed map), ms[3] update +
+ * finalize, ms[4] the whole JFA, ms[5] the whole step,
  * ms[6] the JFA pass count P, ms[7] the moment tile.
  * (No reference counterpart: the reference times stages with clock(),
  * source/main.cpp:181-242.) */

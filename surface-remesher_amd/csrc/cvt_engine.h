@@ -33,21 +33,9 @@ struct Ctx {
     uint32_t *first = nullptr;     // first JFA pass [H][W]: none except at the
                                    // receivers of the current seeds while a JFA runs
     int final_buf = 0;             // lab[final_buf] = labels of the last JFA
-    // persistent JFA (flow_eligible): one map per pass in `maps` (map 0 the
-    // seed scatter's output, map P-1 the labels), per-row completion flags
-    // [P][H] and the control words; lab[] and first are then unused
-    bool flow = false;
-    int npass = 0;
-    uint32_t *maps = nullptr, *flow_flags = nullptr, *flow_ctl = nullptr;
-    uint32_t *labels_map() const
-    {
-        return flow ? maps + (size_t)(npass - 1) * W * H : lab[final_buf];
-    }
+    uint32_t *labels_map() const { return lab[final_buf]; }
     // a map the JFA's result does not live in (scratch for exports)
-    uint32_t *spare_map() const
-    {
-        return flow ? maps + (size_t)(npass - 2) * W * H : lab[final_buf ^ 1];
-    }
+    uint32_t *spare_map() const { return lab[final_buf ^ 1]; }
     bool labels_fresh = false;
     float2 *sites = nullptr;       // [capS + 1], sentinel at capS
     int capS = 1;                  // max(maxS, 1)

@@ -120,13 +120,6 @@ int Ctx::check_flags()
     if (hipMemcpyAsync(f, dflags, sizeof f, hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
         return fail(CVT_ERR_HIP, "flag readback: %s", hipGetErrorString(hipGetLastError()));
-    if (f[FLAG_FLOW]) {
-        int z = 0;
-        hipMemcpyAsync(dflags + FLAG_FLOW, &z, sizeof z, hipMemcpyHostToDevice, stream);
-        hipStreamSynchronize(stream);
-        return fail(CVT_ERR_HIP, "a wait of the persistent JFA ran out of spins; its labels are "
-                                 "not valid");
-    }
     if (f[FLAG_FULL]) {
         int z = 0;
         hipMemcpyAsync(dflags + FLAG_FULL, &z, sizeof z, hipMemcpyHostToDevice, stream);
@@ -161,29 +154,6 @@ int Ctx::enqueue_jfa(hipEvent_t after_first)
 {
     const int np = jfa_num_passes(W, H);
     int step = jfa_first_step(W, H);
-    if (flow) {
-        // the whole JFA in one persistent launch
-        FlowArgs a;
-        a.maps = maps;
-        a.seed = seed;
-        a.sites = sites;
-        a.flags = flow_flags;
-        a.ctl = flow_ctl;
-        a.dflags = dflags;
-        a.stop = dstate + ST_STOP;
-        a.W = W;
-        a.H = H;
-        a.S = S;
-        a.np = np;
-        a.step0 = step;
-        a.table_f4 = jplan.table_f4;
-        a.none = none();
-        flow_phases(a);
-        launch_jfa_flow(stream, jplan, a);
-        if (after_first) hipEventRecord(after_first, stream);
-        seeds_valid = false;
-        return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "jfa launch failed");
-    }
     const uint32_t *in = seed;
     int cur = 0;
     // Pass 0 reads only the seed map: with three passes or more it scatters
@@ -346,8 +316,7 @@ Ctx::~Ctx()
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
     void *bufs[] = {rho, seed, lab[0], lab[1], first, sites, fixed, slots, ovf, ovf_head,
-                    blockpart, hist, dstate, dflags, scratch, eg, maps, flow_flags, flow_ctl,
-                    until, ran};
+                    blockpart, hist, dstate, dflags, scratch, eg, until, ran};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (own_stream && own) hipStreamDestroy(own_stream);
@@ -383,8 +352,6 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     c->W = W; c->H = H; c->maxS = max_sites; c->S = 0;
     c->capS = std::max(max_sites, 1);
     c->jplan = plan_jfa(W, H, c->capS);
-    c->flow = flow_eligible(c->jplan, W, H);
-    c->npass = jfa_num_passes(W, H);
     hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -397,16 +364,10 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     c->hist_cap = 1 << 16;
     const int nb = (int)std::max(1L, (ms + 255) / 256);
     bool ok = hipMalloc(&c->rho, sizeof(float) * n) == hipSuccess &&
-              hipMalloc(&c->seed, sizeof(uint32_t) * n) == hipSuccess;
-    if (ok && c->flow)
-        ok = hipMalloc(&c->maps, sizeof(uint32_t) * n * c->npass) == hipSuccess &&
-             hipMalloc(&c->flow_flags, sizeof(uint32_t) * (size_t)H * c->npass) == hipSuccess &&
-             hipMalloc(&c->flow_ctl, sizeof(uint32_t) * CTL_WORDS) == hipSuccess;
-    else if (ok)
-        ok = hipMalloc(&c->lab[0], sizeof(uint32_t) * n) == hipSuccess &&
-             hipMalloc(&c->lab[1], sizeof(uint32_t) * n) == hipSuccess &&
-             hipMalloc(&c->first, sizeof(uint32_t) * n) == hipSuccess;
-    ok = ok &&
+              hipMalloc(&c->seed, sizeof(uint32_t) * n) == hipSuccess &&
+              hipMalloc(&c->lab[0], sizeof(uint32_t) * n) == hipSuccess &&
+              hipMalloc(&c->lab[1], sizeof(uint32_t) * n) == hipSuccess &&
+              hipMalloc(&c->first, sizeof(uint32_t) * n) == hipSuccess &&
               hipMalloc(&c->sites, sizeof(float2) * (c->capS + 2)) == hipSuccess &&
               hipMalloc(&c->fixed, ms) == hipSuccess &&
               hipMalloc(&c->slots, sizeof(Slot) * 9 * ms) == hipSuccess &&
@@ -425,20 +386,9 @@ int cvt_create(cvt_ctx **out, int W, int H, int max_sites)
     }
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->seed, c->none(), n);
-    if (c->flow) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->maps, c->none(),
-                           n * c->npass);
-        // flags 0 and epoch 1: no row is complete before the first launch
-        hipMemsetAsync(c->flow_flags, 0, sizeof(uint32_t) * (size_t)H * c->npass, s);
-        uint32_t ctl[CTL_WORDS] = {0};
-        ctl[CTL_EPOCH] = 1;
-        hipMemcpyAsync(c->flow_ctl, ctl, sizeof ctl, hipMemcpyHostToDevice, s);
-        hipStreamSynchronize(s);
-    } else {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[0], c->none(), n);
-        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[1], c->none(), n);
-        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->first, c->none(), n);
-    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[0], c->none(), n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->lab[1], c->none(), n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, c->first, c->none(), n);
     hipMemsetAsync(c->rho, 0, sizeof(float) * n, s);
     hipMemsetAsync(c->fixed, 0, ms, s);
     hipMemsetAsync(c->slots, 0, sizeof(Slot) * 9 * ms, s);
@@ -753,14 +703,8 @@ int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms)
     for (auto &e : ev) hipEventDestroy(e);
     if (rc) return rc;
     c->labels_fresh = true;
-    if (c->flow) {
-        // one launch: its time per dense pass, and no separate first pass
-        ms[0] = (acc[0] + acc[1]) / ((double)iterations * (np - 1));
-        ms[2] = 0.0;
-    } else {
-        ms[0] = np > 1 ? acc[1] / ((double)iterations * (np - 1)) : acc[0] / iterations;
-        ms[2] = np > 1 ? acc[0] / iterations : 0.0;
-    }
+    ms[0] = np > 1 ? acc[1] / ((double)iterations * (np - 1)) : acc[0] / iterations;
+    ms[2] = np > 1 ? acc[0] / iterations : 0.0;
     ms[1] = acc[2] / iterations;
     ms[3] = acc[3] / iterations;
     ms[4] = (acc[0] + acc[1]) / iterations;

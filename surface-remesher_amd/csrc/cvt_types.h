@@ -30,9 +30,7 @@ struct __align__(8) OvfRec {
 };
 
 // device flags
-// (FLAG_FLOW: a wait of the persistent JFA ran out of spins; the labels of
-// that JFA are not valid)
-enum { FLAG_FULL = 1, FLAG_BAD = 2, FLAG_FLOW = 3, NUM_FLAGS = 4 };
+enum { FLAG_FULL = 1, FLAG_BAD = 2, NUM_FLAGS = 4 };
 // device state words
 // (ST_DONE counts the finished blocks of a site-update launch; the last block
 // to finish resets it.  ST_STOP, when set, makes every kernel of a Lloyd

@@ -41,54 +41,6 @@ void launch_jfa_seed_pass(hipStream_t st, const uint32_t *seed, uint32_t *out,
                           const float2 *sites, int S, int W, int H, int step,
                           const int *stop = nullptr);
 
-// ---------------------------------------------------------------------------
-// Persistent JFA: the whole JFA(+1) of one map in one launch.  Work items are
-// handed out in a fixed order by a ticket counter: the seed scatter, then the
-// runs of pass 1, the seed map's reset, the runs of pass 2, the first map's
-// reset, then the runs of passes 3 .. np-1.  An item waits only for what it
-// reads: a run of pass p for the rows of pass p-1 under its taps (per-row
-// completion flags), pass 1 and the seed reset for the whole scatter, the
-// first map's reset for the whole of pass 1.  Every item waits only on items
-// with smaller tickets, which have been taken by running teams, so the order
-// cannot deadlock whatever the number of resident workgroups.  Each pass
-// writes its own map (maps[p]), so a map is never rewritten while a launch
-// runs and no cache line of it can be stale.
-// ---------------------------------------------------------------------------
-enum { FLOW_SCATTER = 0, FLOW_PASS = 1, FLOW_SEED_RESET = 2, FLOW_FIRST_RESET = 3 };
-// control words (ctl): ticket counter, teams done, launch epoch, scatter
-// items done, pass-1 items done
-enum { CTL_TICKET = 0, CTL_EXIT = 1, CTL_EPOCH = 2, CTL_SCATTER = 3, CTL_PASS1 = 4, CTL_WORDS = 8 };
-#define CVT_FLOW_MAX_PHASES 20
-// items of the scatter and reset phases cover this many (site, offset) pairs
-// or sites (four per thread of a 256-thread team)
-#define CVT_FLOW_CHUNK 1024
-struct FlowPhase {
-    int start, count;          // tickets [start, start + count)
-    int kind, pass, step;
-};
-struct FlowArgs {
-    uint32_t *maps;            // [np][H][W]: map 0 the scatter's, map p pass p's output
-    uint32_t *seed;            // seed map [H][W]
-    const float2 *sites;
-    uint32_t *flags;           // [np][H]: epoch of the launch that completed row y of pass p
-    uint32_t *ctl;             // CTL_WORDS control words
-    int *dflags;               // FLAG_FLOW on a wait that ran out of spins
-    const int *stop;           // non-null: the launch returns at entry while *stop is set
-    int W, H, S, np, step0;
-    int total, nph;            // tickets, phases
-    int upi;                   // run groups (four rows of a chain, full width) per item
-    int table_f4;
-    uint32_t none;
-    FlowPhase ph[CVT_FLOW_MAX_PHASES];
-};
-// Whether a plan runs as one persistent launch (quad kernel with the site
-// table in LDS, whole or split; W a multiple of 32, so that no cache line
-// holds pixels of two rows; three passes or more).
-bool flow_eligible(const JfaPlan &p, int W, int H);
-// The ticket order for S sites (fills a.total, a.nph, a.ph, a.upi).
-void flow_phases(FlowArgs &a);
-void launch_jfa_flow(hipStream_t st, const JfaPlan &p, const FlowArgs &a);
-
 // A patch of a batch, as the batched kernels see it.  Maps are concatenated
 // (pix_off), site tables too (site_off; entry site_off + cap is the patch's
 // sentinel, so `none` = 8 * (site_off + cap)); labels are byte offsets into

@@ -27,7 +27,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "cvt_types.h"
 #include "jfa.h"
 
 #include <algorithm>
@@ -802,244 +801,6 @@ __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa
 }
 
 // ---------------------------------------------------------------------------
-// Persistent JFA (declared in jfa.h).  A workgroup holds TEAMS teams of four
-// waves; a team takes one ticket at a time (the next one is fetched while the
-// current item runs), waits until what the item reads is complete, runs it
-// and publishes its completion.  Completion is published after every wave of
-// the team has waited for its own map stores, which are device-scope
-// write-through stores: a flag that reads complete therefore implies that the
-// rows it covers are visible to every XCD.
-// ---------------------------------------------------------------------------
-#define FLOW_SPIN_LIMIT (1 << 20)
-
-__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(uint32_t *p, uint32_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t add_agent(uint32_t *p, uint32_t v)
-{
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// s_waitcnt vmcnt(0): every vector memory access of the wave, the map stores
-// included, has completed
-__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0xF70); }
-
-// Spin (wave-uniform) until every lane's flag reads `want`; lanes with
-// need == false take part without a load.  A wait that runs out of spins
-// raises FLAG_FLOW and returns; once the flag is up every later wait returns
-// at its next check, so a launch whose flags never complete still ends
-// promptly (with labels the host then reports as invalid).
-__device__ __forceinline__ void flow_wait(const uint32_t *p, bool need, uint32_t want, int *dflags)
-{
-    for (int spin = 1;; ++spin) {
-        const uint32_t v = need ? ld_agent(p) : want;
-        if (__ballot(v != want) == 0ull) return;
-        if ((spin & 255) == 0 &&
-            (spin >= FLOW_SPIN_LIMIT ||
-             __hip_atomic_load(dflags + FLAG_FLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            if ((threadIdx.x & 63) == 0) atomicOr(dflags + FLAG_FLOW, 1);
-            return;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-}
-
-// the receiver of (site i, offset r) and its value, exactly as k_jfa_seeds
-template <int TAB>
-__device__ __forceinline__ void flow_scatter_one(const FlowArgs &a, const SiteTab<TAB> &tab,
-                                                 __amdgpu_buffer_rsrc_t rseed,
-                                                 __amdgpu_buffer_rsrc_t rout, int pr)
-{
-    const int i = pr / 9, r = pr - 9 * (pr / 9);
-    const float2 s = a.sites[i];
-    const int qx = seed_pixel(s.x, a.W), qy = seed_pixel(s.y, a.H);
-    const int step = a.step0;
-    const int px = qx - (r % 3 - 1) * step, py = qy - (r / 3 - 1) * step;
-    if (px < 0 || px >= a.W || py < 0 || py >= a.H) return;
-    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-    double best = 0.0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const int ox = (k % 3 - 1) * step, oy = (k / 3 - 1) * step;
-        const int tx = (px + ox >= 0 && px + ox < a.W) ? px + ox : px;
-        const int ty = (py + oy >= 0 && py + oy < a.H) ? py + oy : py;
-        const uint32_t l =
-            __builtin_amdgcn_raw_buffer_load_b32(rseed, 4u * (uint32_t)(ty * a.W + tx), 0, 0);
-        const double key = cand(tab(l), l, fx, fy);
-        best = (k == 0) ? key : __builtin_fmin(best, key);
-    }
-    __builtin_amdgcn_raw_buffer_store_b32((uint32_t)__double2loint(best), rout,
-                                          4u * (uint32_t)(py * a.W + px), 0,
-                                          CVT_MAP_STORE_POLICY);
-}
-
-// pixel of site i's seed shifted by d * step (d = -1, 0, 1 per axis), or -1
-// off the grid
-__device__ __forceinline__ int flow_receiver(const FlowArgs &a, int i, int r, int step)
-{
-    const float2 s = a.sites[i];
-    const int x = seed_pixel(s.x, a.W) + (r % 3 - 1) * step;
-    const int y = seed_pixel(s.y, a.H) + (r / 3 - 1) * step;
-    return (x >= 0 && x < a.W && y >= 0 && y < a.H) ? y * a.W + x : -1;
-}
-
-__device__ __forceinline__ int flow_mode(int step)
-{
-    return step == 1 ? STEP_1 : step == 2 ? STEP_2 : STEP_ANY;
-}
-
-template <int ENTRIES, int TEAMS>
-__global__ __launch_bounds__(256 * TEAMS, TEAMS == 1 ? 4 : 1) void k_jfa_flow(const FlowArgs a)
-{
-    constexpr int TAB = ENTRIES > 0 ? TAB_LDS0 : TAB_SPLIT;
-    constexpr int NE = ENTRIES > 0 ? ENTRIES : -ENTRIES;
-    __shared__ float4 lds_tab[NE / 2];
-    __shared__ uint32_t t_slot[TEAMS][2];
-    __shared__ uint32_t t_bar[TEAMS];
-    // a stop rule held (cvt_lloyd_until): every workgroup returns before it
-    // touches the control words
-    if (a.stop && *a.stop) return;
-
-    SiteTab<TAB> tab;
-    tab.lds = (lds_cchar *)lds_tab;
-    tab.src = reinterpret_cast<const f32x4 *>(a.sites);
-    tab.base = 0;
-    if constexpr (ENTRIES > 0) {
-        tab.n4 = a.table_f4;
-    } else {
-        // entries 0 .. NE - 3 staged, then a zero pair; the rest gathered
-        constexpr int kept = NE - 2;
-        tab.n4 = kept / 2;
-        tab.g = buf_rsrc(a.sites + kept, 8u * (uint32_t)(2 * a.table_f4 - kept));
-        tab.split = 8u * (uint32_t)kept;
-    }
-    if (threadIdx.x < TEAMS) t_bar[threadIdx.x] = 0;
-    tab.stage();   // ends with a workgroup barrier
-
-    const int team = (int)(threadIdx.x >> 8);
-    const int wt = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);
-    const int lane = threadIdx.x & 63;
-    const int tid = threadIdx.x & 255;
-    const bool leader = wt == 0 && lane == 0;
-    const uint32_t epoch = __builtin_amdgcn_readfirstlane(ld_agent(a.ctl + CTL_EPOCH));
-    const long N = (long)a.W * a.H;
-    const uint32_t map_bytes = 4u * (uint32_t)N;
-    uint32_t gen = 0;
-    auto team_sync = [&]() {
-        if constexpr (TEAMS == 1) {
-            __syncthreads();
-        } else {
-            gen += 4;
-            if (lane == 0)
-                __hip_atomic_fetch_add(&t_bar[team], 1u, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            while (__hip_atomic_load(&t_bar[team], __ATOMIC_ACQUIRE,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP) < gen)
-                __builtin_amdgcn_s_sleep(1);
-        }
-    };
-
-    if (leader) t_slot[team][0] = add_agent(a.ctl + CTL_TICKET, 1u);
-    team_sync();
-    int cur = 0;
-    for (;;) {
-        const int t = __builtin_amdgcn_readfirstlane((int)t_slot[team][cur]);
-        if (t >= a.total) break;
-        uint32_t tn = 0;
-        if (leader) tn = add_agent(a.ctl + CTL_TICKET, 1u);
-        int ph = 0;
-        while (ph + 1 < a.nph && t >= a.ph[ph + 1].start) ++ph;
-        const int kind = a.ph[ph].kind, pass = a.ph[ph].pass, step = a.ph[ph].step;
-        const int item = t - a.ph[ph].start;
-
-        // rows of a pass item: run group g0 + (lane >> 2) ... (four output rows
-        // per group); its six tap rows per group for the wait
-        int rg_lo = 0, rg_hi = 0, chains = 1;
-        if (kind == FLOW_SCATTER) {
-            const __amdgpu_buffer_rsrc_t rseed = buf_rsrc(a.seed, map_bytes);
-            const __amdgpu_buffer_rsrc_t rout = buf_rsrc(a.maps, map_bytes);
-#pragma unroll
-            for (int kk = 0; kk < CVT_FLOW_CHUNK / 256; ++kk) {
-                const int pr = item * CVT_FLOW_CHUNK + kk * 256 + tid;
-                if (pr < 9 * a.S) flow_scatter_one<TAB>(a, tab, rseed, rout, pr);
-            }
-        } else if (kind == FLOW_SEED_RESET || kind == FLOW_FIRST_RESET) {
-            const bool seedr = kind == FLOW_SEED_RESET;
-            const uint32_t need = a.ph[seedr ? 0 : 1].count;
-            flow_wait(a.ctl + (seedr ? CTL_SCATTER : CTL_PASS1), lane == 0, need, a.dflags);
-            const __amdgpu_buffer_rsrc_t rc = buf_rsrc(seedr ? a.seed : a.maps, map_bytes);
-            const int n = seedr ? a.S : 9 * a.S;
-#pragma unroll
-            for (int kk = 0; kk < CVT_FLOW_CHUNK / 256; ++kk) {
-                const int e = item * CVT_FLOW_CHUNK + kk * 256 + tid;
-                if (e >= n) continue;
-                const int q = seedr ? flow_receiver(a, e, 4, 0) : flow_receiver(a, e / 9, e % 9, a.step0);
-                if (q >= 0)
-                    __builtin_amdgcn_raw_buffer_store_b32(a.none, rc, 4u * (uint32_t)q, 0,
-                                                          CVT_MAP_STORE_POLICY);
-            }
-        } else {
-            MapGeo m = map_geo(a.maps + (long)(pass - 1) * N, a.maps + (long)pass * N, nullptr,
-                               a.W, a.H, step, a.none);
-            const RunGeo g = run_geo(m);
-            chains = g.chains;
-            rg_lo = item * a.upi;
-            rg_hi = min(rg_lo + a.upi, g.chains * g.nq);
-            if (pass == 1) {
-                const uint32_t need = a.ph[0].count;
-                flow_wait(a.ctl + CTL_SCATTER, lane == 0, need, a.dflags);
-            } else {
-                // lane l checks tap row (l % 6) of run group rg_lo + l / 6
-                const int rg = rg_lo + lane / 6, ti = lane % 6;
-                const int y0 = (rg % g.chains) + 4 * (rg / g.chains) * step;
-                const int row = y0 + (ti - 1) * step;
-                const bool need = lane < 6 * (rg_hi - rg_lo) && row >= 0 && row < a.H;
-                flow_wait(a.flags + (long)(pass - 1) * a.H + (need ? row : 0), need, epoch,
-                          a.dflags);
-            }
-            const int u0 = rg_lo * g.nseg, u1 = rg_hi * g.nseg;
-            switch (flow_mode(step)) {
-            case STEP_1: wave_runs<STEP_1, TAB>(m, tab, g, u0 + wt, u1, 4, false); break;
-            case STEP_2: wave_runs<STEP_2, TAB>(m, tab, g, u0 + wt, u1, 4, false); break;
-            default: wave_runs<STEP_ANY, TAB>(m, tab, g, u0 + wt, u1, 4, false); break;
-            }
-        }
-        wait_vm();
-        if (leader) t_slot[team][cur ^ 1] = tn;
-        team_sync();
-        if (wt == 0) {
-            if (kind == FLOW_SCATTER) {
-                if (lane == 0) add_agent(a.ctl + CTL_SCATTER, 1u);
-            } else if (kind == FLOW_PASS) {
-                // lane l publishes output row (l % 4) of run group rg_lo + l / 4
-                const int rg = rg_lo + lane / 4, j = lane % 4;
-                const int row = (rg % chains) + (4 * (rg / chains) + j) * step;
-                if (lane < 4 * (rg_hi - rg_lo) && row < a.H)
-                    st_agent(a.flags + (long)pass * a.H + row, epoch);
-                if (pass == 1 && lane == 0) add_agent(a.ctl + CTL_PASS1, 1u);
-            }
-        }
-        cur ^= 1;
-    }
-    // the last team out resets the counters and advances the epoch for the
-    // next launch (every team has read the epoch and taken its last ticket)
-    if (leader) {
-        const uint32_t nteams = gridDim.x * TEAMS;
-        if (add_agent(a.ctl + CTL_EXIT, 1u) == nteams - 1) {
-            st_agent(a.ctl + CTL_TICKET, 0u);
-            st_agent(a.ctl + CTL_SCATTER, 0u);
-            st_agent(a.ctl + CTL_PASS1, 0u);
-            st_agent(a.ctl + CTL_EXIT, 0u);
-            st_agent(a.ctl + CTL_EPOCH, epoch + 1u);
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Batched pass: one launch over many patches.  Each workgroup takes one work
 // unit (patch, band of rows); patches run their passes right-aligned, so pass
 // k of the batch is pass k - (Pmax - P) of a patch with P passes (skipped
@@ -1292,75 +1053,6 @@ void launch_jfa_seed_pass(hipStream_t st, const uint32_t *seed, uint32_t *out,
     if (S > 0)
         hipLaunchKernelGGL(k_jfa_seeds, dim3((9 * S + 255) / 256), dim3(256), 0, st, seed, out,
                            sites, S, W, H, step, stop);
-}
-
-// ---------------------------------------------------------------------------
-// Persistent JFA: plan and launch
-// ---------------------------------------------------------------------------
-// LDS table sizes of the persistent kernel, in float2 entries: the largest
-// leaves 64 bytes of the 160 KB for the teams' ticket slots and barriers
-static const int kFlowEntries[3] = {2048, 5120, 20472};
-
-static int flow_entries(const JfaPlan &p)
-{
-    const int entries = 2 * p.table_f4;
-    for (int e : kFlowEntries)
-        if (entries <= e) return e;
-    return -kFlowEntries[2];
-}
-
-bool flow_eligible(const JfaPlan &p, int W, int H)
-{
-    // three passes or more: the first step is at least 2
-    return p.quad && p.lds_entries != 0 && W % 32 == 0 && std::max(W, H) > 2;
-}
-
-void flow_phases(FlowArgs &a)
-{
-    const int nseg = (a.W + 255) / 256;
-    a.upi = (4 + nseg - 1) / nseg;
-    const int C = CVT_FLOW_CHUNK;
-    const int nsc = (9 * a.S + C - 1) / C;
-    int n = 0, t = 0;
-    auto add = [&](int kind, int pass, int step, int count) {
-        FlowPhase &f = a.ph[n++];
-        f.start = t;
-        f.count = count;
-        f.kind = kind;
-        f.pass = pass;
-        f.step = step;
-        t += count;
-    };
-    add(FLOW_SCATTER, 0, a.step0, nsc);
-    for (int p = 1; p < a.np; ++p) {
-        const int step = std::max(a.step0 >> p, 1);
-        const int chains = std::min(step, a.H);
-        const int rows = (a.H + step - 1) / step;
-        const int groups = chains * ((rows + 3) / 4);
-        add(FLOW_PASS, p, step, (groups + a.upi - 1) / a.upi);
-        if (p == 1) add(FLOW_SEED_RESET, 0, 0, (a.S + C - 1) / C);
-        if (p == 2) add(FLOW_FIRST_RESET, 0, a.step0, nsc);
-    }
-    a.nph = n;
-    a.total = t;
-}
-
-template <int E, int TEAMS>
-static void launch_flow_e(hipStream_t st, const FlowArgs &a, int grid)
-{
-    hipLaunchKernelGGL((k_jfa_flow<E, TEAMS>), dim3(grid), dim3(256 * TEAMS), 0, st, a);
-}
-
-void launch_jfa_flow(hipStream_t st, const JfaPlan &p, const FlowArgs &a)
-{
-    const int e = flow_entries(p);
-    const int cu = g_num_cu > 0 ? g_num_cu : 256;
-    switch (e) {
-    case 2048: launch_flow_e<2048, 4>(st, a, cu); break;
-    case 5120: launch_flow_e<5120, 4>(st, a, cu); break;
-    case 20472: launch_flow_e<20472, 4>(st, a, cu); break;
-    default: launch_flow_e<-20472, 4>(st, a, cu); break;
-    }
 }
 
 }  // namespace cvt

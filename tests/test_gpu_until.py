@@ -34,10 +34,10 @@ def host_polled(w, stop, cap):
         return len(h), h, e.sites()
 
 
-@pytest.mark.parametrize("name", ["config1", "config2_1024"])
-def test_relative_decrease_rule(gpu, name):
+@pytest.mark.parametrize("name,tol", [("config1", 1e-4), ("config2_1024", 1e-3)])
+def test_relative_decrease_rule(gpu, name, tol):
     w = workload(name)
-    tol, win, cap = 1e-4, 10, 400
+    win, cap = 10, 400
     with engine(w) as e:
         n = e.lloyd_until(cap, rel_tol=tol, window=win)
         h = e.history()[:, 0]
