@@ -197,6 +197,19 @@ int cvt_ref_init_sites(int16_t *voronoi, const float *density, const uint8_t *ma
  * triangle 3 vertex indices per triangle. */
 int cvt_rasterize(const double *points, const double *weight, int num_point, const int *triangle,
                   int num_tri, float *density, double scale, int n);
+/* cvt_rasterize for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc); it
+ * has no counterpart in the reference.  Same arguments and the same result
+ * bit for bit, but every buffer must be device-accessible (device, pinned or
+ * managed memory; CVT_ERR_ARG otherwise), and the call allocates nothing:
+ * its working memory is a static device array of about 14 MB that each
+ * device holds until the process ends.  Limits: n <= 4096, num_tri <= 262144
+ * and at most 2097152 (triangle, 16x16-pixel tile) pairs (CVT_ERR_ARG /
+ * CVT_ERR_CAPACITY beyond).  Calls that overlap on one device share that
+ * array and race; like cvt_rasterize it runs on the legacy default stream
+ * and returns when the density is written. */
+int cvt_rasterize_noalloc(const double *points, const double *weight, int num_point,
+                          const int *triangle, int num_tri, float *density, double scale, int n);
 
 /* CDT input points (the site scan of delaunayInput, source/delaunay.h:48-58):
  * pixels (x, y) whose label is their own coordinates and whose mask byte is

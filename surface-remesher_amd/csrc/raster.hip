@@ -120,13 +120,19 @@ __global__ void k_tile_count(const TriBox *__restrict__ box, int nt, int tiles_x
         for (int x = tx0; x <= tx1; ++x) atomicAdd(&count[y * tiles_x + x], 1);
 }
 
-// (triangle, tile) pairs: each triangle writes its index into its tiles' lists
-__global__ void k_tile_fill(const TriBox *__restrict__ box, int nt, int tiles_x,
+// (triangle, tile) pairs: each triangle writes its index into its tiles' lists.
+// With a list of `cap` entries (cap >= 0) and more pairs than that, nothing
+// is written and *overflow is raised.
+__global__ void k_tile_fill(const TriBox *__restrict__ box, int nt, int tiles_x, int ntiles,
                             const int *__restrict__ start, int *__restrict__ cursor,
-                            int *__restrict__ list)
+                            int *__restrict__ list, int cap, int *__restrict__ overflow)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
+    if (cap >= 0 && start[ntiles] > cap) {
+        if (i == 0) *overflow = 1;
+        return;
+    }
     const TriBox b = box[i];
     if (b.x0 > b.x1 || b.y0 > b.y1) return;
     int tx0, tx1, ty0, ty1;
@@ -146,8 +152,12 @@ __global__ __launch_bounds__(256) void k_raster(const double *__restrict__ pts,
                                                 const int *__restrict__ list,
                                                 const int *__restrict__ start,
                                                 const int *__restrict__ alw, int nalw, int tiles_x,
-                                                double scale, int n, float *__restrict__ density)
+                                                double scale, int n, float *__restrict__ density,
+                                                const int *__restrict__ dev_nalw,
+                                                const int *__restrict__ overflow)
 {
+    if (overflow && *overflow) return;
+    if (dev_nalw) nalw = *dev_nalw;
     const int tile = blockIdx.x;
     const int tx = (tile % tiles_x) * RT + (threadIdx.x & (RT - 1));
     const int ty = (tile / tiles_x) * RT + (threadIdx.x / RT);
@@ -243,9 +253,9 @@ extern "C" int cvt_rasterize(const double *points, const double *weight, int num
     RCHK(hipMalloc(&d_list, sizeof(int) * (size_t)std::max(total, 1)), "alloc");
     if (num_tri > 0)
         hipLaunchKernelGGL(k_tile_fill, dim3((num_tri + 255) / 256), dim3(256), 0, st, d_box,
-                           num_tri, tiles_x, d_start, d_cursor, d_list);
+                           num_tri, tiles_x, ntiles, d_start, d_cursor, d_list, -1, nullptr);
     hipLaunchKernelGGL(k_raster, dim3(ntiles), dim3(256), 0, st, d_pts, d_wt, d_tri, d_list,
-                       d_start, d_alw, nalw, tiles_x, scale, n, d_den);
+                       d_start, d_alw, nalw, tiles_x, scale, n, d_den, nullptr, nullptr);
     RCHK(hipGetLastError(), "raster launch");
     RCHK(hipMemcpyAsync(density, d_den, sizeof(float) * npx, hipMemcpyDefault, st), "download");
     RCHK(hipStreamSynchronize(st), "raster");
@@ -256,3 +266,152 @@ done:
         if (b) (void)hipFree(b);
     return rc;
 }
+
+// ---------------------------------------------------------------------------
+// cvt_rasterize_noalloc: cvt_rasterize for callers that cannot allocate.  All
+// of its working memory (triangle boxes, the thin-triangle list, per-tile
+// counts and list starts, the (triangle, tile) lists and three status words)
+// lives in one static device array, so the call allocates nothing.  Each
+// device holds its own copy of the array; its address is looked up once per
+// device with hipGetSymbolAddress.  Every call on a device shares that
+// device's array: calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+namespace cvt {
+
+#define NOALLOC_MAX_N 4096
+#define NOALLOC_MAX_TRI (1 << 18)
+#define NOALLOC_MAX_PAIRS (1 << 21)
+#define NOALLOC_TILES ((NOALLOC_MAX_N / RT) * (NOALLOC_MAX_N / RT))
+
+// byte offsets of the parts (each a multiple of 256)
+enum : size_t {
+    NA_BOX = 0,
+    NA_ALWAYS = NA_BOX + sizeof(TriBox) * (size_t)NOALLOC_MAX_TRI,
+    NA_ALW = NA_ALWAYS + (size_t)NOALLOC_MAX_TRI,
+    NA_COUNT = NA_ALW + sizeof(int) * (size_t)NOALLOC_MAX_TRI,
+    NA_CURSOR = NA_COUNT + sizeof(int) * (size_t)NOALLOC_TILES,
+    NA_START = NA_CURSOR + sizeof(int) * (size_t)NOALLOC_TILES,
+    NA_LIST = NA_START + sizeof(int) * ((size_t)NOALLOC_TILES + 64),
+    NA_STATUS = NA_LIST + sizeof(int) * (size_t)NOALLOC_MAX_PAIRS,   // bad, overflow, nalw
+    NA_BYTES = NA_STATUS + 256,
+};
+static_assert(NA_ALWAYS % 256 == 0 && NA_ALW % 256 == 0 && NA_COUNT % 256 == 0 &&
+                  NA_CURSOR % 256 == 0 && NA_START % 256 == 0 && NA_LIST % 256 == 0 &&
+                  NA_STATUS % 256 == 0,
+              "scratch parts must stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+#define NOALLOC_MAX_DEVICES 64
+static unsigned char *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// the device address of this device's scratch (looked up on its first use)
+static int scratch_of(int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(CVT_ERR_ARG, "device %d beyond the %d the no-alloc path supports", dev,
+                    NOALLOC_MAX_DEVICES);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CVT_ERR_HIP, "scratch address: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CVT_OK;
+}
+
+// the thin triangles, appended in any order (a pixel keeps the lowest-index
+// hit whatever the order of its candidates)
+static __global__ void k_always_append(const uint8_t *__restrict__ always, int nt,
+                                       int *__restrict__ alw, int *__restrict__ nalw)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nt && always[i]) alw[atomicAdd(nalw, 1)] = i;
+}
+
+static __global__ void k_zero_i32(int *__restrict__ p, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// true when p is memory the device can read and write (device, pinned or
+// managed memory); pageable host memory is not
+static bool device_accessible(const void *p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.devicePointer != nullptr;
+}
+
+}  // namespace cvt
+
+extern "C" int cvt_rasterize_noalloc(const double *points, const double *weight, int num_point,
+                                     const int *triangle, int num_tri, float *density,
+                                     double scale, int n)
+{
+    clear_error();
+    if (n < 1 || n > NOALLOC_MAX_N)
+        return fail(CVT_ERR_ARG, "n = %d outside [1, %d] (the no-alloc limit)", n, NOALLOC_MAX_N);
+    if (num_point < 0 || num_tri < 0) return fail(CVT_ERR_ARG, "negative sizes");
+    if (num_tri > NOALLOC_MAX_TRI)
+        return fail(CVT_ERR_ARG, "%d triangles above the no-alloc limit %d", num_tri,
+                    NOALLOC_MAX_TRI);
+    if (!density || (num_point > 0 && (!points || !weight)) || (num_tri > 0 && !triangle))
+        return fail(CVT_ERR_ARG, "null array");
+    if (!(scale > 0.0) || !isfinite(scale)) return fail(CVT_ERR_ARG, "scale %g", scale);
+    if (!device_accessible(density) ||
+        (num_point > 0 && (!device_accessible(points) || !device_accessible(weight))) ||
+        (num_tri > 0 && !device_accessible(triangle)))
+        return fail(CVT_ERR_ARG, "the no-alloc rasterizer needs device-accessible buffers");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(CVT_ERR_HIP, "no current device");
+    unsigned char *base = nullptr;
+    int rc = scratch_of(dev, &base);
+    if (rc) return rc;
+    TriBox *box = reinterpret_cast<TriBox *>(base + NA_BOX);
+    uint8_t *always = base + NA_ALWAYS;
+    int *alw = reinterpret_cast<int *>(base + NA_ALW);
+    int *count = reinterpret_cast<int *>(base + NA_COUNT);
+    int *cursor = reinterpret_cast<int *>(base + NA_CURSOR);
+    int *start = reinterpret_cast<int *>(base + NA_START);
+    int *list = reinterpret_cast<int *>(base + NA_LIST);
+    int *status = reinterpret_cast<int *>(base + NA_STATUS);   // bad, overflow, nalw
+    const int tiles_x = (n + RT - 1) / RT, ntiles = tiles_x * tiles_x;
+    hipStream_t st = nullptr;   // the legacy default stream, as cvt_rasterize
+    // the counts, cursors and status words left by the last call start at zero
+    hipLaunchKernelGGL(k_zero_i32, dim3(256), dim3(256), 0, st, count, (long)ntiles);
+    hipLaunchKernelGGL(k_zero_i32, dim3(256), dim3(256), 0, st, cursor, (long)ntiles);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, status, 4L);
+    if (num_tri > 0) {
+        const dim3 g((num_tri + 255) / 256), b(256);
+        hipLaunchKernelGGL(k_tri_boxes, g, b, 0, st, points, num_point, triangle, num_tri, scale,
+                           n, box, always, status);
+        hipLaunchKernelGGL(k_always_append, g, b, 0, st, always, num_tri, alw, status + 2);
+        hipLaunchKernelGGL(k_tile_count, g, b, 0, st, box, num_tri, tiles_x, count);
+    }
+    launch_exclusive_scan(st, count, ntiles, start);
+    if (num_tri > 0)
+        hipLaunchKernelGGL(k_tile_fill, dim3((num_tri + 255) / 256), dim3(256), 0, st, box,
+                           num_tri, tiles_x, ntiles, start, cursor, list, NOALLOC_MAX_PAIRS,
+                           status + 1);
+    hipLaunchKernelGGL(k_raster, dim3(ntiles), dim3(256), 0, st, points, weight, triangle, list,
+                       start, alw, 0, tiles_x, scale, n, density, status + 2, status + 1);
+    int h[3] = {0, 0, 0};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, status, sizeof h, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(CVT_ERR_HIP, "no-alloc raster: %s", hipGetErrorString(e));
+    if (h[0]) return fail(CVT_ERR_ARG, "a triangle indexes a vertex outside [0, %d)", num_point);
+    if (h[1])
+        return fail(CVT_ERR_CAPACITY, "more than %d (triangle, tile) pairs (the no-alloc limit)",
+                    NOALLOC_MAX_PAIRS);
+    return CVT_OK;
+}
+

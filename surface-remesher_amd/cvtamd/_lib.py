@@ -61,6 +61,7 @@ SIGNATURES = {
     "cvt_compat_gcvt": (_i, [_vp, _vp, _vp, _i, _i, _i, C.POINTER(_i), C.POINTER(_i)]),
     "cvt_ref_init_sites": (_i, [_vp, _vp, _vp, _i, _i, _vp, C.POINTER(C.c_longlong)]),
     "cvt_rasterize": (_i, [_vp, _vp, _i, _vp, _i, _vp, _d, _i]),
+    "cvt_rasterize_noalloc": (_i, [_vp, _vp, _i, _vp, _i, _vp, _d, _i]),
     "cvt_extract_points": (_i, [_vp, _vp, _i, _d, _d, _d, _vp, C.c_long, C.POINTER(C.c_long)]),
     "cvt_locate_points": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _vp]),
     "cvt_vertex_weights": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
