@@ -229,12 +229,14 @@ int Batch::plan()
     units.clear();
     for (int p = 0; p < np; ++p) {
         const BatchPatch &P = hp[p];
+        // a pass holds at least H/4 * nseg runs (four rows of one 256-pixel
+        // segment each): at most one part per 16 runs keeps every wave busy
         const int nseg = (P.W + 255) / 256;
-        const int min_rows = std::max(1, 16 / nseg);
         int nb = (int)std::max<long>(1, ((long)P.W * P.H + target / 2) / target);
-        nb = std::min(nb, std::max(1, P.H / min_rows));
+        nb = std::min(nb, std::max(1, P.H * nseg / 64));
         for (int b = 0; b < nb; ++b)
-            units.push_back(BatchUnit{p, (int)((long)P.H * b / nb), (int)((long)P.H * (b + 1) / nb), 0});
+            units.push_back(BatchUnit{p, (int)((long)P.H * b / nb), (int)((long)P.H * (b + 1) / nb),
+                                      b, nb, {0, 0, 0}});
     }
     // moment tiles and update blocks
     ntiles = 0;

@@ -55,9 +55,10 @@ struct BatchPatch {
     float rho_scale;           // 2^-k
     float pad;
 };
-// a workgroup's share of a JFA pass: rows [y0, y1) of one patch
+// a workgroup's share of a JFA pass: part b of the nb parts of one patch,
+// either its rows [y0, y1) (bands) or the b-th share of the pass's runs
 struct BatchUnit {
-    int patch, y0, y1, pad;
+    int patch, y0, y1, b, nb, pad[3];
 };
 
 // Pass `pass` (0 .. pmax-1) of the batched JFA over all units; lds_bytes = 0

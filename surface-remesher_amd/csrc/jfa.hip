@@ -250,9 +250,10 @@ struct MapGeo {
     uint32_t none;
 };
 
+// (y is wave-uniform: the product is taken in scalar registers)
 __device__ __forceinline__ uint32_t row_off(const MapGeo &m, int y)
 {
-    return __builtin_amdgcn_readfirstlane((uint32_t)y * m.row_bytes);
+    return (uint32_t)__builtin_amdgcn_readfirstlane(y) * m.row_bytes;
 }
 
 // the labels one tap row contributes to a segment: the left, centre and
@@ -498,8 +499,9 @@ __device__ __forceinline__ void band_of(int g, int G, int N, int &a, int &b)
 {
     const int q8 = G >> 3, r8 = G & 7, xcd = g & 7;
     const int band = xcd * q8 + min(xcd, r8) + (g >> 3);
-    a = (int)((long)N * band / G);
-    b = (int)((long)N * (band + 1) / G);
+    // N * (band + 1) < 2^32: N is at most a map's run or row count (<= 2^16)
+    a = (int)((uint32_t)N * (uint32_t)band / (uint32_t)G);
+    b = (int)((uint32_t)N * (uint32_t)(band + 1) / (uint32_t)G);
 }
 
 // Runs.  The rows of a map fall into chains of rows one step apart
@@ -520,18 +522,34 @@ __device__ __forceinline__ RunGeo run_geo(const MapGeo &m)
     g.k = m.step;
     g.nseg = (m.W + 255) >> 8;
     g.chains = min(m.step, m.H);
-    const int n = (m.H + m.step - 1) / m.step;     // longest chain
+    // longest chain (steps are powers of two)
+    const int n = (m.H + m.step - 1) >> __builtin_ctz((unsigned)m.step);
     g.nq = (n + 3) >> 2;
     g.count = g.chains * g.nq * g.nseg;
     return g;
 }
 
+// u / d and u % d for wave-uniform u and d: shifts when d is a power of two
+// (the segment count of the usual grids, the chain count of every step up to
+// the grid's height), a division otherwise
+__device__ __forceinline__ void divmod_u(int u, int d, int &quo, int &rem)
+{
+    if ((d & (d - 1)) == 0) {
+        quo = u >> __builtin_ctz((unsigned)d);
+        rem = u & (d - 1);
+    } else {
+        quo = u / d;
+        rem = u - quo * d;
+    }
+}
+
 // first centre row and segment of run u (wave-uniform)
 __device__ __forceinline__ void run_of(const RunGeo &g, int u, int &y, int &seg)
 {
-    seg = u % g.nseg;
-    const int qc = u / g.nseg;
-    const int c = qc % g.chains, q = qc / g.chains;
+    u = __builtin_amdgcn_readfirstlane(u);
+    int qc, c, q;
+    divmod_u(u, g.nseg, qc, seg);
+    divmod_u(qc, g.chains, q, c);
     y = __builtin_amdgcn_readfirstlane(c + 4 * q * g.k);
     seg = __builtin_amdgcn_readfirstlane(seg);
 }
@@ -828,19 +846,25 @@ __global__ __launch_bounds__(1024) void k_jfa_batch(const BatchPatch *__restrict
     uint32_t *clear = (local == 1) ? seed + P.pix_off : nullptr;
     const uint32_t none = 8u * (uint32_t)(P.site_off + P.cap);
     const MapGeo m = map_geo(in, out, clear, P.W, P.H, step, none);
+    // part b of the pass's runs (the planner gives a patch at most one part
+    // per 16 runs, one per wave)
+    const RunGeo g = run_geo(m);
+    const int nwaves = blockDim.x >> 6;
+    const int u0 = (int)((long)g.count * u.b / u.nb), u1 = (int)((long)g.count * (u.b + 1) / u.nb);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (SITES_IN_LDS) {
         SiteTab<TAB_LDSB> tab;
         tab.lds = (lds_cchar *)lds_dyn;
         tab.src = reinterpret_cast<const f32x4 *>(sites + P.site_off);
         tab.n4 = (P.cap + 2) / 2;
         tab.base = 8u * (uint32_t)P.site_off;
-        quad_rows<MODE, TAB_LDSB>(m, tab, u.y0, u.y1);
+        wave_runs<MODE, TAB_LDSB>(m, tab, g, u0 + wave, u1, nwaves, true);
     } else {
         SiteTab<TAB_GLOBAL> tab;
         tab.g = buf_rsrc(sites, 0xffffffffu);
         tab.base = 0;
         tab.n4 = 0;
-        quad_rows<MODE, TAB_GLOBAL>(m, tab, u.y0, u.y1);
+        wave_runs<MODE, TAB_GLOBAL>(m, tab, g, u0 + wave, u1, nwaves, false);
     }
 }
 
@@ -958,21 +982,17 @@ static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, ui
     }
 }
 
-// Layout by step, as measured at 2048^2 / 16K sites (us per pass, bands of
-// rows -> runs): steps 32 .. 1 gain (13.9 -> 13.3 at 32, 14.8 -> 11.8 at 1),
-// and so do the two largest dense steps, max(W, H) / 4 and / 8 (16.4 -> 14.4
-// at 512, 17.3 -> 14.1 at 256); steps 128 and 64 do not (13.7 -> 14.8,
-// 13.6 -> 16.2).
-// Runs also need work for every wave: a run is four items, so on small grids
-// (512^2: 256 runs at step 1 for 1024 waves) they would idle most waves.
-static const int kRunsMaxStep = 32;
-// With the split site table (C3) the pass is bound by the remaining gathers,
-// and runs (fewer label loads) win at every step: 683 -> 701 iter/s at
-// 4096^2 / 64K sites.
+// Layout by step.  Runs are evaluated as their tap rows stream in (each
+// label's dx^2 serves up to three output rows), so they take fewer VALU
+// operations per pixel than bands at every step: at 2048^2 / 16K sites the
+// banded steps 128 and 64 took 13.3 us against 10.8 for the runs of the other
+// steps >= 4 (rocprof means), and runs at every step measured 6547 against
+// 6510 iter/s.  Runs need work for every wave, though: a run is four rows of
+// a 256-pixel segment, so on small grids (512^2: 256 runs at step 1 for 1024
+// waves) bands keep more waves busy (15 870 against 13 170 iter/s with runs
+// whenever they fill a quarter of the waves).
 static bool use_runs(const JfaPlan &p, int W, int H, int step)
 {
-    if (p.lds_entries >= 0 && step > kRunsMaxStep && 8L * step < (long)std::max(W, H))
-        return false;
     const long n = (H + step - 1) / step;
     const long runs = (long)std::min(step, H) * ((n + 3) / 4) * ((W + 255) / 256);
     return runs >= (long)p.grid * (p.threads / 64);
