@@ -336,6 +336,20 @@ int cvt_unet_synchronize(cvt_unet *net);
  * sites respectively; more is reported as CVT_ERR_CAPACITY. */
 int cvt_set_tile(cvt_ctx *ctx, int tile);
 int cvt_get_tile(cvt_ctx *ctx);
+/* Launch shape of the JFA passes, a speed choice (the labels do not depend
+ * on it).  runs_mask < 0: runs of four rows one step apart where every wave
+ * of the launch gets one, bands of rows elsewhere (the default); otherwise
+ * bit k set = runs at step 2^k, clear = bands.  blocks_per_cu: workgroups
+ * launched per CU (1 .. 16), 0 = the built-in grid (four 256-thread
+ * workgroups per CU while the site table takes at most 40 KB of LDS, one
+ * 1024-thread workgroup per CU above).  Replaces the fixed per-size launch
+ * tables of pba2DInitializeInput (source/gcvt.cu:842-847); set by the
+ * autotuner.  Takes effect at the next cvt_voronoi / cvt_lloyd call.
+ * cvt_get_jfa_shape reports the steps that take runs under the current
+ * shape (bit k for step 2^k; 0 where W is not a multiple of 4, which has a
+ * per-pixel layout of its own) and the blocks_per_cu last set. */
+int cvt_set_jfa_shape(cvt_ctx *ctx, int runs_mask, int blocks_per_cu);
+int cvt_get_jfa_shape(cvt_ctx *ctx, int *runs_mask, int *blocks_per_cu);
 
 #ifdef __cplusplus
 }

@@ -730,6 +730,29 @@ int cvt_set_tile(cvt_ctx *ctx, int tile)
     return set_tile_internal(c, tile);
 }
 
+int cvt_set_jfa_shape(cvt_ctx *ctx, int runs_mask, int blocks_per_cu)
+{
+    CHECK_CTX(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (blocks_per_cu < 0 || blocks_per_cu > 16)
+        return fail(CVT_ERR_ARG, "blocks_per_cu %d (0 .. 16)", blocks_per_cu);
+    if (runs_mask < 0) runs_mask = JFA_AUTO;
+    if (c->jplan.runs_mask != runs_mask || c->jplan.blocks_per_cu != blocks_per_cu) {
+        c->jplan = plan_jfa(c->W, c->H, c->capS, runs_mask, blocks_per_cu);
+        c->graph_valid = false;
+    }
+    return CVT_OK;
+}
+
+int cvt_get_jfa_shape(cvt_ctx *ctx, int *runs_mask, int *blocks_per_cu)
+{
+    CHECK_CTX(ctx);
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (runs_mask) *runs_mask = jfa_runs_mask(c->jplan, c->W, c->H);
+    if (blocks_per_cu) *blocks_per_cu = c->jplan.blocks_per_cu;
+    return CVT_OK;
+}
+
 int cvt_get_tile(cvt_ctx *ctx)
 {
     CHECK_CTX(ctx);

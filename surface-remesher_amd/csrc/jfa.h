@@ -13,12 +13,22 @@ namespace cvt {
 // first 20478 entries and a zero entry in LDS, the rest gathered from global
 // memory;
 // otherwise one pixel per lane.
+// runs_mask < 0 (JFA_AUTO): runs where every wave of the launch gets one,
+// bands of rows elsewhere; otherwise bit k set = runs at step 2^k, clear =
+// bands (the labels do not depend on it).
+enum { JFA_AUTO = -1 };
 struct JfaPlan {
     bool quad = false, lds = false;
     int grid = 0, threads = 0, table_f4 = 0, lds_entries = 0;
     size_t lds_bytes = 0;
+    int runs_mask = JFA_AUTO, blocks_per_cu = 0;
 };
-JfaPlan plan_jfa(int W, int H, int capS);
+// blocks_per_cu > 0 replaces the built-in grid (four 256-thread workgroups
+// per CU for tables of up to 5120 entries, one 1024-thread workgroup per CU
+// above); runs_mask as in JfaPlan.
+JfaPlan plan_jfa(int W, int H, int capS, int runs_mask = JFA_AUTO, int blocks_per_cu = 0);
+// the steps 2^k that take runs under this plan (bit k)
+int jfa_runs_mask(const JfaPlan &p, int W, int H);
 
 // One JFA pass over a W x H map of byte-offset labels (8 * site index, `none`
 // for no site): out(p) = best of the 9 labels at p + step * {-1, 0, 1}^2.

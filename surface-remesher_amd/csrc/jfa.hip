@@ -913,7 +913,7 @@ static int g_num_cu = 0;
 // static LDS table sizes of the single-map kernel, in float2 entries
 static const int kLdsEntries[3] = {2048, 5120, 20480};
 
-JfaPlan plan_jfa(int W, int H, int capS)
+JfaPlan plan_jfa(int W, int H, int capS, int runs_mask, int blocks_per_cu)
 {
     if (!g_num_cu) {
         int dev = 0;
@@ -945,6 +945,9 @@ JfaPlan plan_jfa(int W, int H, int capS)
         p.threads = 256;
         p.grid = g_num_cu * 4;
     }
+    p.runs_mask = runs_mask < 0 ? JFA_AUTO : runs_mask;
+    p.blocks_per_cu = blocks_per_cu;
+    if (blocks_per_cu > 0) p.grid = g_num_cu * blocks_per_cu;
     // at least one item per wave
     const long waves = (long)p.grid * (p.threads / 64);
     if (waves > items) p.grid = (int)std::max<long>(1, items / (p.threads / 64));
@@ -993,9 +996,22 @@ static void launch_quad(hipStream_t st, const JfaPlan &p, const uint32_t *in, ui
 // whenever they fill a quarter of the waves).
 static bool use_runs(const JfaPlan &p, int W, int H, int step)
 {
+    if (p.runs_mask >= 0) return (p.runs_mask >> __builtin_ctz((unsigned)step)) & 1;
     const long n = (H + step - 1) / step;
     const long runs = (long)std::min(step, H) * ((n + 3) / 4) * ((W + 255) / 256);
     return runs >= (long)p.grid * (p.threads / 64);
+}
+
+int jfa_runs_mask(const JfaPlan &p, int W, int H)
+{
+    // steps 1, 2, 4 .. the first step (the largest power of two below max(W, H))
+    int mask = 0;
+    if (p.quad)
+        for (int k = 0;; ++k) {
+            if (use_runs(p, W, H, 1 << k)) mask |= 1 << k;
+            if ((2 << k) >= std::max(W, H)) break;
+        }
+    return mask;
 }
 
 void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,

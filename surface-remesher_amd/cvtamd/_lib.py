@@ -98,6 +98,8 @@ SIGNATURES = {
     "cvt_unet_synchronize": (_i, [_vp]),
     "cvt_set_tile": (_i, [_vp, _i]),
     "cvt_get_tile": (_i, [_vp]),
+    "cvt_set_jfa_shape": (_i, [_vp, _i, _i]),
+    "cvt_get_jfa_shape": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
 }
 
 
@@ -110,6 +112,12 @@ def lib():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()' "
             f"or make -C {PKG_ROOT})")
+    try:
+        # torch ships its own HIP runtime; loaded after ours, it fails to
+        # initialise the device, so it goes first when it is installed
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         f = getattr(L, name)

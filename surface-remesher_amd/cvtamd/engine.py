@@ -194,6 +194,21 @@ class Engine:
         call("cvt_synchronize", self._h)
 
     @property
+    def jfa_shape(self) -> tuple:
+        """JFA launch shape (runs_mask, blocks_per_cu).  Set: runs_mask -1 =
+        automatic, else bit k = runs at step 2^k (bands where clear);
+        blocks_per_cu 0 = the built-in grid (cvt_set_jfa_shape).  Read: the
+        steps that take runs under the current shape, and blocks_per_cu."""
+        mask, bpc = C.c_int(), C.c_int()
+        call("cvt_get_jfa_shape", self._h, C.byref(mask), C.byref(bpc))
+        return mask.value, bpc.value
+
+    @jfa_shape.setter
+    def jfa_shape(self, shape):
+        mask, bpc = shape
+        call("cvt_set_jfa_shape", self._h, int(mask), int(bpc))
+
+    @property
     def tile(self) -> int:
         return call("cvt_get_tile", self._h)
 

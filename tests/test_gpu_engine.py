@@ -59,10 +59,10 @@ def check_moments(m, ref, W_tol=1e-6):
     (260, 130, 5000, 12),     # 40 KB site table, width not a multiple of 256
     (512, 384, 8000, 10),     # 160 KB site table on a small grid
     (2560, 1700, 8000, 13),   # runs layout over chains of unequal length
-    (704, 608, 20470, 17),    # persistent JFA: the largest table its LDS holds
-    (704, 608, 20471, 18),    # persistent JFA: just too large, split table
-    (256, 320, 900, 19),      # persistent JFA, one segment per row: four run groups per item
-    (32, 40, 20, 20),         # persistent JFA on the narrowest grid it takes
+    (704, 608, 20470, 17),    # runs with the largest table that LDS holds whole
+    (704, 608, 20471, 18),    # runs with the smallest split table
+    (256, 320, 900, 19),      # one 256-pixel segment per row
+    (32, 40, 20, 20),         # runs on a grid narrower than one segment
 ])
 def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
     xy = rand_sites(seed, W, H, S)
@@ -219,6 +219,36 @@ def test_lloyd_one_step_matches_update(gpu, orc):
     lab = orc.jfa(w.n, w.n, w.sites)
     ref = orc.update(orc.moments(w.density, lab, w.sites), w.sites, 1.0)
     assert np.abs(out - ref).max() < 1e-4
+
+
+@pytest.mark.parametrize("W,H,S", [(512, 512, 1024), (1024, 768, 5000), (704, 608, 20471)])
+def test_jfa_shapes_bit_identical(gpu, orc, W, H, S):
+    """Every JFA launch shape (runs/bands per step, workgroups per CU) gives
+    the oracle's labels, and the same sites and energies over Lloyd steps."""
+    xy = rand_sites(21, W, H, S)
+    ref = orc.jfa(W, H, xy)
+    rho = np.ones((H, W), np.float32)
+    with make_engine(W, xy, rho, H=H) as e:
+        auto = e.jfa_shape
+        assert auto[1] == 0
+        full = (1 << (max(W, H) - 1).bit_length()) - 1
+        shapes = [(-1, 0), (0, 0), (full, 0), (0x555 & full, 0), (0xAAA & full, 0),
+                  (-1, 1), (full, 2), (0, 8), (auto[0] ^ 1, 4)]
+        base = None
+        for shape in shapes:
+            e.jfa_shape = shape
+            e.set_sites(xy)
+            e.clear_history()
+            e.voronoi()
+            assert np.array_equal(e.labels(), ref), shape
+            e.lloyd(4)
+            got = (e.sites(), e.history())
+            if base is None:
+                base = got
+            assert np.array_equal(got[0], base[0]), shape
+            assert np.array_equal(got[1], base[1]), shape
+        e.jfa_shape = (-1, 0)
+        assert e.jfa_shape == auto
 
 
 def test_lloyd_deterministic_and_graph_equals_eager(gpu):

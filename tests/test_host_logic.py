@@ -49,6 +49,35 @@ def test_shipped_tile_table_is_consistent():
     for k, v in t.items():
         assert v["tile"] in (32, 64, 128, 256)
         assert min(v["ms"], key=lambda s: v["ms"][s]) == str(v["tile"])
+        if "jfa" in v:
+            j = v["jfa"]
+            W = int(k.split("x")[0])
+            assert 0 <= j["runs_mask"] < (1 << len(at.jfa_steps(W, W)))
+            assert j["blocks_per_cu"] in (0, 1, 2, 4, 8)
+            # the recorded choice is among the timed ones and within the
+            # tuner's 1% margin of the fastest
+            ms = j["ms"][f"{j['runs_mask']}/{j['blocks_per_cu']}"]
+            assert ms <= min(j["ms"].values()) / (1.0 - 0.01) + 1e-9
+    assert any("jfa" in v for v in t.values())
+
+
+def test_jfa_shape_lookup_and_apply(tmp_path):
+    from cvtamd import autotune as at
+    table = {at.key(512, 512, 1024): {"tile": 32, "jfa": {"runs_mask": 5, "blocks_per_cu": 2}},
+             at.key(1024, 1024, 2048): {"tile": 64}}
+    assert at.jfa_shape_for(512, 512, 1024, table) == (5, 2)
+    assert at.jfa_shape_for(1024, 1024, 2048, table) is None
+    assert at.jfa_steps(512, 512) == list(range(9))        # steps 1 .. 256
+    assert at.jfa_steps(2048, 1024) == list(range(11))     # steps 1 .. 1024
+    assert at.jfa_steps(3, 3) == [0, 1]
+
+    class FakeEngine:
+        tile, jfa_shape = 0, None
+
+    e = FakeEngine()
+    assert at.apply(e, 512, 512, 1024, table) == 32 and e.jfa_shape == (5, 2)
+    at.apply(e, 1024, 1024, 2048, table)
+    assert e.tile == 64 and e.jfa_shape == (-1, 0)          # built-in rule
 
 
 def test_time_to_tol_summary():
