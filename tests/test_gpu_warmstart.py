@@ -36,6 +36,8 @@ def test_trained_warm_start_reaches_tolerance_in_fewer_iterations(gpu):
                                  "ratio", "fallback")})
         assert r["plain_reached"] and r["learned_reached"]
         assert not r["fallback"]
-        # the network must at least halve the iterations to the tolerance
-        assert r["learned_iters"] < 0.5 * r["plain_iters"]
-    assert np.median([r["ratio"] for r in rows]) > 1.5
+        # measured on these two patches: 15 of 1944 and 21 of 1030 iterations,
+        # 66x and 29x in time to the tolerance
+        assert r["learned_iters"] < 0.2 * r["plain_iters"]
+        assert r["ratio"] >= 3.0
+    assert np.median([r["ratio"] for r in rows]) >= 3.0
