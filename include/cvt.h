@@ -291,6 +291,39 @@ int cvt_batch_synchronize(cvt_batch *batch);
  * (units of the domain side) sampled bilinearly at the sites.  All buffers
  * may be host or device pointers.
  * ------------------------------------------------------------------------- */
+/* ---------------------------------------------------------------------
+ * Delaunay triangulation of 2D points (the unconstrained part of gDel2D's
+ * GpuDel::compute, source/gDel2D/gDel2D/GPU/GpuDelaunay.cu:46-83, with
+ * GDel2DInput / GDel2DOutput at source/gDel2D/gDel2D/GpuDelaunay.h:28-64).
+ * A context owns the device memory for up to max_points points.
+ * cvt_dt_compute takes num_points float64 (x, y) pairs and writes the
+ * finite Delaunay triangles (counterclockwise vertex index triples, like
+ * GDel2DOutput::triVec) and, if opp is not null, their neighbours in
+ * gDel2D's TriOpp encoding ((triangle << 4) | edge, edge i opposite vertex
+ * i; -1 on the convex hull; GDel2DOutput::triOppVec).  *num_tri receives the
+ * count (CVT_ERR_CAPACITY when it exceeds capacity triangles).  Predicates
+ * are exact; exact duplicate points are dropped (cvt_dt_stats counts them);
+ * CVT_ERR_ARG for non-finite points, for inputs that are all collinear or
+ * put an inserted point exactly on an edge, and for coordinates whose
+ * magnitudes span more than 2^71 units of their finest bit.  In general
+ * position (no three points collinear, no four cocircular) the result is
+ * the unique Delaunay triangulation.  Arrays may be host or device memory;
+ * the call returns when the output is written.
+ * --------------------------------------------------------------------- */
+typedef struct cvt_dt cvt_dt;
+int cvt_dt_create(cvt_dt **out, int max_points);
+int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, int *tri, int *opp,
+                   int capacity, int *num_tri);
+/* stats[4]: insertion rounds, flip passes, flips and dropped duplicates of
+ * the last cvt_dt_compute (gDel2D's Statistics, CommonTypes.h:532) */
+int cvt_dt_stats(cvt_dt *dt, int *stats);
+int cvt_dt_destroy(cvt_dt *dt);
+/* orient[i] = sign of orient2d(a, b, c) and incircle[i] = sign of
+ * incircle(a, b, c, d) for count 8-double records (ax ay bx by cx cy dx dy),
+ * evaluated on the device by the triangulation's exact predicates (gDel2D's
+ * orient2dFastExact / incircleExact, source/gDel2D/gDel2D/GPU/KerShewchuk.h:636,726) */
+int cvt_dt_predicates(const double *quads, int count, int *orient, int *incircle);
+
 typedef struct cvt_unet cvt_unet;
 
 int cvt_unet_create(cvt_unet **net, int R, int max_batch, unsigned long long seed);

@@ -1,0 +1,729 @@
+// delaunay.hip -- 2D Delaunay triangulation on the GPU (the unconstrained part
+// of gDel2D's GpuDel::compute, source/gDel2D/gDel2D/GPU/GpuDelaunay.cu:46-83).
+//
+// The triangulation is kept on the sphere: the points plus one vertex at
+// infinity (index n), so every edge has two triangles.  A triangle holding
+// the infinite vertex stands for the exterior beyond its finite edge; for
+// point location that exterior is cut into wedges by rays from an interior
+// point O (the centroid of the first triangle, stored as point n): the
+// triangle (x, y, inf) owns the wedge between the rays O->x and O->y beyond
+// the edge x->y.
+//
+// Rounds of parallel insertion, each followed by flipping to Delaunay:
+//   vote      every uninserted point bids for its triangle (the point closest
+//             to the centroid of a finite triangle, the farthest beyond the
+//             edge of an infinite one; ties by index)
+//   split     each triangle with a bid splits 1 -> 3 around the winner; the
+//             two new triangles go to slots after the current ones, placed
+//             by a prefix sum over the splitting triangles
+//   relocate  the other points of a split triangle move to the child that
+//             holds them (orientation tests against the new point)
+//   flip      Lawson flips until no edge is illegal: every illegal edge bids
+//             its id (3 t + i, t the lower triangle) on both its triangles,
+//             an edge that wins both flips, the outer neighbours are re-linked
+//             through per-edge forwarding records, and points follow their
+//             triangle across the new diagonal.  The lowest illegal edge
+//             always wins, so every round makes progress.
+// Predicates are exact (dt_predicates.h).  Inputs must be in general position
+// for the result to be the unique Delaunay triangulation: exactly collinear
+// insertions are reported as degenerate; four exactly cocircular points give
+// one of the valid triangulations.  Exact duplicate points are dropped.
+//
+// Output: the finite triangles, counterclockwise, vertex indices into the
+// input; neighbours as gDel2D's TriOpp encoding ((triangle << 4) | edge,
+// edge i opposite vertex i; -1 on the convex hull).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "../../include/cvt.h"
+#include "cvt_engine.h"
+#include "dt_predicates.h"
+
+namespace cvt {
+
+using namespace dt;
+
+enum { C_UNINS = 0, C_DUP = 1, C_ERR = 2, C_FLIPS = 3, C_A = 4, C_B = 5, C_C = 6, NUM_C = 8 };
+static constexpr unsigned long long NO_BID = ~0ull;
+static constexpr int NO_FLIP = 0x7fffffff;
+
+struct Pt {
+    double x, y;
+};
+
+__device__ __forceinline__ int orient_p(const Pt *P, int a, int b, int c, int *err)
+{
+    return orient2d(P[a].x, P[a].y, P[b].x, P[b].y, P[c].x, P[c].y, err);
+}
+
+__device__ __forceinline__ bool same_pt(const Pt &a, const Pt &b) { return a.x == b.x && a.y == b.y; }
+
+// block-wide count of pred, added to *counter once per block
+__device__ __forceinline__ void count_add(int *counter, bool pred)
+{
+    const int c = __syncthreads_count(pred);
+    if (threadIdx.x == 0 && c) atomicAdd(counter, c);
+}
+
+template <typename T>
+__global__ void k_dt_fill(T *__restrict__ p, T v, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// first triangle: a = lowest x (then y, then index), b = highest, c = the
+// point farthest from the line ab (approximate distance, ties by index)
+__global__ __launch_bounds__(1024) void k_dt_extremes(const Pt *__restrict__ P, int n,
+                                                      int *__restrict__ cnt)
+{
+    __shared__ int sa[1024], sb[1024];
+    int a = -1, b = -1;
+    auto less = [&](int i, int j) {                 // i before j in (x, y, index) order
+        if (j < 0) return true;
+        if (P[i].x != P[j].x) return P[i].x < P[j].x;
+        if (P[i].y != P[j].y) return P[i].y < P[j].y;
+        return i < j;
+    };
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        if (!isfinite(P[i].x) || !isfinite(P[i].y)) {
+            atomicOr(cnt + C_ERR, (int)DT_ERR_INPUT);
+            continue;
+        }
+        if (a < 0 || less(i, a)) a = i;
+        if (b < 0 || !less(i, b)) b = i;
+    }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const int a2 = sa[threadIdx.x + s], b2 = sb[threadIdx.x + s];
+            if (a2 >= 0 && (sa[threadIdx.x] < 0 || less(a2, sa[threadIdx.x]))) sa[threadIdx.x] = a2;
+            if (b2 >= 0 && (sb[threadIdx.x] < 0 || !less(b2, sb[threadIdx.x]))) sb[threadIdx.x] = b2;
+        }
+        __syncthreads();
+    }
+    a = sa[0];
+    b = sb[0];
+    __syncthreads();
+    __shared__ double sv[1024];
+    int c = -1;
+    double best = -1.0;
+    if (a >= 0 && b >= 0) {
+        const double ux = P[b].x - P[a].x, uy = P[b].y - P[a].y;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const double v = fabs(ux * (P[i].y - P[a].y) - uy * (P[i].x - P[a].x));
+            if (v > best) { best = v; c = i; }
+        }
+    }
+    sv[threadIdx.x] = best;
+    sa[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double v2 = sv[threadIdx.x + s];
+            const int c2 = sa[threadIdx.x + s];
+            if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && c2 >= 0 && c2 < sa[threadIdx.x])) {
+                sv[threadIdx.x] = v2;
+                sa[threadIdx.x] = c2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cnt[C_A] = a;
+        cnt[C_B] = b;
+        cnt[C_C] = sa[0];
+    }
+}
+
+// the four triangles of the first triangle on the sphere, and O
+__global__ void k_dt_init(Pt *__restrict__ P, int n, int *__restrict__ tv, int *__restrict__ to,
+                          int *__restrict__ cnt)
+{
+    int *err = cnt + C_ERR;
+    int a = cnt[C_A], b = cnt[C_B], c = cnt[C_C];
+    if (a < 0 || b < 0 || c < 0 || a == b || a == c || b == c) {
+        set_flag(err, DT_ERR_DEGENERATE);
+        return;
+    }
+    const int o = orient_p(P, a, b, c, err);
+    if (o == 0) {
+        set_flag(err, DT_ERR_DEGENERATE);                  // all points collinear
+        return;
+    }
+    if (o < 0) { const int t = b; b = c; c = t; }
+    cnt[C_A] = a;
+    cnt[C_B] = b;
+    cnt[C_C] = c;
+    Pt O;
+    O.x = (P[a].x + P[b].x + P[c].x) / 3.0;
+    O.y = (P[a].y + P[b].y + P[c].y) / 3.0;
+    P[n] = O;
+    if (orient_p(P, a, b, n, err) <= 0 || orient_p(P, b, c, n, err) <= 0 ||
+        orient_p(P, c, a, n, err) <= 0) {
+        set_flag(err, DT_ERR_DEGENERATE);                  // a sliver first triangle
+        return;
+    }
+    const int I = n;
+    // T0 = (a, b, c), T1 = (b, a, inf), T2 = (c, b, inf), T3 = (a, c, inf)
+    const int V[12] = {a, b, c, b, a, I, c, b, I, a, c, I};
+    // edge i is opposite vertex i; links (triangle << 4) | edge
+    const int L[12] = {(2 << 4) | 2, (3 << 4) | 2, (1 << 4) | 2,
+                       (3 << 4) | 1, (2 << 4) | 0, (0 << 4) | 2,
+                       (1 << 4) | 1, (3 << 4) | 0, (0 << 4) | 0,
+                       (2 << 4) | 1, (1 << 4) | 0, (0 << 4) | 1};
+    for (int i = 0; i < 12; ++i) {
+        tv[i] = V[i];
+        to[i] = L[i];
+    }
+}
+
+// every other point to T0 or to the wedge of T1..T3 that holds it
+__global__ void k_dt_locate0(const Pt *__restrict__ P, int n, int *__restrict__ loc,
+                             int *__restrict__ cnt)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool dup = false;
+    if (p < n) {
+        int *err = cnt + C_ERR;
+        const int a = cnt[C_A], b = cnt[C_B], c = cnt[C_C], O = n;
+        int t;
+        if (p == a || p == b || p == c) {
+            t = -1;
+        } else if (same_pt(P[p], P[a]) || same_pt(P[p], P[b]) || same_pt(P[p], P[c])) {
+            t = -2;
+            dup = true;
+        } else if (orient_p(P, a, b, p, err) >= 0 && orient_p(P, b, c, p, err) >= 0 &&
+                   orient_p(P, c, a, p, err) >= 0) {
+            t = 0;
+        } else {
+            const int sa = orient_p(P, O, a, p, err), sb = orient_p(P, O, b, p, err),
+                      sc = orient_p(P, O, c, p, err);
+            // wedge (x, y) between the rays O->x and O->y, clockwise
+            if (sb <= 0 && sa > 0) t = 1;
+            else if (sc <= 0 && sb > 0) t = 2;
+            else t = 3;
+        }
+        loc[p] = t;
+    }
+    count_add(cnt + C_DUP, dup);
+}
+
+__device__ __forceinline__ int inf_pos(const int *v, int I)
+{
+    return v[0] == I ? 0 : v[1] == I ? 1 : v[2] == I ? 2 : -1;
+}
+
+__global__ void k_dt_vote(const Pt *__restrict__ P, int n, const int *__restrict__ tv,
+                          int *__restrict__ loc, unsigned long long *__restrict__ vote,
+                          int *__restrict__ cnt)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = false, dup = false;
+    if (p < n && loc[p] >= 0) {
+        const int t = loc[p];
+        const int v[3] = {tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]};
+        const int k = inf_pos(v, n);
+        for (int j = 0; j < 3; ++j)
+            if (j != k && same_pt(P[p], P[v[j]])) dup = true;
+        if (dup) {
+            loc[p] = -2;
+        } else {
+            live = true;
+            float key;
+            if (k < 0) {
+                const double cx = (P[v[0]].x + P[v[1]].x + P[v[2]].x) / 3.0 - P[p].x;
+                const double cy = (P[v[0]].y + P[v[1]].y + P[v[2]].y) / 3.0 - P[p].y;
+                key = (float)(cx * cx + cy * cy);
+            } else {
+                const Pt &u = P[v[(k + 1) % 3]], &w = P[v[(k + 2) % 3]];
+                const double d = (w.x - u.x) * (P[p].y - u.y) - (w.y - u.y) * (P[p].x - u.x);
+                key = d > 0.0 ? 1.0f / (1.0f + (float)d) : 1.0f;
+            }
+            if (!(key >= 0.0f)) key = 3.0e38f;
+            const unsigned long long bid =
+                ((unsigned long long)__float_as_uint(key) << 32) | (unsigned)p;
+            atomicMin(&vote[t], bid);
+        }
+    }
+    count_add(cnt + C_UNINS, live);
+    count_add(cnt + C_DUP, dup);
+}
+
+__global__ void k_dt_split_flags(const unsigned long long *__restrict__ vote, int T,
+                                 int *__restrict__ flag)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < T) flag[t] = vote[t] != NO_BID;
+}
+
+__device__ __forceinline__ int child_slot(int t, int i, int T, const int *base)
+{
+    return i == 0 ? t : T + 2 * base[t] + (i - 1);
+}
+
+// 1 -> 3 split of every triangle with a bid: child i = (v[i+1], v[i+2], p)
+// takes slot t (i = 0) or T + 2 rank + i - 1, its edge 2 is the parent's edge i
+__global__ void k_dt_split(const Pt *__restrict__ P, int n, int T, int *__restrict__ tv,
+                           int *__restrict__ to, const unsigned long long *__restrict__ vote,
+                           const int *__restrict__ base, int *__restrict__ loc,
+                           int *__restrict__ cnt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T || vote[t] == NO_BID) return;
+    int *err = cnt + C_ERR;
+    const int p = (int)(vote[t] & 0xffffffffu);
+    const int v[3] = {tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]};
+    const int o[3] = {to[3 * t], to[3 * t + 1], to[3 * t + 2]};
+    for (int i = 0; i < 3; ++i) {
+        // p strictly left of every finite edge (v[i+1], v[i+2])
+        const int e0 = v[(i + 1) % 3], e1 = v[(i + 2) % 3];
+        if (e0 == n || e1 == n) continue;
+        if (orient_p(P, e0, e1, p, err) <= 0) set_flag(err, DT_ERR_DEGENERATE);
+    }
+    int slot[3];
+    for (int i = 0; i < 3; ++i) slot[i] = child_slot(t, i, T, base);
+    for (int i = 0; i < 3; ++i) {
+        const int s = slot[i];
+        tv[3 * s] = v[(i + 1) % 3];
+        tv[3 * s + 1] = v[(i + 2) % 3];
+        tv[3 * s + 2] = p;
+        to[3 * s] = (slot[(i + 1) % 3] << 4) | 1;
+        to[3 * s + 1] = (slot[(i + 2) % 3] << 4) | 0;
+        const int m = o[i] >> 4, e = o[i] & 3;
+        if (vote[m] != NO_BID) {
+            to[3 * s + 2] = (child_slot(m, e, T, base) << 4) | 2;
+        } else {
+            to[3 * s + 2] = o[i];
+            to[3 * m + e] = (s << 4) | 2;
+        }
+    }
+    loc[p] = -1;
+}
+
+// the other points of a split triangle to the child that holds them
+__global__ void k_dt_relocate_split(const Pt *__restrict__ P, int n, int T,
+                                    const int *__restrict__ tv,
+                                    const unsigned long long *__restrict__ vote,
+                                    const int *__restrict__ base, int *__restrict__ loc,
+                                    int *__restrict__ cnt)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    bool dup = false;
+    if (q < n) {
+        const int t = loc[q];
+        if (t >= 0 && vote[t] != NO_BID) {
+            int *err = cnt + C_ERR;
+            const int p = (int)(vote[t] & 0xffffffffu);
+            const int c1 = child_slot(t, 1, T, base);
+            // child 0 = (v1, v2, p), child 1 = (v2, v0, p)
+            const int v[3] = {tv[3 * c1 + 1], tv[3 * t], tv[3 * t + 1]};
+            if (same_pt(P[q], P[p])) {
+                dup = true;
+                loc[q] = -2;
+            } else {
+                const int k = inf_pos(v, n);
+                int ci = 0;
+                if (k < 0) {
+                    int s[3];
+                    for (int j = 0; j < 3; ++j) s[j] = orient_p(P, p, v[j], q, err);
+                    for (int i = 0; i < 3; ++i)
+                        if (s[(i + 1) % 3] >= 0 && s[(i + 2) % 3] < 0) ci = i;
+                } else {
+                    const int u = v[(k + 1) % 3], w = v[(k + 2) % 3];
+                    if (orient_p(P, w, p, q, err) >= 0 && orient_p(P, p, u, q, err) >= 0)
+                        ci = k;                                 // (u, w, p)
+                    else if (orient_p(P, n, p, q, err) > 0)
+                        ci = (k + 2) % 3;                       // (inf, u, p)
+                    else
+                        ci = (k + 1) % 3;                       // (w, inf, p)
+                }
+                loc[q] = child_slot(t, ci, T, base);
+            }
+        }
+    }
+    count_add(cnt + C_DUP, dup);
+}
+
+// w strictly inside the circumcircle of t (for t with the infinite vertex:
+// strictly beyond its finite edge)
+__device__ __forceinline__ bool illegal(const Pt *P, int n, const int *v, int w, int *err)
+{
+    if (w == n) return false;
+    const int k = inf_pos(v, n);
+    if (k < 0)
+        return incircle(P[v[0]].x, P[v[0]].y, P[v[1]].x, P[v[1]].y, P[v[2]].x, P[v[2]].y, P[w].x,
+                        P[w].y, err) > 0;
+    return orient_p(P, v[(k + 1) % 3], v[(k + 2) % 3], w, err) > 0;
+}
+
+__global__ void k_dt_flip_mark(const Pt *__restrict__ P, int n, int T, const int *__restrict__ tv,
+                               const int *__restrict__ to, int *__restrict__ fvote,
+                               int *__restrict__ cnt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int *err = cnt + C_ERR;
+    const int v[3] = {tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]};
+    for (int i = 0; i < 3; ++i) {
+        const int r = to[3 * t + i];
+        const int m = r >> 4;
+        if (r < 0 || m < t) continue;               // each edge once, by its lower triangle
+        const int w = tv[3 * m + (r & 3)];
+        if (illegal(P, n, v, w, err)) {
+            const int id = 3 * t + i;
+            atomicMin(&fvote[t], id);
+            atomicMin(&fvote[m], id);
+        }
+    }
+}
+
+// flip of the edge i of t shared with edge k of m: t' = (v_i, v_i+1, w) in
+// slot t, m' = (w, v_i+2, v_i) in slot m; the outer links are kept as they
+// were (fixed by k_dt_flip_fix) and each old outer edge forwards to its new
+// place in fwd[3 * slot + old edge]
+__global__ void k_dt_flip(int T, int *__restrict__ tv, int *__restrict__ to,
+                          const int *__restrict__ fvote, int *__restrict__ fwd,
+                          int *__restrict__ fstamp, int4 *__restrict__ fdiag, int round,
+                          int *__restrict__ cnt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool did = false;
+    if (t < T) {
+        const int id = fvote[t];
+        if (id != NO_FLIP && id / 3 == t) {
+            const int i = id - 3 * t;
+            const int r = to[3 * t + i], m = r >> 4, k = r & 3;
+            if (fvote[m] == id) {
+                did = true;
+                const int vi = tv[3 * t + i], vi1 = tv[3 * t + (i + 1) % 3],
+                          vi2 = tv[3 * t + (i + 2) % 3];
+                const int w = tv[3 * m + k];
+                const int ot1 = to[3 * t + (i + 1) % 3], ot2 = to[3 * t + (i + 2) % 3];
+                const int om1 = to[3 * m + (k + 1) % 3], om2 = to[3 * m + (k + 2) % 3];
+                tv[3 * t] = vi;
+                tv[3 * t + 1] = vi1;
+                tv[3 * t + 2] = w;
+                tv[3 * m] = w;
+                tv[3 * m + 1] = vi2;
+                tv[3 * m + 2] = vi;
+                // t': edge 0 = (vi1, w) was m's edge k+1; edge 1 = diagonal;
+                //     edge 2 = (vi, vi1) was t's edge i+2
+                // m': edge 0 = (vi2, vi) was t's edge i+1; edge 1 = diagonal;
+                //     edge 2 = (w, vi2) was m's edge k+2
+                to[3 * t] = om1;
+                to[3 * t + 1] = (m << 4) | 1;
+                to[3 * t + 2] = ot2;
+                to[3 * m] = ot1;
+                to[3 * m + 1] = (t << 4) | 1;
+                to[3 * m + 2] = om2;
+                fwd[3 * t + (i + 2) % 3] = (t << 4) | 2;
+                fwd[3 * t + (i + 1) % 3] = (m << 4) | 0;
+                fwd[3 * m + (k + 1) % 3] = (t << 4) | 0;
+                fwd[3 * m + (k + 2) % 3] = (m << 4) | 2;
+                fstamp[t] = round;
+                fstamp[m] = round;
+                const int4 d = make_int4(vi, w, t, m);
+                fdiag[t] = d;
+                fdiag[m] = d;
+            }
+        }
+    }
+    count_add(cnt + C_FLIPS, did);
+}
+
+__global__ void k_dt_flip_fix(int T, int *__restrict__ to, const int *__restrict__ fwd,
+                              const int *__restrict__ fstamp, int round)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T || fstamp[t] != round) return;
+    for (int e = 0; e < 3; e += 2) {
+        const int r = to[3 * t + e];
+        const int m = r >> 4, k = r & 3;
+        if (fstamp[m] == round) {
+            to[3 * t + e] = fwd[3 * m + k];
+        } else {
+            to[3 * m + k] = (t << 4) | e;
+        }
+    }
+}
+
+__global__ void k_dt_relocate_flip(const Pt *__restrict__ P, int n, int *__restrict__ loc,
+                                   const int *__restrict__ fstamp,
+                                   const int4 *__restrict__ fdiag, int round,
+                                   int *__restrict__ cnt)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int t = loc[q];
+    if (t < 0 || fstamp[t] != round) return;
+    const int4 d = fdiag[t];
+    // t' = (vi, vi1, w) lies right of vi -> w
+    loc[q] = orient_p(P, d.x, d.y, q, cnt + C_ERR) < 0 ? d.z : d.w;
+}
+
+__global__ void k_dt_alive(const int *__restrict__ tv, int T, int n, int *__restrict__ flag)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < T) flag[t] = tv[3 * t] != n && tv[3 * t + 1] != n && tv[3 * t + 2] != n;
+}
+
+__global__ void k_dt_output(const int *__restrict__ tv, const int *__restrict__ to, int T, int n,
+                            const int *__restrict__ flag, const int *__restrict__ pos,
+                            int *__restrict__ tri, int *__restrict__ opp)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T || !flag[t]) return;
+    const int j = pos[t];
+    for (int e = 0; e < 3; ++e) {
+        tri[3 * j + e] = tv[3 * t + e];
+        if (opp) {
+            const int r = to[3 * t + e], m = r >> 4;
+            opp[3 * j + e] = flag[m] ? (pos[m] << 4) | (r & 3) : -1;
+        }
+    }
+}
+
+// predicate check: orient(a, b, c) and incircle(a, b, c, d) per 8 coordinates
+__global__ void k_dt_predicates(const double *__restrict__ q, int count, int *__restrict__ o,
+                                int *__restrict__ ic, int *__restrict__ err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const double *x = q + 8 * i;
+    o[i] = orient2d(x[0], x[1], x[2], x[3], x[4], x[5], err);
+    ic[i] = incircle(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], err);
+}
+
+struct DtCtx {
+    int maxN = 0, Tmax = 0;
+    hipStream_t st = nullptr;
+    Pt *P = nullptr;                       // [maxN + 1]; point n is O
+    int *tv = nullptr, *to = nullptr;      // [Tmax][3]
+    unsigned long long *vote = nullptr;    // [Tmax]
+    int *flag = nullptr, *pos = nullptr;   // [Tmax], [Tmax + 1]
+    int *fvote = nullptr, *fstamp = nullptr, *fwd = nullptr;
+    int4 *fdiag = nullptr;
+    int *loc = nullptr;                    // [maxN]
+    int *otri = nullptr, *oopp = nullptr;  // [Tmax][3] output staging
+    int *cnt = nullptr;                    // [NUM_C]
+    int *hcnt = nullptr;                   // pinned [NUM_C + 1]
+    char *arena = nullptr;
+    int stats[4] = {0, 0, 0, 0};           // rounds, flip passes, flips, duplicates
+};
+
+static void dt_free(DtCtx *c)
+{
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->hcnt) (void)hipHostFree(c->hcnt);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+static inline unsigned nb(long n) { return (unsigned)std::max<long>(1, (n + 255) / 256); }
+
+}  // namespace cvt
+
+using namespace cvt;
+
+#define DCHK(call, what)                                                                   \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) return fail(CVT_ERR_HIP, "%s: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int cvt_dt_create(cvt_dt **out, int max_points)
+{
+    clear_error();
+    if (!out) return fail(CVT_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (max_points < 3 || max_points > (1 << 26))
+        return fail(CVT_ERR_ARG, "max_points %d outside [3, 2^26]", max_points);
+    DtCtx *c = new DtCtx;
+    c->maxN = max_points;
+    c->Tmax = 2 * (max_points + 1) + 8;
+    const size_t T = (size_t)c->Tmax, N = (size_t)max_points;
+    size_t at = 0;
+    auto carve = [&](auto *&p, size_t count, bool set) {
+        at = (at + 255) & ~(size_t)255;
+        if (set) p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(c->arena + at);
+        at += count * sizeof(*p);
+    };
+    auto layout = [&](bool set) {
+        at = 0;
+        carve(c->P, N + 1, set); carve(c->tv, 3 * T, set); carve(c->to, 3 * T, set);
+        carve(c->vote, T, set); carve(c->flag, T, set); carve(c->pos, T + 1, set);
+        carve(c->fvote, T, set); carve(c->fstamp, T, set); carve(c->fwd, 3 * T, set);
+        carve(c->fdiag, T, set); carve(c->loc, N, set); carve(c->cnt, NUM_C, set);
+        carve(c->otri, 3 * T, set); carve(c->oopp, 3 * T, set);
+    };
+    layout(false);
+    if (hipMalloc(&c->arena, at) != hipSuccess) {
+        c->arena = nullptr;
+        dt_free(c);
+        return fail(CVT_ERR_NOMEM, "%zu bytes for %d points", at, max_points);
+    }
+    layout(true);
+    if (hipHostMalloc((void **)&c->hcnt, sizeof(int) * (NUM_C + 1), 0) != hipSuccess ||
+        hipStreamCreate(&c->st) != hipSuccess) {
+        dt_free(c);
+        return fail(CVT_ERR_HIP, "stream / pinned counters");
+    }
+    *out = reinterpret_cast<cvt_dt *>(c);
+    return CVT_OK;
+}
+
+extern "C" int cvt_dt_destroy(cvt_dt *dt)
+{
+    if (dt) dt_free(reinterpret_cast<DtCtx *>(dt));
+    return CVT_OK;
+}
+
+extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, int *tri,
+                              int *opp, int capacity, int *num_tri)
+{
+    clear_error();
+    if (!dt) return fail(CVT_ERR_ARG, "null context");
+    DtCtx *c = reinterpret_cast<DtCtx *>(dt);
+    const int n = num_points;
+    if (n < 3 || n > c->maxN) return fail(CVT_ERR_ARG, "num_points %d outside [3, %d]", n, c->maxN);
+    if (!points || !num_tri || (capacity > 0 && !tri) || capacity < 0)
+        return fail(CVT_ERR_ARG, "null array or negative capacity");
+    hipStream_t st = c->st;
+    int *h = c->hcnt;
+    for (int i = 0; i < 4; ++i) c->stats[i] = 0;
+    DCHK(hipMemcpyAsync(c->P, points, sizeof(Pt) * n, hipMemcpyDefault, st), "point upload");
+    hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt, 0, (long)NUM_C);
+    hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(c->Tmax)), dim3(256), 0, st, c->fstamp, -1,
+                       (long)c->Tmax);
+    hipLaunchKernelGGL(k_dt_extremes, dim3(1), dim3(1024), 0, st, c->P, n, c->cnt);
+    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, c->P, n, c->tv, c->to, c->cnt);
+    DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "check");
+    DCHK(hipStreamSynchronize(st), "first triangle");
+    if (h[C_ERR] & DT_ERR_INPUT) return fail(CVT_ERR_ARG, "a point is not finite");
+    if (h[C_ERR] & DT_ERR_RANGE)
+        return fail(CVT_ERR_ARG, "coordinates span more than 2^71 units of their finest bit");
+    if (h[C_ERR]) return fail(CVT_ERR_ARG, "degenerate input: all points collinear or coincident");
+    hipLaunchKernelGGL(k_dt_locate0, dim3(nb(n)), dim3(256), 0, st, c->P, n, c->loc, c->cnt);
+    int T = 4, round = 0;
+    // each round inserts at least one point
+    for (int guard = 0; guard <= n; ++guard) {
+        hipLaunchKernelGGL(k_dt_fill<unsigned long long>, dim3(nb(T)), dim3(256), 0, st, c->vote,
+                           NO_BID, (long)T);
+        hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_UNINS, 0, 1L);
+        hipLaunchKernelGGL(k_dt_vote, dim3(nb(n)), dim3(256), 0, st, c->P, n, c->tv, c->loc,
+                           c->vote, c->cnt);
+        hipLaunchKernelGGL(k_dt_split_flags, dim3(nb(T)), dim3(256), 0, st, c->vote, T, c->flag);
+        launch_exclusive_scan(st, c->flag, T, c->pos);
+        DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "counters");
+        DCHK(hipMemcpyAsync(h + NUM_C, c->pos + T, sizeof(int), hipMemcpyDeviceToHost, st),
+             "counters");
+        DCHK(hipStreamSynchronize(st), "vote");
+        if (h[C_ERR]) break;
+        if (h[C_UNINS] == 0) break;
+        const int splits = h[NUM_C];
+        if (T + 2 * splits > c->Tmax) return fail(CVT_ERR_CAPACITY, "triangle capacity");
+        hipLaunchKernelGGL(k_dt_split, dim3(nb(T)), dim3(256), 0, st, c->P, n, T, c->tv, c->to,
+                           c->vote, c->pos, c->loc, c->cnt);
+        hipLaunchKernelGGL(k_dt_relocate_split, dim3(nb(n)), dim3(256), 0, st, c->P, n, T, c->tv,
+                           c->vote, c->pos, c->loc, c->cnt);
+        T += 2 * splits;
+        ++c->stats[0];
+        // Lawson flips to Delaunay (each pass flips at least the lowest
+        // illegal edge; the bound only guards against a broken mesh)
+        for (int pass = 0;; ++pass) {
+            if (pass > 4 * c->Tmax + 64)
+                return fail(CVT_ERR_STATE, "flipping did not converge");
+            ++round;
+            hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(T)), dim3(256), 0, st, c->fvote, NO_FLIP,
+                               (long)T);
+            hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_FLIPS, 0, 1L);
+            hipLaunchKernelGGL(k_dt_flip_mark, dim3(nb(T)), dim3(256), 0, st, c->P, n, T, c->tv,
+                               c->to, c->fvote, c->cnt);
+            hipLaunchKernelGGL(k_dt_flip, dim3(nb(T)), dim3(256), 0, st, T, c->tv, c->to, c->fvote,
+                               c->fwd, c->fstamp, c->fdiag, round, c->cnt);
+            hipLaunchKernelGGL(k_dt_flip_fix, dim3(nb(T)), dim3(256), 0, st, T, c->to, c->fwd,
+                               c->fstamp, round);
+            hipLaunchKernelGGL(k_dt_relocate_flip, dim3(nb(n)), dim3(256), 0, st, c->P, n, c->loc,
+                               c->fstamp, c->fdiag, round, c->cnt);
+            DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st),
+                 "counters");
+            DCHK(hipStreamSynchronize(st), "flip");
+            ++c->stats[1];
+            c->stats[2] += h[C_FLIPS];
+            if (h[C_FLIPS] == 0 || h[C_ERR]) break;
+        }
+        if (h[C_ERR]) break;
+    }
+    DCHK(hipGetLastError(), "launch");
+    c->stats[3] = h[C_DUP];
+    if (h[C_ERR] & DT_ERR_RANGE)
+        return fail(CVT_ERR_ARG, "coordinates span more than 2^71 units of their finest bit");
+    if (h[C_ERR] & DT_ERR_DEGENERATE)
+        return fail(CVT_ERR_ARG, "degenerate input: collinear points where a triangle was needed");
+    hipLaunchKernelGGL(k_dt_alive, dim3(nb(T)), dim3(256), 0, st, c->tv, T, n, c->flag);
+    launch_exclusive_scan(st, c->flag, T, c->pos);
+    DCHK(hipMemcpyAsync(h + NUM_C, c->pos + T, sizeof(int), hipMemcpyDeviceToHost, st), "count");
+    DCHK(hipStreamSynchronize(st), "count");
+    const int nt = h[NUM_C];
+    *num_tri = nt;
+    if (nt > capacity) return fail(CVT_ERR_CAPACITY, "%d triangles, capacity %d", nt, capacity);
+    // staged, then copied: the caller's arrays may be host or device memory
+    hipLaunchKernelGGL(k_dt_output, dim3(nb(T)), dim3(256), 0, st, c->tv, c->to, T, n, c->flag,
+                       c->pos, c->otri, opp ? c->oopp : nullptr);
+    DCHK(hipGetLastError(), "output launch");
+    if (nt > 0) {
+        DCHK(hipMemcpyAsync(tri, c->otri, sizeof(int) * 3 * nt, hipMemcpyDefault, st), "download");
+        if (opp)
+            DCHK(hipMemcpyAsync(opp, c->oopp, sizeof(int) * 3 * nt, hipMemcpyDefault, st), "download");
+    }
+    DCHK(hipStreamSynchronize(st), "output");
+    return CVT_OK;
+}
+
+extern "C" int cvt_dt_stats(cvt_dt *dt, int *stats)
+{
+    if (!dt || !stats) return fail(CVT_ERR_ARG, "null argument");
+    const DtCtx *c = reinterpret_cast<const DtCtx *>(dt);
+    for (int i = 0; i < 4; ++i) stats[i] = c->stats[i];
+    return CVT_OK;
+}
+
+extern "C" int cvt_dt_predicates(const double *quads, int count, int *orient, int *incircle)
+{
+    clear_error();
+    if (count < 0 || (count > 0 && (!quads || !orient || !incircle)))
+        return fail(CVT_ERR_ARG, "null array or negative count");
+    if (count == 0) return CVT_OK;
+    hipStream_t st = nullptr;
+    double *dq = nullptr;
+    int *d = nullptr;
+    int rc = CVT_OK, err = 0;
+    if (hipMalloc(&dq, sizeof(double) * 8 * count) != hipSuccess ||
+        hipMalloc(&d, sizeof(int) * (2 * (size_t)count + 1)) != hipSuccess) {
+        rc = fail(CVT_ERR_NOMEM, "predicate buffers");
+    } else if (hipMemcpyAsync(dq, quads, sizeof(double) * 8 * count, hipMemcpyDefault, st) !=
+                   hipSuccess ||
+               hipMemsetAsync(d + 2 * count, 0, sizeof(int), st) != hipSuccess) {
+        rc = fail(CVT_ERR_HIP, "predicate upload");
+    } else {
+        hipLaunchKernelGGL(k_dt_predicates, dim3(nb(count)), dim3(256), 0, st, dq, count, d,
+                           d + count, d + 2 * count);
+        if (hipMemcpyAsync(orient, d, sizeof(int) * count, hipMemcpyDefault, st) != hipSuccess ||
+            hipMemcpyAsync(incircle, d + count, sizeof(int) * count, hipMemcpyDefault, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(&err, d + 2 * count, sizeof(int), hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = fail(CVT_ERR_HIP, "predicates");
+        else if (err & DT_ERR_RANGE)
+            rc = fail(CVT_ERR_ARG, "coordinates span more than 2^71 units of their finest bit");
+    }
+    if (dq) (void)hipFree(dq);
+    if (d) (void)hipFree(d);
+    return rc;
+}

@@ -98,6 +98,11 @@ SIGNATURES = {
     "cvt_unet_synchronize": (_i, [_vp]),
     "cvt_set_tile": (_i, [_vp, _i]),
     "cvt_get_tile": (_i, [_vp]),
+    "cvt_dt_create": (_i, [C.POINTER(_vp), _i]),
+    "cvt_dt_compute": (_i, [_vp, _vp, _i, _vp, _vp, _i, C.POINTER(_i)]),
+    "cvt_dt_stats": (_i, [_vp, _vp]),
+    "cvt_dt_destroy": (_i, [_vp]),
+    "cvt_dt_predicates": (_i, [_vp, _i, _vp, _vp]),
     "cvt_set_jfa_shape": (_i, [_vp, _i, _i]),
     "cvt_get_jfa_shape": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
 }
