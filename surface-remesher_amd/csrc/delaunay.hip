@@ -45,7 +45,9 @@ namespace cvt {
 
 using namespace dt;
 
-enum { C_UNINS = 0, C_DUP = 1, C_ERR = 2, C_FLIPS = 3, C_A = 4, C_B = 5, C_C = 6, NUM_C = 8 };
+enum { C_UNINS = 0, C_DUP = 1, C_ERR = 2, C_FLIPS = 3, C_A = 4, C_B = 5, C_C = 6, C_FTOT = 7,
+       NUM_C = 8 };
+static constexpr int kParts = 256;              // workgroups of the first-triangle search
 static constexpr unsigned long long NO_BID = ~0ull;
 static constexpr int NO_FLIP = 0x7fffffff;
 
@@ -75,69 +77,125 @@ __global__ void k_dt_fill(T *__restrict__ p, T v, long n)
 }
 
 // first triangle: a = lowest x (then y, then index), b = highest, c = the
-// point farthest from the line ab (approximate distance, ties by index)
-__global__ __launch_bounds__(1024) void k_dt_extremes(const Pt *__restrict__ P, int n,
-                                                      int *__restrict__ cnt)
+// point farthest from the line ab (approximate distance, ties by index);
+// each search is a per-workgroup pass over the points and a one-workgroup
+// pass over the kParts partial results
+__device__ __forceinline__ bool pt_before(const Pt *P, int i, int j)   // (x, y, index) order
 {
-    __shared__ int sa[1024], sb[1024];
-    int a = -1, b = -1;
-    auto less = [&](int i, int j) {                 // i before j in (x, y, index) order
-        if (j < 0) return true;
-        if (P[i].x != P[j].x) return P[i].x < P[j].x;
-        if (P[i].y != P[j].y) return P[i].y < P[j].y;
-        return i < j;
-    };
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        if (!isfinite(P[i].x) || !isfinite(P[i].y)) {
-            atomicOr(cnt + C_ERR, (int)DT_ERR_INPUT);
-            continue;
-        }
-        if (a < 0 || less(i, a)) a = i;
-        if (b < 0 || !less(i, b)) b = i;
-    }
+    if (j < 0) return true;
+    if (i < 0) return false;
+    if (P[i].x != P[j].x) return P[i].x < P[j].x;
+    if (P[i].y != P[j].y) return P[i].y < P[j].y;
+    return i < j;
+}
+
+// block-wide: (a, b) = first and last candidates; reduce pairs in LDS
+__device__ void reduce_ab(const Pt *P, int &a, int &b)
+{
+    __shared__ int sa[256], sb[256];
     sa[threadIdx.x] = a;
     sb[threadIdx.x] = b;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             const int a2 = sa[threadIdx.x + s], b2 = sb[threadIdx.x + s];
-            if (a2 >= 0 && (sa[threadIdx.x] < 0 || less(a2, sa[threadIdx.x]))) sa[threadIdx.x] = a2;
-            if (b2 >= 0 && (sb[threadIdx.x] < 0 || !less(b2, sb[threadIdx.x]))) sb[threadIdx.x] = b2;
+            if (a2 >= 0 && pt_before(P, a2, sa[threadIdx.x])) sa[threadIdx.x] = a2;
+            if (b2 >= 0 && (sb[threadIdx.x] < 0 || pt_before(P, sb[threadIdx.x], b2)))
+                sb[threadIdx.x] = b2;
         }
         __syncthreads();
     }
     a = sa[0];
     b = sb[0];
-    __syncthreads();
-    __shared__ double sv[1024];
-    int c = -1;
-    double best = -1.0;
-    if (a >= 0 && b >= 0) {
-        const double ux = P[b].x - P[a].x, uy = P[b].y - P[a].y;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const double v = fabs(ux * (P[i].y - P[a].y) - uy * (P[i].x - P[a].x));
-            if (v > best) { best = v; c = i; }
+}
+
+__global__ __launch_bounds__(256) void k_dt_ab_part(const Pt *__restrict__ P, int n,
+                                                    int *__restrict__ part, int *__restrict__ cnt)
+{
+    int a = -1, b = -1;
+    bool bad = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (!isfinite(P[i].x) || !isfinite(P[i].y)) {
+            bad = true;
+            continue;
         }
+        if (pt_before(P, i, a)) a = i;
+        if (b < 0 || pt_before(P, b, i)) b = i;
     }
-    sv[threadIdx.x] = best;
-    sa[threadIdx.x] = c;
+    if (bad) atomicOr(cnt + C_ERR, (int)DT_ERR_INPUT);
+    reduce_ab(P, a, b);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dt_ab_final(const Pt *__restrict__ P,
+                                                     const int *__restrict__ part,
+                                                     int *__restrict__ cnt)
+{
+    int a = part[2 * threadIdx.x], b = part[2 * threadIdx.x + 1];
+    reduce_ab(P, a, b);
+    if (threadIdx.x == 0) {
+        cnt[C_A] = a;
+        cnt[C_B] = b;
+    }
+}
+
+// block-wide: the largest v, ties by the lower index
+__device__ void reduce_far(double &v, int &c)
+{
+    __shared__ double sv[256];
+    __shared__ int sc[256];
+    sv[threadIdx.x] = v;
+    sc[threadIdx.x] = c;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             const double v2 = sv[threadIdx.x + s];
-            const int c2 = sa[threadIdx.x + s];
-            if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && c2 >= 0 && c2 < sa[threadIdx.x])) {
+            const int c2 = sc[threadIdx.x + s];
+            if (c2 >= 0 && (sc[threadIdx.x] < 0 || v2 > sv[threadIdx.x] ||
+                            (v2 == sv[threadIdx.x] && c2 < sc[threadIdx.x]))) {
                 sv[threadIdx.x] = v2;
-                sa[threadIdx.x] = c2;
+                sc[threadIdx.x] = c2;
             }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        cnt[C_A] = a;
-        cnt[C_B] = b;
-        cnt[C_C] = sa[0];
+    v = sv[0];
+    c = sc[0];
+}
+
+__global__ __launch_bounds__(256) void k_dt_c_part(const Pt *__restrict__ P, int n,
+                                                   const int *__restrict__ cnt,
+                                                   double *__restrict__ partv,
+                                                   int *__restrict__ partc)
+{
+    const int a = cnt[C_A], b = cnt[C_B];
+    double best = -1.0;
+    int c = -1;
+    if (a >= 0 && b >= 0) {
+        const double ux = P[b].x - P[a].x, uy = P[b].y - P[a].y;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            const double v = fabs(ux * (P[i].y - P[a].y) - uy * (P[i].x - P[a].x));
+            if (v > best || (v == best && i < c)) { best = v; c = i; }
+        }
     }
+    reduce_far(best, c);
+    if (threadIdx.x == 0) {
+        partv[blockIdx.x] = best;
+        partc[blockIdx.x] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dt_c_final(const double *__restrict__ partv,
+                                                    const int *__restrict__ partc,
+                                                    int *__restrict__ cnt)
+{
+    double v = partv[threadIdx.x];
+    int c = partc[threadIdx.x];
+    reduce_far(v, c);
+    if (threadIdx.x == 0) cnt[C_C] = c;
 }
 
 // the four triangles of the first triangle on the sphere, and O
@@ -218,40 +276,114 @@ __device__ __forceinline__ int inf_pos(const int *v, int I)
     return v[0] == I ? 0 : v[1] == I ? 1 : v[2] == I ? 2 : -1;
 }
 
-__global__ void k_dt_vote(const Pt *__restrict__ P, int n, const int *__restrict__ tv,
-                          int *__restrict__ loc, unsigned long long *__restrict__ vote,
-                          int *__restrict__ cnt)
+// atomicMin of each lane's bid on vote[t], one atomic per distinct t in the
+// wave (in early rounds most of a wave's points share a triangle); called
+// by every lane of the wave
+__device__ __forceinline__ void wave_min_bid(unsigned long long *vote, int t,
+                                             unsigned long long bid, bool active)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int t0 = __shfl(t, leader);
+        const bool mine = active && t == t0;
+        unsigned long long v = mine ? bid : NO_BID;
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(v, off);
+            v = o < v ? o : v;
+        }
+        if (lane == leader) atomicMin(&vote[t0], v);
+        pending &= ~__ballot(mine);
+        active = active && !mine;
+    }
+}
+
+// the bid of point p on its triangle (NO_BID and *dup when p duplicates one
+// of the triangle's vertices)
+__device__ __forceinline__ unsigned long long point_bid(const Pt *P, int n, const int *tv, int p,
+                                                        int t, bool &dup)
+{
+    const int v[3] = {tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]};
+    const int k = inf_pos(v, n);
+    dup = false;
+    for (int j = 0; j < 3; ++j)
+        if (j != k && same_pt(P[p], P[v[j]])) dup = true;
+    if (dup) return NO_BID;
+    float key;
+    if (k < 0) {
+        const double cx = (P[v[0]].x + P[v[1]].x + P[v[2]].x) / 3.0 - P[p].x;
+        const double cy = (P[v[0]].y + P[v[1]].y + P[v[2]].y) / 3.0 - P[p].y;
+        key = (float)(cx * cx + cy * cy);
+    } else {
+        const Pt &u = P[v[(k + 1) % 3]], &w = P[v[(k + 2) % 3]];
+        const double d = (w.x - u.x) * (P[p].y - u.y) - (w.y - u.y) * (P[p].x - u.x);
+        key = d > 0.0 ? 1.0f / (1.0f + (float)d) : 1.0f;
+    }
+    if (!(key >= 0.0f)) key = 3.0e38f;
+    return ((unsigned long long)__float_as_uint(key) << 32) | (unsigned)p;
+}
+
+__global__ __launch_bounds__(256) void k_dt_vote(const Pt *__restrict__ P, int n,
+                                                 const int *__restrict__ tv,
+                                                 int *__restrict__ loc,
+                                                 unsigned long long *__restrict__ vote,
+                                                 int *__restrict__ cnt)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     bool live = false, dup = false;
+    int t = -1;
+    unsigned long long bid = NO_BID;
     if (p < n && loc[p] >= 0) {
-        const int t = loc[p];
-        const int v[3] = {tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]};
-        const int k = inf_pos(v, n);
-        for (int j = 0; j < 3; ++j)
-            if (j != k && same_pt(P[p], P[v[j]])) dup = true;
-        if (dup) {
-            loc[p] = -2;
-        } else {
-            live = true;
-            float key;
-            if (k < 0) {
-                const double cx = (P[v[0]].x + P[v[1]].x + P[v[2]].x) / 3.0 - P[p].x;
-                const double cy = (P[v[0]].y + P[v[1]].y + P[v[2]].y) / 3.0 - P[p].y;
-                key = (float)(cx * cx + cy * cy);
-            } else {
-                const Pt &u = P[v[(k + 1) % 3]], &w = P[v[(k + 2) % 3]];
-                const double d = (w.x - u.x) * (P[p].y - u.y) - (w.y - u.y) * (P[p].x - u.x);
-                key = d > 0.0 ? 1.0f / (1.0f + (float)d) : 1.0f;
-            }
-            if (!(key >= 0.0f)) key = 3.0e38f;
-            const unsigned long long bid =
-                ((unsigned long long)__float_as_uint(key) << 32) | (unsigned)p;
-            atomicMin(&vote[t], bid);
-        }
+        t = loc[p];
+        bid = point_bid(P, n, tv, p, t, dup);
+        if (dup) loc[p] = -2;
+        else live = true;
     }
+    wave_min_bid(vote, t, bid, live);
     count_add(cnt + C_UNINS, live);
     count_add(cnt + C_DUP, dup);
+}
+
+// few triangles (T <= kVoteLds): a workgroup takes a grid-stride share of the
+// points and keeps the minimum bid per triangle in LDS, then adds its
+// non-empty entries to the global minimum (one global atomic per triangle
+// per workgroup instead of one per wave)
+static constexpr int kVoteLds = 4096;
+__global__ __launch_bounds__(1024) void k_dt_vote_lds(const Pt *__restrict__ P, int n, int T,
+                                                      const int *__restrict__ tv,
+                                                      int *__restrict__ loc,
+                                                      unsigned long long *__restrict__ vote,
+                                                      int *__restrict__ cnt)
+{
+    __shared__ unsigned long long sv[kVoteLds];
+    __shared__ int scount[2];
+    for (int i = threadIdx.x; i < T; i += blockDim.x) sv[i] = NO_BID;
+    if (threadIdx.x < 2) scount[threadIdx.x] = 0;
+    __syncthreads();
+    int live = 0, dups = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int t = loc[p];
+        if (t < 0) continue;
+        bool dup;
+        const unsigned long long bid = point_bid(P, n, tv, p, t, dup);
+        if (dup) {
+            loc[p] = -2;
+            ++dups;
+        } else {
+            ++live;
+            atomicMin(&sv[t], bid);
+        }
+    }
+    if (live) atomicAdd(&scount[0], live);
+    if (dups) atomicAdd(&scount[1], dups);
+    __syncthreads();
+    for (int i = threadIdx.x; i < T; i += blockDim.x)
+        if (sv[i] != NO_BID) atomicMin(&vote[i], sv[i]);
+    if (threadIdx.x == 0) {
+        if (scount[0]) atomicAdd(cnt + C_UNINS, scount[0]);
+        if (scount[1]) atomicAdd(cnt + C_DUP, scount[1]);
+    }
 }
 
 __global__ void k_dt_split_flags(const unsigned long long *__restrict__ vote, int T,
@@ -271,7 +403,7 @@ __device__ __forceinline__ int child_slot(int t, int i, int T, const int *base)
 __global__ void k_dt_split(const Pt *__restrict__ P, int n, int T, int *__restrict__ tv,
                            int *__restrict__ to, const unsigned long long *__restrict__ vote,
                            const int *__restrict__ base, int *__restrict__ loc,
-                           int *__restrict__ cnt)
+                           int *__restrict__ chk, int next_pass, int *__restrict__ cnt)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T || vote[t] == NO_BID) return;
@@ -289,6 +421,7 @@ __global__ void k_dt_split(const Pt *__restrict__ P, int n, int T, int *__restri
     for (int i = 0; i < 3; ++i) slot[i] = child_slot(t, i, T, base);
     for (int i = 0; i < 3; ++i) {
         const int s = slot[i];
+        chk[s] = next_pass;
         tv[3 * s] = v[(i + 1) % 3];
         tv[3 * s + 1] = v[(i + 2) % 3];
         tv[3 * s + 2] = p;
@@ -361,18 +494,23 @@ __device__ __forceinline__ bool illegal(const Pt *P, int n, const int *v, int w,
     return orient_p(P, v[(k + 1) % 3], v[(k + 2) % 3], w, err) > 0;
 }
 
+// an edge can turn illegal only when one of its triangles changed (split or
+// flipped) before this pass: chk[t] holds the pass that must look at t
 __global__ void k_dt_flip_mark(const Pt *__restrict__ P, int n, int T, const int *__restrict__ tv,
-                               const int *__restrict__ to, int *__restrict__ fvote,
-                               int *__restrict__ cnt)
+                               const int *__restrict__ to, const int *__restrict__ chk, int pass,
+                               int *__restrict__ fvote, int *__restrict__ cnt)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) cnt[C_FLIPS] = 0;                  // this pass's flips (k_dt_flip counts)
     if (t >= T) return;
     int *err = cnt + C_ERR;
+    const bool mine = chk[t] == pass;
     const int v[3] = {tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]};
     for (int i = 0; i < 3; ++i) {
         const int r = to[3 * t + i];
         const int m = r >> 4;
         if (r < 0 || m < t) continue;               // each edge once, by its lower triangle
+        if (!mine && chk[m] != pass) continue;
         const int w = tv[3 * m + (r & 3)];
         if (illegal(P, n, v, w, err)) {
             const int id = 3 * t + i;
@@ -388,13 +526,16 @@ __global__ void k_dt_flip_mark(const Pt *__restrict__ P, int n, int T, const int
 // place in fwd[3 * slot + old edge]
 __global__ void k_dt_flip(int T, int *__restrict__ tv, int *__restrict__ to,
                           const int *__restrict__ fvote, int *__restrict__ fwd,
-                          int *__restrict__ fstamp, int4 *__restrict__ fdiag, int round,
-                          int *__restrict__ cnt)
+                          int *__restrict__ fstamp, int4 *__restrict__ fdiag, int *__restrict__ chk,
+                          int round, int *__restrict__ cnt)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     bool did = false;
     if (t < T) {
         const int id = fvote[t];
+        // a triangle with an illegal edge is looked at again next pass,
+        // whether that edge flips now or lost the bid to a lower one
+        if (id != NO_FLIP) chk[t] = round + 1;
         if (id != NO_FLIP && id / 3 == t) {
             const int i = id - 3 * t;
             const int r = to[3 * t + i], m = r >> 4, k = r & 3;
@@ -427,43 +568,52 @@ __global__ void k_dt_flip(int T, int *__restrict__ tv, int *__restrict__ to,
                 fwd[3 * m + (k + 2) % 3] = (m << 4) | 2;
                 fstamp[t] = round;
                 fstamp[m] = round;
+                chk[t] = round + 1;
+                chk[m] = round + 1;
                 const int4 d = make_int4(vi, w, t, m);
                 fdiag[t] = d;
                 fdiag[m] = d;
             }
         }
     }
-    count_add(cnt + C_FLIPS, did);
-}
-
-__global__ void k_dt_flip_fix(int T, int *__restrict__ to, const int *__restrict__ fwd,
-                              const int *__restrict__ fstamp, int round)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T || fstamp[t] != round) return;
-    for (int e = 0; e < 3; e += 2) {
-        const int r = to[3 * t + e];
-        const int m = r >> 4, k = r & 3;
-        if (fstamp[m] == round) {
-            to[3 * t + e] = fwd[3 * m + k];
-        } else {
-            to[3 * m + k] = (t << 4) | e;
-        }
+    const int c = __syncthreads_count(did);
+    if (threadIdx.x == 0 && c) {
+        atomicAdd(cnt + C_FLIPS, c);
+        atomicAdd(cnt + C_FTOT, c);
     }
 }
 
-__global__ void k_dt_relocate_flip(const Pt *__restrict__ P, int n, int *__restrict__ loc,
-                                   const int *__restrict__ fstamp,
-                                   const int4 *__restrict__ fdiag, int round,
-                                   int *__restrict__ cnt)
+// after a flip pass, per triangle t < T: re-link the outer edges of flipped
+// triangles (through the forwarding records of flipped neighbours) and
+// reset the bid for the next pass; per point q < n: follow a flipped
+// triangle across its new diagonal
+__global__ void k_dt_flip_fix(const Pt *__restrict__ P, int n, int T, int *__restrict__ to,
+                              const int *__restrict__ fwd, const int *__restrict__ fstamp,
+                              const int4 *__restrict__ fdiag, int *__restrict__ fvote,
+                              int *__restrict__ loc, int round, int *__restrict__ cnt)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const int t = loc[q];
-    if (t < 0 || fstamp[t] != round) return;
-    const int4 d = fdiag[t];
-    // t' = (vi, vi1, w) lies right of vi -> w
-    loc[q] = orient_p(P, d.x, d.y, q, cnt + C_ERR) < 0 ? d.z : d.w;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < T) {
+        fvote[i] = NO_FLIP;
+        if (fstamp[i] == round) {
+            for (int e = 0; e < 3; e += 2) {
+                const int r = to[3 * i + e];
+                const int m = r >> 4, k = r & 3;
+                if (fstamp[m] == round)
+                    to[3 * i + e] = fwd[3 * m + k];
+                else
+                    to[3 * m + k] = (i << 4) | e;
+            }
+        }
+    }
+    if (i < n) {
+        const int t = loc[i];
+        if (t >= 0 && fstamp[t] == round) {
+            const int4 d = fdiag[t];
+            // t' = (vi, vi1, w) lies right of vi -> w
+            loc[i] = orient_p(P, d.x, d.y, i, cnt + C_ERR) < 0 ? d.z : d.w;
+        }
+    }
 }
 
 __global__ void k_dt_alive(const int *__restrict__ tv, int T, int n, int *__restrict__ flag)
@@ -506,11 +656,13 @@ struct DtCtx {
     int *tv = nullptr, *to = nullptr;      // [Tmax][3]
     unsigned long long *vote = nullptr;    // [Tmax]
     int *flag = nullptr, *pos = nullptr;   // [Tmax], [Tmax + 1]
-    int *fvote = nullptr, *fstamp = nullptr, *fwd = nullptr;
+    int *fvote = nullptr, *fstamp = nullptr, *fwd = nullptr, *chk = nullptr;
     int4 *fdiag = nullptr;
     int *loc = nullptr;                    // [maxN]
     int *otri = nullptr, *oopp = nullptr;  // [Tmax][3] output staging
     int *cnt = nullptr;                    // [NUM_C]
+    int *part = nullptr;                   // [3 kParts] first-triangle search
+    double *partv = nullptr;               // [kParts]
     int *hcnt = nullptr;                   // pinned [NUM_C + 1]
     char *arena = nullptr;
     int stats[4] = {0, 0, 0, 0};           // rounds, flip passes, flips, duplicates
@@ -558,8 +710,10 @@ extern "C" int cvt_dt_create(cvt_dt **out, int max_points)
         carve(c->P, N + 1, set); carve(c->tv, 3 * T, set); carve(c->to, 3 * T, set);
         carve(c->vote, T, set); carve(c->flag, T, set); carve(c->pos, T + 1, set);
         carve(c->fvote, T, set); carve(c->fstamp, T, set); carve(c->fwd, 3 * T, set);
+        carve(c->chk, T, set);
         carve(c->fdiag, T, set); carve(c->loc, N, set); carve(c->cnt, NUM_C, set);
         carve(c->otri, 3 * T, set); carve(c->oopp, 3 * T, set);
+        carve(c->part, 3 * kParts, set); carve(c->partv, kParts, set);
     };
     layout(false);
     if (hipMalloc(&c->arena, at) != hipSuccess) {
@@ -600,7 +754,16 @@ extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, 
     hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt, 0, (long)NUM_C);
     hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(c->Tmax)), dim3(256), 0, st, c->fstamp, -1,
                        (long)c->Tmax);
-    hipLaunchKernelGGL(k_dt_extremes, dim3(1), dim3(1024), 0, st, c->P, n, c->cnt);
+    hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(c->Tmax)), dim3(256), 0, st, c->chk, -1,
+                       (long)c->Tmax);
+    hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(c->Tmax)), dim3(256), 0, st, c->fvote, NO_FLIP,
+                       (long)c->Tmax);
+    hipLaunchKernelGGL(k_dt_ab_part, dim3(kParts), dim3(256), 0, st, c->P, n, c->part, c->cnt);
+    hipLaunchKernelGGL(k_dt_ab_final, dim3(1), dim3(kParts), 0, st, c->P, c->part, c->cnt);
+    hipLaunchKernelGGL(k_dt_c_part, dim3(kParts), dim3(256), 0, st, c->P, n, c->cnt, c->partv,
+                       c->part + 2 * kParts);
+    hipLaunchKernelGGL(k_dt_c_final, dim3(1), dim3(kParts), 0, st, c->partv, c->part + 2 * kParts,
+                       c->cnt);
     hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, c->P, n, c->tv, c->to, c->cnt);
     DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "check");
     DCHK(hipStreamSynchronize(st), "first triangle");
@@ -615,8 +778,12 @@ extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, 
         hipLaunchKernelGGL(k_dt_fill<unsigned long long>, dim3(nb(T)), dim3(256), 0, st, c->vote,
                            NO_BID, (long)T);
         hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_UNINS, 0, 1L);
-        hipLaunchKernelGGL(k_dt_vote, dim3(nb(n)), dim3(256), 0, st, c->P, n, c->tv, c->loc,
-                           c->vote, c->cnt);
+        if (T <= kVoteLds)
+            hipLaunchKernelGGL(k_dt_vote_lds, dim3((unsigned)std::min<long>(256, (n + 1023) / 1024)),
+                               dim3(1024), 0, st, c->P, n, T, c->tv, c->loc, c->vote, c->cnt);
+        else
+            hipLaunchKernelGGL(k_dt_vote, dim3(nb(n)), dim3(256), 0, st, c->P, n, c->tv, c->loc,
+                               c->vote, c->cnt);
         hipLaunchKernelGGL(k_dt_split_flags, dim3(nb(T)), dim3(256), 0, st, c->vote, T, c->flag);
         launch_exclusive_scan(st, c->flag, T, c->pos);
         DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "counters");
@@ -628,38 +795,39 @@ extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, 
         const int splits = h[NUM_C];
         if (T + 2 * splits > c->Tmax) return fail(CVT_ERR_CAPACITY, "triangle capacity");
         hipLaunchKernelGGL(k_dt_split, dim3(nb(T)), dim3(256), 0, st, c->P, n, T, c->tv, c->to,
-                           c->vote, c->pos, c->loc, c->cnt);
+                           c->vote, c->pos, c->loc, c->chk, round + 1, c->cnt);
         hipLaunchKernelGGL(k_dt_relocate_split, dim3(nb(n)), dim3(256), 0, st, c->P, n, T, c->tv,
                            c->vote, c->pos, c->loc, c->cnt);
         T += 2 * splits;
         ++c->stats[0];
-        // Lawson flips to Delaunay (each pass flips at least the lowest
-        // illegal edge; the bound only guards against a broken mesh)
-        for (int pass = 0;; ++pass) {
+        // Lawson flips to Delaunay.  Each pass flips at least the lowest
+        // illegal edge; passes are enqueued kFlipChunk at a time and the
+        // flips of the chunk's last pass read back (a pass with nothing to
+        // check costs only its launches).  The bound guards a broken mesh.
+        constexpr int kFlipChunk = 4;
+        for (int pass = 0;; pass += kFlipChunk) {
             if (pass > 4 * c->Tmax + 64)
                 return fail(CVT_ERR_STATE, "flipping did not converge");
-            ++round;
-            hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(T)), dim3(256), 0, st, c->fvote, NO_FLIP,
-                               (long)T);
-            hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_FLIPS, 0, 1L);
-            hipLaunchKernelGGL(k_dt_flip_mark, dim3(nb(T)), dim3(256), 0, st, c->P, n, T, c->tv,
-                               c->to, c->fvote, c->cnt);
-            hipLaunchKernelGGL(k_dt_flip, dim3(nb(T)), dim3(256), 0, st, T, c->tv, c->to, c->fvote,
-                               c->fwd, c->fstamp, c->fdiag, round, c->cnt);
-            hipLaunchKernelGGL(k_dt_flip_fix, dim3(nb(T)), dim3(256), 0, st, T, c->to, c->fwd,
-                               c->fstamp, round);
-            hipLaunchKernelGGL(k_dt_relocate_flip, dim3(nb(n)), dim3(256), 0, st, c->P, n, c->loc,
-                               c->fstamp, c->fdiag, round, c->cnt);
+            for (int q = 0; q < kFlipChunk; ++q) {
+                ++round;
+                hipLaunchKernelGGL(k_dt_flip_mark, dim3(nb(T)), dim3(256), 0, st, c->P, n, T, c->tv,
+                                   c->to, c->chk, round, c->fvote, c->cnt);
+                hipLaunchKernelGGL(k_dt_flip, dim3(nb(T)), dim3(256), 0, st, T, c->tv, c->to,
+                                   c->fvote, c->fwd, c->fstamp, c->fdiag, c->chk, round, c->cnt);
+                hipLaunchKernelGGL(k_dt_flip_fix, dim3(nb(std::max(T, n))), dim3(256), 0, st, c->P, n,
+                                   T, c->to, c->fwd, c->fstamp, c->fdiag, c->fvote, c->loc, round,
+                                   c->cnt);
+                ++c->stats[1];
+            }
             DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st),
                  "counters");
             DCHK(hipStreamSynchronize(st), "flip");
-            ++c->stats[1];
-            c->stats[2] += h[C_FLIPS];
             if (h[C_FLIPS] == 0 || h[C_ERR]) break;
         }
         if (h[C_ERR]) break;
     }
     DCHK(hipGetLastError(), "launch");
+    c->stats[2] = h[C_FTOT];
     c->stats[3] = h[C_DUP];
     if (h[C_ERR] & DT_ERR_RANGE)
         return fail(CVT_ERR_ARG, "coordinates span more than 2^71 units of their finest bit");

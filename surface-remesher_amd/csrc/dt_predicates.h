@@ -21,8 +21,10 @@
 
 #if defined(__HIPCC__)
 #define DT_HD __host__ __device__ inline
+#define DT_HD_COLD __host__ __device__ inline __attribute__((noinline))
 #else
 #define DT_HD inline
+#define DT_HD_COLD inline __attribute__((noinline))
 #endif
 
 namespace cvt {
@@ -171,7 +173,9 @@ template <int K> DT_HD bool to_integers(const double (&x)[K], i128 (&X)[K])
     return true;
 }
 
-DT_HD int orient_exact(double ax, double ay, double bx, double by, double cx, double cy, int *err)
+// the exact stages are rarely taken: kept out of line so that the callers'
+// filters stay small
+DT_HD_COLD int orient_exact(double ax, double ay, double bx, double by, double cx, double cy, int *err)
 {
     const double x[6] = {ax, ay, bx, by, cx, cy};
     i128 X[6];
@@ -199,7 +203,7 @@ DT_HD int orient2d(double ax, double ay, double bx, double by, double cx, double
     return orient_exact(ax, ay, bx, by, cx, cy, err);
 }
 
-DT_HD int incircle_exact(const double (&x)[8], int *err)
+DT_HD_COLD int incircle_exact(const double (&x)[8], int *err)
 {
     i128 X[8];
     if (!to_integers<8>(x, X)) {
