@@ -302,13 +302,16 @@ int cvt_batch_synchronize(cvt_batch *batch);
  * gDel2D's TriOpp encoding ((triangle << 4) | edge, edge i opposite vertex
  * i; -1 on the convex hull; GDel2DOutput::triOppVec).  *num_tri receives the
  * count (CVT_ERR_CAPACITY when it exceeds capacity triangles).  Predicates
- * are exact; exact duplicate points are dropped (cvt_dt_stats counts them);
- * CVT_ERR_ARG for non-finite points, for inputs that are all collinear or
- * put an inserted point exactly on an edge, and for coordinates whose
- * magnitudes span more than 2^71 units of their finest bit.  In general
- * position (no three points collinear, no four cocircular) the result is
- * the unique Delaunay triangulation.  Arrays may be host or device memory;
- * the call returns when the output is written.
+ * are exact, and exact degeneracies (collinear, cocircular points) are
+ * decided by a symbolic perturbation of the points by index, as gDel2D's
+ * SoS predicates do: the result is a Delaunay triangulation holding every
+ * distinct point, with zero-area triangles along collinear hull points; in
+ * general position (no three points collinear, no four cocircular) it is
+ * the unique one.  Exact duplicate points are dropped (cvt_dt_stats counts
+ * them).  CVT_ERR_ARG for non-finite points, for inputs whose points are
+ * all collinear, and for coordinates whose magnitudes span more than 2^71
+ * units of their finest bit.  Arrays may be host or device memory; the
+ * call returns when the output is written.
  * --------------------------------------------------------------------- */
 typedef struct cvt_dt cvt_dt;
 int cvt_dt_create(cvt_dt **out, int max_points);

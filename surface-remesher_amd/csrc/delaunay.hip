@@ -24,10 +24,16 @@
 //             through per-edge forwarding records, and points follow their
 //             triangle across the new diagonal.  The lowest illegal edge
 //             always wins, so every round makes progress.
-// Predicates are exact (dt_predicates.h).  Inputs must be in general position
-// for the result to be the unique Delaunay triangulation: exactly collinear
-// insertions are reported as degenerate; four exactly cocircular points give
-// one of the valid triangulations.  Exact duplicate points are dropped.
+// Predicates are exact, and exactly degenerate cases (three collinear, four
+// cocircular points) are decided by a symbolic perturbation of the points
+// by index (Simulation of Simplicity, dt_predicates.h), as gDel2D does
+// (source/gDel2D/gDel2D/GPU/KerPredWrapper.h:69-136, :203-333): the result
+// is the Delaunay triangulation of the perturbed points, a Delaunay
+// triangulation of the input (every circumcircle empty of points in its
+// interior) that includes every distinct point; collinear points on the
+// convex hull give zero-area triangles along it.  In general position it is
+// the unique Delaunay triangulation.  Exact duplicate points are dropped
+// (the lowest index of each set is kept).
 //
 // Output: the finite triangles, counterclockwise, vertex indices into the
 // input; neighbours as gDel2D's TriOpp encoding ((triangle << 4) | edge,
@@ -55,7 +61,15 @@ struct Pt {
     double x, y;
 };
 
+// orientation of indexed points under the perturbation (never 0 for
+// distinct points)
 __device__ __forceinline__ int orient_p(const Pt *P, int a, int b, int c, int *err)
+{
+    return orient2d_sos(P[a].x, P[a].y, P[b].x, P[b].y, P[c].x, P[c].y, a, b, c, err);
+}
+
+// the exact, unperturbed orientation
+__device__ __forceinline__ int orient_x(const Pt *P, int a, int b, int c, int *err)
 {
     return orient2d(P[a].x, P[a].y, P[b].x, P[b].y, P[c].x, P[c].y, err);
 }
@@ -208,7 +222,7 @@ __global__ void k_dt_init(Pt *__restrict__ P, int n, int *__restrict__ tv, int *
         set_flag(err, DT_ERR_DEGENERATE);
         return;
     }
-    const int o = orient_p(P, a, b, c, err);
+    const int o = orient_x(P, a, b, c, err);
     if (o == 0) {
         set_flag(err, DT_ERR_DEGENERATE);                  // all points collinear
         return;
@@ -221,8 +235,8 @@ __global__ void k_dt_init(Pt *__restrict__ P, int n, int *__restrict__ tv, int *
     O.x = (P[a].x + P[b].x + P[c].x) / 3.0;
     O.y = (P[a].y + P[b].y + P[c].y) / 3.0;
     P[n] = O;
-    if (orient_p(P, a, b, n, err) <= 0 || orient_p(P, b, c, n, err) <= 0 ||
-        orient_p(P, c, a, n, err) <= 0) {
+    if (orient_x(P, a, b, n, err) <= 0 || orient_x(P, b, c, n, err) <= 0 ||
+        orient_x(P, c, a, n, err) <= 0) {
         set_flag(err, DT_ERR_DEGENERATE);                  // a sliver first triangle
         return;
     }
@@ -489,8 +503,8 @@ __device__ __forceinline__ bool illegal(const Pt *P, int n, const int *v, int w,
     if (w == n) return false;
     const int k = inf_pos(v, n);
     if (k < 0)
-        return incircle(P[v[0]].x, P[v[0]].y, P[v[1]].x, P[v[1]].y, P[v[2]].x, P[v[2]].y, P[w].x,
-                        P[w].y, err) > 0;
+        return incircle_sos(P[v[0]].x, P[v[0]].y, P[v[1]].x, P[v[1]].y, P[v[2]].x, P[v[2]].y,
+                            P[w].x, P[w].y, v[0], v[1], v[2], w, err) > 0;
     return orient_p(P, v[(k + 1) % 3], v[(k + 2) % 3], w, err) > 0;
 }
 

@@ -253,5 +253,142 @@ DT_HD int incircle(double ax, double ay, double bx, double by, double cx, double
     return incircle_exact(x, err);
 }
 
+
+// ---------------------------------------------------------------------------
+// Simulation of Simplicity (Edelsbrunner and Muecke, ACM TOG 9(1), 1990):
+// where a predicate is exactly zero, every point i is taken as moved by
+// infinitesimals, coordinate c by eps^(2^m(i, c)) with m(i, y) = 3i,
+// m(i, x) = 3i + 1 and, for the in-circle lifting z = x^2 + y^2,
+// m(i, z) = 3i + 2 (lower indices move more; y before x before z).  The
+// perturbed determinant is a polynomial in eps whose terms, largest first,
+// have as coefficients +-minors of the unperturbed matrix; the first
+// non-zero one gives the sign.  For points sorted by index the order of
+// the terms is fixed (the tables below: sign, rows kept, columns kept), so
+// no three points are ever collinear and no four cocircular, and
+// orientation and in-circle answer for the same perturbed points.
+// ---------------------------------------------------------------------------
+enum { SOS_ORIENT = 0, SOS_DIFF = 1, SOS_ONE = 2 };
+enum { PL_XZ = 0, PL_YZ = 1, PL_XY = 2 };      // planes for SOS_ORIENT
+enum { CO_X = 0, CO_Y = 1, CO_Z = 2 };         // coordinates for SOS_DIFF
+
+struct SosTerm {
+    signed char sign, kind, r0, r1, r2, what;
+};
+
+// orientation of rows (0, 1, 2) of [x y 1] after the determinant itself
+static constexpr SosTerm kOrientSos[] = {
+    {-1, SOS_DIFF, 1, 2, 0, CO_X},
+    {+1, SOS_DIFF, 1, 2, 0, CO_Y},
+    {+1, SOS_DIFF, 0, 2, 0, CO_X},
+    {+1, SOS_ONE, 0, 0, 0, 0},
+};
+
+// in-circle of rows (0, 1, 2, 3) of [x y z 1] after the determinant (terms
+// whose minor repeats an earlier one are left out: they are zero when
+// reached)
+static constexpr SosTerm kIncircleSos[] = {
+    {-1, SOS_ORIENT, 1, 2, 3, PL_XZ},
+    {+1, SOS_ORIENT, 1, 2, 3, PL_YZ},
+    {+1, SOS_ORIENT, 1, 2, 3, PL_XY},
+    {+1, SOS_ORIENT, 0, 2, 3, PL_XZ},
+    {+1, SOS_DIFF, 2, 3, 0, CO_Z},
+    {-1, SOS_DIFF, 2, 3, 0, CO_X},
+    {-1, SOS_ORIENT, 0, 2, 3, PL_YZ},
+    {+1, SOS_DIFF, 2, 3, 0, CO_Y},
+    {-1, SOS_ORIENT, 0, 2, 3, PL_XY},
+    {-1, SOS_ORIENT, 0, 1, 3, PL_XZ},
+    {-1, SOS_DIFF, 1, 3, 0, CO_Z},
+    {+1, SOS_DIFF, 1, 3, 0, CO_X},
+    {+1, SOS_DIFF, 0, 3, 0, CO_Z},
+    {+1, SOS_ONE, 0, 0, 0, 0},
+};
+
+// sorts k indices ascending, permuting the coordinate pairs alongside;
+// returns the parity of the permutation (+1 / -1)
+template <int K> DT_HD int sort_by_index(int (&idx)[K], double (&x)[2 * K])
+{
+    int par = 1;
+    for (int i = 1; i < K; ++i)
+        for (int j = i; j > 0 && idx[j - 1] > idx[j]; --j) {
+            const int ti = idx[j]; idx[j] = idx[j - 1]; idx[j - 1] = ti;
+            const double tx = x[2 * j], ty = x[2 * j + 1];
+            x[2 * j] = x[2 * j - 2]; x[2 * j + 1] = x[2 * j - 1];
+            x[2 * j - 2] = tx; x[2 * j - 1] = ty;
+            par = -par;
+        }
+    return par;
+}
+
+DT_HD_COLD int orient_sos(const double (&x)[6])
+{
+    for (const SosTerm &t : kOrientSos) {
+        if (t.kind == SOS_ONE) return t.sign;
+        const double a = x[2 * t.r0 + t.what], b = x[2 * t.r1 + t.what];
+        if (a != b) return a > b ? t.sign : -t.sign;
+    }
+    return 1;
+}
+
+// exact sign of a 2x2 orientation in a plane of (X, Y, Z) among rows r0..r2
+DT_HD int lifted_orient(const Big<4> (&U)[4][3], int r0, int r1, int r2, int plane)
+{
+    const int ca = plane == PL_YZ ? 1 : 0, cb = plane == PL_XY ? 1 : 2;
+    const Big<4> a0 = big_sub(U[r0][ca], U[r2][ca]), a1 = big_sub(U[r1][ca], U[r2][ca]);
+    const Big<4> b0 = big_sub(U[r0][cb], U[r2][cb]), b1 = big_sub(U[r1][cb], U[r2][cb]);
+    return big_sign(big_sub(big_mul<4>(a0, b1), big_mul<4>(b0, a1)));
+}
+
+DT_HD_COLD int incircle_sos(const double (&x)[8], int *err)
+{
+    i128 X[8];
+    if (!to_integers<8>(x, X)) {
+        set_flag(err, DT_ERR_RANGE);
+        return 0;
+    }
+    // columns x, y, z = x^2 + y^2 at the common scale, as 256-bit integers
+    Big<4> U[4][3];
+    for (int r = 0; r < 4; ++r) {
+        U[r][0] = big_from<4>(X[2 * r]);
+        U[r][1] = big_from<4>(X[2 * r + 1]);
+        const Big<4> xx = big_mul<2>(big_from<2>(X[2 * r]), big_from<2>(X[2 * r]));
+        const Big<4> yy = big_mul<2>(big_from<2>(X[2 * r + 1]), big_from<2>(X[2 * r + 1]));
+        U[r][2] = big_add(xx, yy);
+    }
+    for (const SosTerm &t : kIncircleSos) {
+        if (t.kind == SOS_ONE) return t.sign;
+        int s;
+        if (t.kind == SOS_DIFF)
+            s = big_sign(big_sub(U[t.r0][t.what], U[t.r1][t.what]));
+        else
+            s = lifted_orient(U, t.r0, t.r1, t.r2, t.what);
+        if (s) return s * t.sign;
+    }
+    return 1;
+}
+
+// orient2d and incircle of indexed points, never zero for distinct points
+// (an exactly degenerate case is decided by the perturbation above)
+DT_HD int orient2d_sos(double ax, double ay, double bx, double by, double cx, double cy, int ia,
+                       int ib, int ic, int *err)
+{
+    const int o = orient2d(ax, ay, bx, by, cx, cy, err);
+    if (o) return o;
+    int idx[3] = {ia, ib, ic};
+    double x[6] = {ax, ay, bx, by, cx, cy};
+    const int par = sort_by_index<3>(idx, x);
+    return par * orient_sos(x);
+}
+
+DT_HD int incircle_sos(double ax, double ay, double bx, double by, double cx, double cy,
+                       double dx, double dy, int ia, int ib, int ic, int id, int *err)
+{
+    const int o = incircle(ax, ay, bx, by, cx, cy, dx, dy, err);
+    if (o) return o;
+    int idx[4] = {ia, ib, ic, id};
+    double x[8] = {ax, ay, bx, by, cx, cy, dx, dy};
+    const int par = sort_by_index<4>(idx, x);
+    return par * incircle_sos(x, err);
+}
+
 }  // namespace dt
 }  // namespace cvt

@@ -9,7 +9,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from dt_cases import exact_incircle, exact_orient, records
+from dt_cases import (degenerate_records, exact_incircle, exact_orient, records, sos_incircle,
+                      sos_orient)
 
 ROOT = Path(__file__).resolve().parent.parent
 HARNESS = r'''
@@ -28,7 +29,15 @@ int main(int argc, char **argv) {
         int err = 0;
         const int o = cvt::dt::orient2d(x[0], x[1], x[2], x[3], x[4], x[5], &err);
         const int c = cvt::dt::incircle(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], &err);
-        std::printf("%d %d %d\n", o, c, err);
+        // indices for the perturbation: a permutation of 10, 20, 30, 40 by record
+        static const int perms[6][4] = {{10, 20, 30, 40}, {40, 30, 20, 10}, {20, 40, 10, 30},
+                                        {30, 10, 40, 20}, {10, 40, 20, 30}, {40, 20, 30, 10}};
+        const int *id = perms[i % 6];
+        const int so = cvt::dt::orient2d_sos(x[0], x[1], x[2], x[3], x[4], x[5], id[0], id[1],
+                                             id[2], &err);
+        const int sc = cvt::dt::incircle_sos(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], id[0],
+                                             id[1], id[2], id[3], &err);
+        std::printf("%d %d %d %d %d\n", o, c, err, so, sc);
     }
     return 0;
 }
@@ -52,7 +61,7 @@ def harness(tmp_path_factory):
         np.ascontiguousarray(q, np.float64).tofile(p)
         out = subprocess.run([str(exe), str(p)], capture_output=True, text=True, check=True,
                              timeout=120).stdout.split()
-        return np.array(out, dtype=np.int64).reshape(-1, 3)
+        return np.array(out, dtype=np.int64).reshape(-1, 5)
     return run
 
 
@@ -84,3 +93,25 @@ def test_range_limit_reported(harness):
     q = np.array([[1e-30, 1e-30, 1e30, 1e30, 2e30, 2e30, 0.0, 0.0]])
     got = harness(q)
     assert got[0, 2] & 2
+
+
+PERMS = [(10, 20, 30, 40), (40, 30, 20, 10), (20, 40, 10, 30), (30, 10, 40, 20),
+         (10, 40, 20, 30), (40, 20, 30, 10)]
+
+
+def test_simulation_of_simplicity(harness):
+    """where the exact sign is zero, the perturbed sign equals the leading
+    term of the determinant expanded symbolically in eps (tests/dt_cases.py),
+    for six index orders; elsewhere it equals the exact sign"""
+    q = np.concatenate([degenerate_records(), records(seed=3, n_random=40)])
+    got = harness(q)
+    assert (got[:, 2] == 0).all()
+    n_deg = 0
+    for i, r in enumerate(q):
+        idx = PERMS[i % 6]
+        o, c = exact_orient(r), exact_incircle(r)
+        assert got[i, 3] == (o if o else sos_orient(r, idx[:3])), (i, r)
+        assert got[i, 4] == (c if c else sos_incircle(r, idx)), (i, r)
+        assert got[i, 3] != 0 and got[i, 4] != 0
+        n_deg += (o == 0) + (c == 0)
+    assert n_deg >= 150

@@ -20,12 +20,15 @@ def tri_set(t):
     return set(map(tuple, np.sort(np.asarray(t), axis=1).tolist()))
 
 
-def check_structure(pts, tri, opp):
+def check_structure(pts, tri, opp, flat_ok=False):
     pts = np.asarray(pts, np.float64)
     n, nt = len(pts), len(tri)
     a, b, c = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
     o = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
-    assert (o > 0).all(), "every triangle counterclockwise"
+    if flat_ok:
+        assert (o >= 0).all(), "every triangle counterclockwise or flat"
+    else:
+        assert (o > 0).all(), "every triangle counterclockwise"
     hull_edges = 0
     for t in range(nt):
         for e in range(3):
@@ -42,8 +45,8 @@ def check_structure(pts, tri, opp):
 
 
 def locally_delaunay(pts, tri, opp):
-    """every interior edge: the opposite vertex is not inside the circle
-    (exact device predicates)"""
+    """every interior edge between triangles of positive area: the opposite
+    vertex is not inside the circle (exact device predicates)"""
     from cvtamd.delaunay import predicates
     q = []
     for t in range(len(tri)):
@@ -55,8 +58,43 @@ def locally_delaunay(pts, tri, opp):
             v = tri[t]
             w = tri[m, k]
             q.append(np.concatenate([pts[v[0]], pts[v[1]], pts[v[2]], pts[w]]))
-    _, ic = predicates(np.array(q))
-    return (ic <= 0).all()
+    q = np.array(q)
+    o, ic = predicates(q)
+    # a flat triangle (zero orientation) has no finite circle to test
+    keep = o != 0
+    return (ic[keep] <= 0).all()
+
+
+def hull_area2(pts):
+    """twice the convex hull area, exactly (monotone chain in rationals)"""
+    from fractions import Fraction as F
+    P = sorted(set((F(float(x)), F(float(y))) for x, y in pts))
+
+    def cross(o, a, b):
+        return (a[0] - o[0]) * (b[1] - o[1]) - (a[1] - o[1]) * (b[0] - o[0])
+    lower, upper = [], []
+    for p in P:
+        while len(lower) >= 2 and cross(lower[-2], lower[-1], p) <= 0:
+            lower.pop()
+        lower.append(p)
+    for p in reversed(P):
+        while len(upper) >= 2 and cross(upper[-2], upper[-1], p) <= 0:
+            upper.pop()
+        upper.append(p)
+    H = lower[:-1] + upper[:-1]
+    return abs(sum(H[i][0] * H[(i + 1) % len(H)][1] - H[(i + 1) % len(H)][0] * H[i][1]
+                   for i in range(len(H))))
+
+
+def tri_area2(pts, tri):
+    from fractions import Fraction as F
+    tot = F(0)
+    for a, b, c in tri:
+        ax, ay = F(float(pts[a][0])), F(float(pts[a][1]))
+        bx, by = F(float(pts[b][0])), F(float(pts[b][1]))
+        cx, cy = F(float(pts[c][0])), F(float(pts[c][1]))
+        tot += (bx - ax) * (cy - ay) - (by - ay) * (cx - ax)
+    return tot
 
 
 def test_device_predicates_exact(gpu):
@@ -152,3 +190,73 @@ def test_errors(gpu):
             d.compute(np.zeros((101, 2)))                                          # capacity
         tri, _ = d.compute(np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]))
         assert tri.tolist() == [[0, 1, 2]]
+
+
+def circle_lattice_points(R):
+    """every integer point of the circle x^2 + y^2 = R^2"""
+    pts = []
+    for x in range(-R, R + 1):
+        y2 = R * R - x * x
+        y = int(round(y2 ** 0.5))
+        if y * y == y2:
+            pts.append((x, y))
+            if y:
+                pts.append((x, -y))
+    return np.array(pts, np.float64)
+
+
+@pytest.mark.parametrize("case", ["grid", "grid_subset", "circle", "circle_center", "lines"])
+def test_degenerate_inputs(gpu, case):
+    """collinear and cocircular points everywhere: every distinct point is a
+    vertex, triangles are counterclockwise or flat, the links agree, the
+    triangles tile the convex hull exactly, and every interior edge between
+    triangles of positive area is Delaunay (exact predicates)"""
+    from cvtamd.delaunay import Delaunay
+    g = np.random.default_rng(7)
+    if case == "grid":
+        xs, ys = np.meshgrid(np.arange(30.0), np.arange(30.0))
+        pts = np.stack([xs.ravel(), ys.ravel()], 1)
+    elif case == "grid_subset":
+        cells = g.choice(200 * 200, 5000, replace=False)
+        pts = np.stack([cells % 200, cells // 200], 1).astype(np.float64) * 0.5 + 3.0
+    elif case == "circle":
+        pts = circle_lattice_points(5525) + 10000.0           # 108 cocircular points
+    elif case == "circle_center":
+        pts = np.concatenate([circle_lattice_points(5525), [[0.0, 0.0], [1.0, 2.0]]])
+    else:
+        t = np.arange(200.0)
+        pts = np.concatenate([np.stack([t, 0 * t], 1), np.stack([0 * t, t], 1)[1:],
+                              np.stack([t, t], 1)[1:], np.stack([t, 199 - t], 1)[1:199]])
+    g.shuffle(pts)
+    print(f"\ncase {case} n={len(pts)}", flush=True)
+    with Delaunay(len(pts)) as d:
+        tri, opp = d.compute(pts)
+        st = d.stats()
+    print(st, flush=True)
+    assert st["duplicates"] == 0
+    assert set(np.unique(tri)) == set(range(len(pts)))
+    check_structure(pts, tri, opp, flat_ok=True)
+    assert tri_area2(pts, tri) == hull_area2(pts)
+    assert locally_delaunay(pts, tri, opp)
+
+
+def test_extracted_sites(gpu, orc):
+    """the CDT input points of a compat-mode run (self-labelled pixels of the
+    label map scaled to the mesh's coordinates, delaunay.h:48-58), which lie
+    on a lattice: triangulated as GpuDel::compute takes them (main.cpp:236,
+    unconstrained)"""
+    from cvtamd import compat
+    from cvtamd.delaunay import delaunay
+    from test_gpu_compat import reference_init
+    n, num = 256, 600
+    dens = np.ones((n, n), np.float32)
+    mask = np.zeros((n, n), np.uint8)
+    vor = reference_init(orc, dens, mask, num)
+    it, ns = compat.gcvt(vor, dens, mask, 1, 200)
+    sites = compat.extract_points(vor, mask, 1.0 / 255, -0.5, 0.25)
+    assert len(sites) == ns
+    tri, opp = delaunay(sites)
+    assert set(np.unique(tri)) == set(range(len(sites)))
+    check_structure(sites, tri, opp, flat_ok=True)
+    assert tri_area2(sites, tri) == hull_area2(sites)
+    assert locally_delaunay(sites, tri, opp)
