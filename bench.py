@@ -254,6 +254,52 @@ def raster_sample(with_cpu: bool) -> dict:
     return res
 
 
+def delaunay_sample(n: int, with_cpu: bool, sites=None) -> dict:
+    """The GPU Delaunay triangulation (cvt_dt_compute, gDel2D's GpuDel::compute
+    without constraints) of n random points (float32 values in [0, 4096)^2,
+    held as float64 in device memory; the call includes its host round trips
+    and returns when the triangles and links are written to host arrays),
+    best of 3; and, when given, of the CVT sites of this run.  Baseline:
+    scipy.spatial.Delaunay (Qhull, one host thread) on the same points, whose
+    triangle set must be the same."""
+    import numpy as np
+    import torch
+    from cvtamd.delaunay import Delaunay
+    g = np.random.default_rng(n)
+    pts = (g.random((n, 2)) * 4096.0).astype(np.float32).astype(np.float64)
+    dp = torch.from_numpy(pts).cuda()
+    with Delaunay(n) as d:
+        d.compute(dp)
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tri, _ = d.compute(dp)
+            ts.append(time.perf_counter() - t0)
+        st = d.stats()
+    ms = min(ts) * 1e3
+    res = {"points": n, "triangles": int(len(tri)), "gpu_ms": round(ms, 3),
+           "mpoints_per_s": round(n / ms / 1e3, 2), **st}
+    if sites is not None:
+        sp = torch.from_numpy(np.ascontiguousarray(sites, np.float64)).cuda()
+        with Delaunay(len(sites)) as d:
+            d.compute(sp)
+            t0 = time.perf_counter()
+            tri_s, _ = d.compute(sp)
+            res["sites"] = {"points": int(len(sites)), "triangles": int(len(tri_s)),
+                            "gpu_ms": round((time.perf_counter() - t0) * 1e3, 3)}
+    if with_cpu:
+        from scipy.spatial import Delaunay as QDelaunay
+        t0 = time.perf_counter()
+        ref = QDelaunay(pts).simplices
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        same = set(map(tuple, np.sort(tri, 1).tolist())) == set(map(tuple, np.sort(ref, 1).tolist()))
+        res.update({"scipy_ms": round(cpu_ms, 1), "scipy_threads": 1,
+                    "cpu_kind": "scipy.spatial.Delaunay (Qhull)", "same_triangles": same,
+                    "speedup": round(cpu_ms / ms, 1)})
+    return res
+
+
 def locate_sample(with_cpu: bool) -> dict:
     """recover()'s point location on one patch mesh: 49152 queries (16384
     sites + 32768 triangle centroids) in a 11858-face mesh, on the GPU
@@ -412,6 +458,10 @@ def main():
                     help="MFMA throughput of the warm-start network at batch B (0 = skip)")
     ap.add_argument("--unet-only", action="store_true",
                     help="only the network measurement (for profiling its kernels)")
+    ap.add_argument("--dt", type=int, default=1000000, metavar="N",
+                    help="GPU Delaunay triangulation of N random points (0 = skip)")
+    ap.add_argument("--dt-only", action="store_true",
+                    help="only the triangulation measurement (for profiling its kernels)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the CVT step, time a plain device copy of MIB MiB "
                          "(calibrates the HBM byte counters against a known size)")
@@ -434,6 +484,10 @@ def main():
         return copy_probe(args)
     if args.unet_only:
         print(json.dumps({"mfma": unet_mfma(max(args.unet, 1))}), flush=True)
+        return
+    if args.dt_only:
+        print(json.dumps({"delaunay": delaunay_sample(max(args.dt, 3), not args.no_cpu)}),
+              flush=True)
         return
     if args.workload == "c4":
         return bench_batch(args)
@@ -474,6 +528,7 @@ def main():
     jfa_gbs = jfa_bytes / (launch_ms * 1e-3) / 1e9
     iter_bytes = n * n * (8.0 * P + 8.0)           # N^2 (8P + 8)
     iter_gbs = iter_bytes / (ms_step * 1e-3) / 1e9
+    final_sites = eng.sites()
     eng.close()
 
     out = {
@@ -515,6 +570,8 @@ def main():
         out["weights"] = weights_sample(not args.no_cpu)
     if args.unet > 0 and args.workload == "c2":
         out["mfma"] = unet_mfma(args.unet)
+    if args.dt > 0 and args.workload == "c2":
+        out["delaunay"] = delaunay_sample(args.dt, not args.no_cpu, final_sites)
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_seconds)
         out["cpu_baseline"] = cb
