@@ -317,6 +317,16 @@ typedef struct cvt_dt cvt_dt;
 int cvt_dt_create(cvt_dt **out, int max_points);
 int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, int *tri, int *opp,
                    int capacity, int *num_tri);
+/* The same with constraint segments (GDel2DInput::constraintVec; gDel2D's
+ * doInsertConstraints, GpuDelaunay.cu:638-747): num_segments pairs of point
+ * indices that become edges of the result, a constrained Delaunay
+ * triangulation (every other edge has an empty circle among the points
+ * visible across it).  Their links in opp carry bit 2, as gDel2D's TriOpp
+ * marks constraints.  Segments may share end points; segments that cross
+ * each other or pass through a point give CVT_ERR_ARG. */
+int cvt_dt_compute_constrained(cvt_dt *dt, const double *points, int num_points,
+                               const int *segments, int num_segments, int *tri, int *opp,
+                               int capacity, int *num_tri);
 /* stats[4]: insertion rounds, flip passes, flips and dropped duplicates of
  * the last cvt_dt_compute (gDel2D's Statistics, CommonTypes.h:532) */
 int cvt_dt_stats(cvt_dt *dt, int *stats);

@@ -42,6 +42,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "../../include/cvt.h"
 #include "cvt_engine.h"
@@ -52,7 +53,7 @@ namespace cvt {
 using namespace dt;
 
 enum { C_UNINS = 0, C_DUP = 1, C_ERR = 2, C_FLIPS = 3, C_A = 4, C_B = 5, C_C = 6, C_FTOT = 7,
-       NUM_C = 8 };
+       C_MISS = 8, C_NCROSS = 9, NUM_C = 12 };
 static constexpr int kParts = 256;              // workgroups of the first-triangle search
 static constexpr unsigned long long NO_BID = ~0ull;
 static constexpr int NO_FLIP = 0x7fffffff;
@@ -75,6 +76,10 @@ __device__ __forceinline__ int orient_x(const Pt *P, int a, int b, int c, int *e
 }
 
 __device__ __forceinline__ bool same_pt(const Pt &a, const Pt &b) { return a.x == b.x && a.y == b.y; }
+
+// point locations: a triangle (>= 0), inserted (-1), or a dropped duplicate
+// of vertex v (-2 - v)
+__device__ __forceinline__ int dup_code(int v) { return -2 - v; }
 
 // block-wide count of pred, added to *counter once per block
 __device__ __forceinline__ void count_add(int *counter, bool pred)
@@ -267,7 +272,7 @@ __global__ void k_dt_locate0(const Pt *__restrict__ P, int n, int *__restrict__ 
         if (p == a || p == b || p == c) {
             t = -1;
         } else if (same_pt(P[p], P[a]) || same_pt(P[p], P[b]) || same_pt(P[p], P[c])) {
-            t = -2;
+            t = dup_code(same_pt(P[p], P[a]) ? a : same_pt(P[p], P[b]) ? b : c);
             dup = true;
         } else if (orient_p(P, a, b, p, err) >= 0 && orient_p(P, b, c, p, err) >= 0 &&
                    orient_p(P, c, a, p, err) >= 0) {
@@ -313,17 +318,17 @@ __device__ __forceinline__ void wave_min_bid(unsigned long long *vote, int t,
     }
 }
 
-// the bid of point p on its triangle (NO_BID and *dup when p duplicates one
-// of the triangle's vertices)
+// the bid of point p on its triangle (NO_BID, and dupv = the vertex, when p
+// duplicates one of the triangle's vertices)
 __device__ __forceinline__ unsigned long long point_bid(const Pt *P, int n, const int *tv, int p,
-                                                        int t, bool &dup)
+                                                        int t, int &dupv)
 {
     const int v[3] = {tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]};
     const int k = inf_pos(v, n);
-    dup = false;
+    dupv = -1;
     for (int j = 0; j < 3; ++j)
-        if (j != k && same_pt(P[p], P[v[j]])) dup = true;
-    if (dup) return NO_BID;
+        if (j != k && same_pt(P[p], P[v[j]])) dupv = v[j];
+    if (dupv >= 0) return NO_BID;
     float key;
     if (k < 0) {
         const double cx = (P[v[0]].x + P[v[1]].x + P[v[2]].x) / 3.0 - P[p].x;
@@ -350,8 +355,10 @@ __global__ __launch_bounds__(256) void k_dt_vote(const Pt *__restrict__ P, int n
     unsigned long long bid = NO_BID;
     if (p < n && loc[p] >= 0) {
         t = loc[p];
-        bid = point_bid(P, n, tv, p, t, dup);
-        if (dup) loc[p] = -2;
+        int dupv;
+        bid = point_bid(P, n, tv, p, t, dupv);
+        dup = dupv >= 0;
+        if (dup) loc[p] = dup_code(dupv);
         else live = true;
     }
     wave_min_bid(vote, t, bid, live);
@@ -379,10 +386,10 @@ __global__ __launch_bounds__(1024) void k_dt_vote_lds(const Pt *__restrict__ P, 
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         const int t = loc[p];
         if (t < 0) continue;
-        bool dup;
-        const unsigned long long bid = point_bid(P, n, tv, p, t, dup);
-        if (dup) {
-            loc[p] = -2;
+        int dupv;
+        const unsigned long long bid = point_bid(P, n, tv, p, t, dupv);
+        if (dupv >= 0) {
+            loc[p] = dup_code(dupv);
             ++dups;
         } else {
             ++live;
@@ -471,7 +478,7 @@ __global__ void k_dt_relocate_split(const Pt *__restrict__ P, int n, int T,
             const int v[3] = {tv[3 * c1 + 1], tv[3 * t], tv[3 * t + 1]};
             if (same_pt(P[q], P[p])) {
                 dup = true;
-                loc[q] = -2;
+                loc[q] = dup_code(p);
             } else {
                 const int k = inf_pos(v, n);
                 int ci = 0;
@@ -524,6 +531,7 @@ __global__ void k_dt_flip_mark(const Pt *__restrict__ P, int n, int T, const int
         const int r = to[3 * t + i];
         const int m = r >> 4;
         if (r < 0 || m < t) continue;               // each edge once, by its lower triangle
+        if (r & 4) continue;                        // a constraint edge never flips
         if (!mine && chk[m] != pass) continue;
         const int w = tv[3 * m + (r & 3)];
         if (illegal(P, n, v, w, err)) {
@@ -613,10 +621,11 @@ __global__ void k_dt_flip_fix(const Pt *__restrict__ P, int n, int T, int *__res
             for (int e = 0; e < 3; e += 2) {
                 const int r = to[3 * i + e];
                 const int m = r >> 4, k = r & 3;
+                // bit 2 marks a constraint edge on both of its sides
                 if (fstamp[m] == round)
-                    to[3 * i + e] = fwd[3 * m + k];
+                    to[3 * i + e] = fwd[3 * m + k] | (r & 4);
                 else
-                    to[3 * m + k] = (i << 4) | e;
+                    to[3 * m + k] = (i << 4) | e | (r & 4);
             }
         }
     }
@@ -628,6 +637,254 @@ __global__ void k_dt_flip_fix(const Pt *__restrict__ P, int n, int T, int *__res
             loc[i] = orient_p(P, d.x, d.y, i, cnt + C_ERR) < 0 ? d.z : d.w;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Constraint segments (gDel2D's doInsertConstraints, GpuDelaunay.cu:638-747).
+// A missing segment is walked through the triangulation: the triangles it
+// crosses (its cavity) form a polygon that the segment splits in two.  The
+// segment claims its cavity and the cavity's neighbours (lowest segment
+// index wins each triangle); a segment holding all of them replaces its
+// cavity by the segment plus a Delaunay triangulation of each side (the
+// apex of each base is the chain vertex whose circle with the base holds no
+// other chain vertex, applied recursively: Anglada, "An improved
+// incremental algorithm for constructing restricted Delaunay
+// triangulations", 1997), in the same triangle slots.  The lowest missing
+// segment always proceeds, so every round inserts at least one.  Present
+// segments get bit 2 on both sides' links; the Lawson passes that follow
+// never flip them.
+// ---------------------------------------------------------------------------
+static constexpr int kCavMax = 128;            // edges one segment may cross
+// per-segment workspace (ints): cavity slots, their saved vertices and
+// links, the two chains, the new triangles and the recursion stack
+static constexpr int kWsC = 0, kWsOV = kCavMax + 1, kWsOL = 4 * (kCavMax + 1),
+                     kWsUL = 7 * (kCavMax + 1), kWsUR = kWsUL + kCavMax + 2,
+                     kWsNT = kWsUR + kCavMax + 2, kWsST = kWsNT + 3 * (kCavMax + 1),
+                     kWs = kWsST + 2 * (kCavMax + 2);
+
+// vt[v] = a triangle incident to vertex v
+__global__ void k_dt_vertex_tri(const int *__restrict__ tv, int T, int n, int *__restrict__ vt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    for (int j = 0; j < 3; ++j) {
+        const int v = tv[3 * t + j];
+        if (v != n) vt[v] = t;
+    }
+}
+
+__device__ __forceinline__ int kept_vertex(const int *loc, int v)
+{
+    const int l = loc[v];
+    return l <= -2 ? -2 - l : v;                    // a dropped duplicate's kept twin
+}
+
+// one thread per segment: marks it when present (ncr = 0), else lists the
+// edges it crosses from a to b (ncr of them; -1 beyond kCavMax)
+__global__ void k_dt_walk(const Pt *__restrict__ P, int n, int T, const int *__restrict__ tv,
+                          int *__restrict__ to, const int *__restrict__ vt,
+                          const int *__restrict__ loc, const int *__restrict__ seg, int nseg,
+                          int2 *__restrict__ cross, int *__restrict__ ncr, int *__restrict__ cnt)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    bool missing = false;
+    if (s < nseg) {
+        int *err = cnt + C_ERR;
+        int count = 0;
+        const int a = kept_vertex(loc, seg[2 * s]), b = kept_vertex(loc, seg[2 * s + 1]);
+        if (a != b) {
+            int t = vt[a], ct = -1, ce = -1;
+            // around a, counterclockwise, until the edge ab or the wedge holding b
+            for (int it = 0; it <= T; ++it) {
+                const int i = tv[3 * t] == a ? 0 : tv[3 * t + 1] == a ? 1 : 2;
+                const int v1 = tv[3 * t + (i + 1) % 3], v2 = tv[3 * t + (i + 2) % 3];
+                if (v1 == b || v2 == b) {
+                    const int e = v1 == b ? (i + 2) % 3 : (i + 1) % 3;
+                    const int r = to[3 * t + e];
+                    atomicOr(&to[3 * t + e], 4);
+                    atomicOr(&to[3 * (r >> 4) + (r & 3)], 4);
+                    break;
+                }
+                if (v1 != n && v2 != n && orient_p(P, a, v1, b, err) > 0 &&
+                    orient_p(P, a, v2, b, err) < 0) {
+                    ct = t;
+                    ce = i;
+                    break;
+                }
+                if (it == T) set_flag(err, DT_ERR_DEGENERATE);
+                t = to[3 * t + (i + 1) % 3] >> 4;   // across (v2, a): the next one ccw
+            }
+            if (ct >= 0) {
+                missing = true;
+                // crossed edge ce of ct, as (R, L): R right of a->b, L left
+                for (int it = 0; it <= T; ++it) {
+                    if (count < kCavMax) cross[(long)s * kCavMax + count] = make_int2(ct, ce);
+                    ++count;
+                    const int r = to[3 * ct + ce], m = r >> 4, k = r & 3;
+                    if (r & 4) {
+                        set_flag(err, DT_ERR_CROSS);        // through a present segment
+                        break;
+                    }
+                    const int w = tv[3 * m + k];
+                    if (w == b) break;
+                    if (w == n || it == T) {
+                        set_flag(err, DT_ERR_DEGENERATE);
+                        break;
+                    }
+                    // m = (w, L, R) from position k: leave through (R, w) or (w, L)
+                    ce = orient_p(P, a, b, w, err) > 0 ? (k + 1) % 3 : (k + 2) % 3;
+                    ct = m;
+                }
+                if (count > kCavMax) {
+                    set_flag(err, DT_ERR_CAPACITY);
+                    count = -1;
+                }
+            }
+        }
+        ncr[s] = count;
+    }
+    count_add(cnt + C_MISS, missing);
+}
+
+// the cavity of segment s: the crossed triangles, then the one holding b
+__device__ __forceinline__ int cavity_tri(const int2 *cr, int k, int j, const int *to)
+{
+    if (j < k) return cr[j].x;
+    const int2 l = cr[k - 1];
+    return to[3 * l.x + l.y] >> 4;
+}
+
+// claims (lowest segment index) on each cavity triangle and its neighbours
+__global__ void k_dt_claim(const int *__restrict__ to, const int2 *__restrict__ cross,
+                           const int *__restrict__ ncr, int nseg, int *__restrict__ own)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg || ncr[s] <= 0) return;
+    const int k = ncr[s];
+    const int2 *cr = cross + (long)s * kCavMax;
+    for (int j = 0; j <= k; ++j) {
+        const int t = cavity_tri(cr, k, j, to);
+        atomicMin(&own[t], s);
+        for (int e = 0; e < 3; ++e) atomicMin(&own[to[3 * t + e] >> 4], s);
+    }
+}
+
+// apexes for the chain ch[0 .. len) over the base (ch[0], ch[len-1]), the
+// chain on the base's left: appends counterclockwise triangles to nt
+__device__ int triangulate_chain(const Pt *P, const int *ch, int len, int *nt, int cnt0,
+                                 int *stack, int *err)
+{
+    int sp = 0, nn = cnt0;
+    stack[sp++] = 0;
+    stack[sp++] = len - 1;
+    while (sp > 0) {
+        const int i1 = stack[--sp], i0 = stack[--sp];
+        if (i1 - i0 < 2) continue;
+        int c = i0 + 1;
+        for (int i = i0 + 2; i < i1; ++i) {
+            const int p = ch[i0], q = ch[i1], r = ch[c], x = ch[i];
+            if (incircle_sos(P[p].x, P[p].y, P[q].x, P[q].y, P[r].x, P[r].y, P[x].x, P[x].y, p,
+                             q, r, x, err) > 0)
+                c = i;
+        }
+        nt[3 * nn] = ch[i0];
+        nt[3 * nn + 1] = ch[i1];
+        nt[3 * nn + 2] = ch[c];
+        ++nn;
+        stack[sp++] = i0;
+        stack[sp++] = c;
+        stack[sp++] = c;
+        stack[sp++] = i1;
+    }
+    return nn;
+}
+
+// one thread per segment that holds its claims: the cavity replaced by the
+// segment and the triangulations of its two sides
+__global__ void k_dt_cavity(const Pt *__restrict__ P, int n, int *__restrict__ tv,
+                            int *__restrict__ to, const int *__restrict__ loc,
+                            const int *__restrict__ seg, const int2 *__restrict__ cross,
+                            const int *__restrict__ ncr, int nseg, const int *__restrict__ own,
+                            int *__restrict__ ws_all, int *__restrict__ cnt)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg || ncr[s] <= 0) return;
+    int *err = cnt + C_ERR;
+    const int k = ncr[s];
+    const int2 *cr = cross + (long)s * kCavMax;
+    int *ws = ws_all + (long)s * kWs;
+    int *C = ws + kWsC, *OV = ws + kWsOV, *OL = ws + kWsOL, *UL = ws + kWsUL, *UR = ws + kWsUR,
+        *NT = ws + kWsNT, *ST = ws + kWsST;
+    for (int j = 0; j <= k; ++j) {
+        const int t = cavity_tri(cr, k, j, to);
+        if (own[t] != s) return;
+        for (int e = 0; e < 3; ++e)
+            if (own[to[3 * t + e] >> 4] != s) return;
+        C[j] = t;
+    }
+    const int a = kept_vertex(loc, seg[2 * s]), b = kept_vertex(loc, seg[2 * s + 1]);
+    for (int j = 0; j <= k; ++j)
+        for (int e = 0; e < 3; ++e) {
+            OV[3 * j + e] = tv[3 * C[j] + e];
+            OL[3 * j + e] = to[3 * C[j] + e];
+        }
+    // the chains left and right of a -> b
+    int nl = 0, nr = 0;
+    UL[nl++] = a;
+    UR[nr++] = a;
+    for (int j = 0; j < k; ++j) {
+        const int R = OV[3 * j + (cr[j].y + 1) % 3], L = OV[3 * j + (cr[j].y + 2) % 3];
+        if (UL[nl - 1] != L) UL[nl++] = L;
+        if (UR[nr - 1] != R) UR[nr++] = R;
+    }
+    UL[nl++] = b;
+    UR[nr++] = b;
+    // the right chain, reversed, lies left of b -> a
+    for (int i = 0; i < nr / 2; ++i) {
+        const int x = UR[i];
+        UR[i] = UR[nr - 1 - i];
+        UR[nr - 1 - i] = x;
+    }
+    int nn = triangulate_chain(P, UL, nl, NT, 0, ST, err);
+    nn = triangulate_chain(P, UR, nr, NT, nn, ST, err);
+    if (nn != k + 1) {
+        set_flag(err, DT_ERR_DEGENERATE);
+        return;
+    }
+    // links: the other new triangle across an edge, else the old outer link
+    for (int e = 0; e < nn; ++e) {
+        for (int f = 0; f < 3; ++f) {
+            const int u = NT[3 * e + (f + 1) % 3], v = NT[3 * e + (f + 2) % 3];
+            int link = -1;
+            for (int g = 0; g < nn && link < 0; ++g) {
+                if (g == e) continue;
+                for (int h = 0; h < 3; ++h)
+                    if (NT[3 * g + (h + 1) % 3] == v && NT[3 * g + (h + 2) % 3] == u) {
+                        const bool ab = (u == a && v == b) || (u == b && v == a);
+                        link = (C[g] << 4) | h | (ab ? 4 : 0);
+                        break;
+                    }
+            }
+            if (link < 0) {
+                for (int j = 0; j <= k && link < 0; ++j)
+                    for (int h = 0; h < 3; ++h)
+                        if (OV[3 * j + (h + 1) % 3] == u && OV[3 * j + (h + 2) % 3] == v) {
+                            link = OL[3 * j + h];
+                            break;
+                        }
+                if (link < 0) {
+                    set_flag(err, DT_ERR_DEGENERATE);
+                    return;
+                }
+                // the outer neighbour is claimed by s: nobody else writes it
+                to[3 * (link >> 4) + (link & 3)] = (C[e] << 4) | f | (link & 4);
+            }
+            to[3 * C[e] + f] = link;
+        }
+    }
+    for (int e = 0; e < nn; ++e)
+        for (int f = 0; f < 3; ++f) tv[3 * C[e] + f] = NT[3 * e + f];
+    atomicAdd(cnt + C_FLIPS, 1);
 }
 
 __global__ void k_dt_alive(const int *__restrict__ tv, int T, int n, int *__restrict__ flag)
@@ -647,7 +904,7 @@ __global__ void k_dt_output(const int *__restrict__ tv, const int *__restrict__ 
         tri[3 * j + e] = tv[3 * t + e];
         if (opp) {
             const int r = to[3 * t + e], m = r >> 4;
-            opp[3 * j + e] = flag[m] ? (pos[m] << 4) | (r & 3) : -1;
+            opp[3 * j + e] = flag[m] ? (pos[m] << 4) | (r & 7) : -1;
         }
     }
 }
@@ -675,6 +932,13 @@ struct DtCtx {
     int *loc = nullptr;                    // [maxN]
     int *otri = nullptr, *oopp = nullptr;  // [Tmax][3] output staging
     int *cnt = nullptr;                    // [NUM_C]
+    int *vt = nullptr;                     // [maxN] a triangle per vertex (constraints)
+    // constraint buffers, sized by the largest segment count seen:
+    // segments [2], crossings [kCavMax], crossing counts, workspace [kWs]
+    char *cons = nullptr;
+    int segCap = 0;
+    int *seg = nullptr, *ncr = nullptr, *ws = nullptr, *own = nullptr;
+    int2 *cross = nullptr;
     int *part = nullptr;                   // [3 kParts] first-triangle search
     double *partv = nullptr;               // [kParts]
     int *hcnt = nullptr;                   // pinned [NUM_C + 1]
@@ -684,6 +948,7 @@ struct DtCtx {
 
 static void dt_free(DtCtx *c)
 {
+    if (c->cons) (void)hipFree(c->cons);
     if (c->arena) (void)hipFree(c->arena);
     if (c->hcnt) (void)hipHostFree(c->hcnt);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -728,6 +993,7 @@ extern "C" int cvt_dt_create(cvt_dt **out, int max_points)
         carve(c->fdiag, T, set); carve(c->loc, N, set); carve(c->cnt, NUM_C, set);
         carve(c->otri, 3 * T, set); carve(c->oopp, 3 * T, set);
         carve(c->part, 3 * kParts, set); carve(c->partv, kParts, set);
+        carve(c->vt, N, set);
     };
     layout(false);
     if (hipMalloc(&c->arena, at) != hipSuccess) {
@@ -751,8 +1017,74 @@ extern "C" int cvt_dt_destroy(cvt_dt *dt)
     return CVT_OK;
 }
 
-extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, int *tri,
-                              int *opp, int capacity, int *num_tri)
+// Lawson flips to Delaunay over T triangles.  Each pass flips at least the
+// lowest illegal edge; passes are enqueued kFlipChunk at a time and the
+// flips of the chunk's last pass read back (a pass with nothing to check
+// costs only its launches).  The bound guards a broken mesh.
+static int flip_to_delaunay(DtCtx *c, int n, int T, int &round)
+{
+    hipStream_t st = c->st;
+    int *h = c->hcnt;
+    constexpr int kFlipChunk = 4;
+    for (int pass = 0;; pass += kFlipChunk) {
+        if (pass > 4 * c->Tmax + 64) return fail(CVT_ERR_STATE, "flipping did not converge");
+        for (int q = 0; q < kFlipChunk; ++q) {
+            ++round;
+            hipLaunchKernelGGL(k_dt_flip_mark, dim3(nb(T)), dim3(256), 0, st, c->P, n, T, c->tv,
+                               c->to, c->chk, round, c->fvote, c->cnt);
+            hipLaunchKernelGGL(k_dt_flip, dim3(nb(T)), dim3(256), 0, st, T, c->tv, c->to,
+                               c->fvote, c->fwd, c->fstamp, c->fdiag, c->chk, round, c->cnt);
+            hipLaunchKernelGGL(k_dt_flip_fix, dim3(nb(std::max(T, n))), dim3(256), 0, st, c->P, n,
+                               T, c->to, c->fwd, c->fstamp, c->fdiag, c->fvote, c->loc, round,
+                               c->cnt);
+            ++c->stats[1];
+        }
+        DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "counters");
+        DCHK(hipStreamSynchronize(st), "flip");
+        if (h[C_FLIPS] == 0 || h[C_ERR]) return CVT_OK;
+    }
+}
+
+// the constraint segments as edges (gDel2D's doInsertConstraints), then the
+// constrained Lawson flips
+static int insert_constraints(DtCtx *c, int n, int T, int nseg, int &round)
+{
+    hipStream_t st = c->st;
+    int *h = c->hcnt;
+    for (int outer = 0;; ++outer) {
+        hipLaunchKernelGGL(k_dt_vertex_tri, dim3(nb(T)), dim3(256), 0, st, c->tv, T, n, c->vt);
+        hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_MISS, 0, 1L);
+        hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_FLIPS, 0, 1L);
+        hipLaunchKernelGGL(k_dt_walk, dim3(nb(nseg)), dim3(256), 0, st, c->P, n, T, c->tv, c->to,
+                           c->vt, c->loc, c->seg, nseg, c->cross, c->ncr, c->cnt);
+        hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(T)), dim3(256), 0, st, c->own, 0x7fffffff,
+                           (long)T);
+        hipLaunchKernelGGL(k_dt_claim, dim3(nb(nseg)), dim3(256), 0, st, c->to, c->cross, c->ncr,
+                           nseg, c->own);
+        hipLaunchKernelGGL(k_dt_cavity, dim3((nseg + 63) / 64), dim3(64), 0, st, c->P, n, c->tv, c->to,
+                           c->loc, c->seg, c->cross, c->ncr, nseg, c->own, c->ws, c->cnt);
+        DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "counters");
+        DCHK(hipStreamSynchronize(st), "constraints");
+        ++c->stats[1];
+        if (h[C_ERR] & DT_ERR_CAPACITY)
+            return fail(CVT_ERR_CAPACITY, "a constraint segment crosses more than %d edges",
+                        kCavMax);
+        if (h[C_ERR] & DT_ERR_CROSS)
+            return fail(CVT_ERR_ARG, "a constraint segment crosses another (round %d)", outer);
+        if (h[C_ERR]) return CVT_OK;                 // reported by the caller
+        if (h[C_MISS] == 0) break;
+        if (h[C_FLIPS] == 0 || outer > nseg)
+            return fail(CVT_ERR_ARG,
+                        "constraint insertion made no progress (round %d, %d segments missing, "
+                        "%d inserted)", outer, h[C_MISS], h[C_FLIPS]);
+    }
+    // every edge may have to change now: look at all triangles once
+    hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(T)), dim3(256), 0, st, c->chk, round + 1, (long)T);
+    return flip_to_delaunay(c, n, T, round);
+}
+
+static int dt_run(cvt_dt *dt, const double *points, int num_points, const int *segments,
+                  int num_segments, int *tri, int *opp, int capacity, int *num_tri)
 {
     clear_error();
     if (!dt) return fail(CVT_ERR_ARG, "null context");
@@ -761,6 +1093,44 @@ extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, 
     if (n < 3 || n > c->maxN) return fail(CVT_ERR_ARG, "num_points %d outside [3, %d]", n, c->maxN);
     if (!points || !num_tri || (capacity > 0 && !tri) || capacity < 0)
         return fail(CVT_ERR_ARG, "null array or negative capacity");
+    if (num_segments < 0 || (num_segments > 0 && !segments))
+        return fail(CVT_ERR_ARG, "null segment array or negative count");
+    if (num_segments > 0) {
+        if (num_segments > c->segCap) {
+            if (c->cons) (void)hipFree(c->cons);
+            c->cons = nullptr;
+            c->segCap = 0;
+            const size_t S = (size_t)num_segments;
+            size_t at = 0;
+            auto carve = [&](auto *&p, size_t count, bool set) {
+                at = (at + 255) & ~(size_t)255;
+                if (set) p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(c->cons + at);
+                at += count * sizeof(*p);
+            };
+            auto layout = [&](bool set) {
+                at = 0;
+                carve(c->seg, 2 * S, set); carve(c->ncr, S, set);
+                carve(c->cross, S * kCavMax, set); carve(c->ws, S * kWs, set);
+                carve(c->own, (size_t)c->Tmax, set);
+            };
+            layout(false);
+            if (hipMalloc(&c->cons, at) != hipSuccess) {
+                c->cons = nullptr;
+                return fail(CVT_ERR_NOMEM, "%zu bytes for %d constraint segments", at,
+                            num_segments);
+            }
+            layout(true);
+            c->segCap = num_segments;
+        }
+        DCHK(hipMemcpyAsync(c->seg, segments, sizeof(int) * 2 * num_segments, hipMemcpyDefault,
+                            c->st), "segment upload");
+        std::vector<int> hs(2 * (size_t)num_segments);
+        DCHK(hipMemcpyAsync(hs.data(), c->seg, sizeof(int) * hs.size(), hipMemcpyDeviceToHost,
+                            c->st), "segment check");
+        DCHK(hipStreamSynchronize(c->st), "segment check");
+        for (int v : hs)
+            if (v < 0 || v >= n) return fail(CVT_ERR_ARG, "segment vertex %d outside [0, %d)", v, n);
+    }
     hipStream_t st = c->st;
     int *h = c->hcnt;
     for (int i = 0; i < 4; ++i) c->stats[i] = 0;
@@ -814,31 +1184,13 @@ extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, 
                            c->vote, c->pos, c->loc, c->cnt);
         T += 2 * splits;
         ++c->stats[0];
-        // Lawson flips to Delaunay.  Each pass flips at least the lowest
-        // illegal edge; passes are enqueued kFlipChunk at a time and the
-        // flips of the chunk's last pass read back (a pass with nothing to
-        // check costs only its launches).  The bound guards a broken mesh.
-        constexpr int kFlipChunk = 4;
-        for (int pass = 0;; pass += kFlipChunk) {
-            if (pass > 4 * c->Tmax + 64)
-                return fail(CVT_ERR_STATE, "flipping did not converge");
-            for (int q = 0; q < kFlipChunk; ++q) {
-                ++round;
-                hipLaunchKernelGGL(k_dt_flip_mark, dim3(nb(T)), dim3(256), 0, st, c->P, n, T, c->tv,
-                                   c->to, c->chk, round, c->fvote, c->cnt);
-                hipLaunchKernelGGL(k_dt_flip, dim3(nb(T)), dim3(256), 0, st, T, c->tv, c->to,
-                                   c->fvote, c->fwd, c->fstamp, c->fdiag, c->chk, round, c->cnt);
-                hipLaunchKernelGGL(k_dt_flip_fix, dim3(nb(std::max(T, n))), dim3(256), 0, st, c->P, n,
-                                   T, c->to, c->fwd, c->fstamp, c->fdiag, c->fvote, c->loc, round,
-                                   c->cnt);
-                ++c->stats[1];
-            }
-            DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st),
-                 "counters");
-            DCHK(hipStreamSynchronize(st), "flip");
-            if (h[C_FLIPS] == 0 || h[C_ERR]) break;
-        }
+        const int rc = flip_to_delaunay(c, n, T, round);
+        if (rc) return rc;
         if (h[C_ERR]) break;
+    }
+    if (!h[C_ERR] && num_segments > 0) {
+        const int rc = insert_constraints(c, n, T, num_segments, round);
+        if (rc) return rc;
     }
     DCHK(hipGetLastError(), "launch");
     c->stats[2] = h[C_FTOT];
@@ -865,6 +1217,19 @@ extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, 
     }
     DCHK(hipStreamSynchronize(st), "output");
     return CVT_OK;
+}
+
+extern "C" int cvt_dt_compute(cvt_dt *dt, const double *points, int num_points, int *tri,
+                              int *opp, int capacity, int *num_tri)
+{
+    return dt_run(dt, points, num_points, nullptr, 0, tri, opp, capacity, num_tri);
+}
+
+extern "C" int cvt_dt_compute_constrained(cvt_dt *dt, const double *points, int num_points,
+                                          const int *segments, int num_segments, int *tri,
+                                          int *opp, int capacity, int *num_tri)
+{
+    return dt_run(dt, points, num_points, segments, num_segments, tri, opp, capacity, num_tri);
 }
 
 extern "C" int cvt_dt_stats(cvt_dt *dt, int *stats)

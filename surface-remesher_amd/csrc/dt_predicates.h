@@ -30,7 +30,8 @@
 namespace cvt {
 namespace dt {
 
-enum { DT_ERR_DEGENERATE = 1, DT_ERR_RANGE = 2, DT_ERR_CAPACITY = 4, DT_ERR_INPUT = 8 };
+enum { DT_ERR_DEGENERATE = 1, DT_ERR_RANGE = 2, DT_ERR_CAPACITY = 4, DT_ERR_INPUT = 8,
+       DT_ERR_CROSS = 16 };
 
 typedef unsigned __int128 u128;
 typedef __int128 i128;

@@ -100,6 +100,7 @@ SIGNATURES = {
     "cvt_get_tile": (_i, [_vp]),
     "cvt_dt_create": (_i, [C.POINTER(_vp), _i]),
     "cvt_dt_compute": (_i, [_vp, _vp, _i, _vp, _vp, _i, C.POINTER(_i)]),
+    "cvt_dt_compute_constrained": (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp, _i, C.POINTER(_i)]),
     "cvt_dt_stats": (_i, [_vp, _vp]),
     "cvt_dt_destroy": (_i, [_vp]),
     "cvt_dt_predicates": (_i, [_vp, _i, _vp, _vp]),

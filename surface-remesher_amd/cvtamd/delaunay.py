@@ -25,9 +25,11 @@ class Delaunay:
         call("cvt_dt_create", C.byref(self._h), int(max_points))
         self.max_points = int(max_points)
 
-    def compute(self, points, want_opp: bool = True):
+    def compute(self, points, want_opp: bool = True, segments=None):
         """float64 points (n, 2) (numpy, or a contiguous torch tensor) ->
-        (tri, opp) as int32 numpy arrays (opp None unless want_opp)."""
+        (tri, opp) as int32 numpy arrays (opp None unless want_opp).  With
+        segments (int32 (m, 2) point index pairs) the result is the
+        constrained triangulation, and opp marks constraint edges with bit 2."""
         if hasattr(points, "numel"):
             n = points.numel() // 2
         else:
@@ -38,8 +40,15 @@ class Delaunay:
         tri = np.zeros((cap, 3), np.int32)
         opp = np.zeros((cap, 3), np.int32) if want_opp else None
         nt = C.c_int(0)
-        call("cvt_dt_compute", self._h, pp, int(n), tri.ctypes.data_as(C.c_void_p),
-             opp.ctypes.data_as(C.c_void_p) if want_opp else None, cap, C.byref(nt))
+        po = opp.ctypes.data_as(C.c_void_p) if want_opp else None
+        if segments is None:
+            call("cvt_dt_compute", self._h, pp, int(n), tri.ctypes.data_as(C.c_void_p), po, cap,
+                 C.byref(nt))
+        else:
+            ps, keep_s = _ptr(segments, np.int32)
+            m = (segments.numel() if hasattr(segments, "numel") else np.asarray(segments).size) // 2
+            call("cvt_dt_compute_constrained", self._h, pp, int(n), ps, int(m),
+                 tri.ctypes.data_as(C.c_void_p), po, cap, C.byref(nt))
         k = nt.value
         return tri[:k].copy(), (opp[:k].copy() if want_opp else None)
 
@@ -67,11 +76,11 @@ class Delaunay:
             pass
 
 
-def delaunay(points, want_opp: bool = True):
+def delaunay(points, want_opp: bool = True, segments=None):
     """One-shot triangulation (a context sized for these points)."""
     n = (points.numel() // 2) if hasattr(points, "numel") else len(np.asarray(points).reshape(-1, 2))
     with Delaunay(max(n, 3)) as d:
-        return d.compute(points, want_opp)
+        return d.compute(points, want_opp, segments)
 
 
 def predicates(quads):

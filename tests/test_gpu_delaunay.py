@@ -37,7 +37,7 @@ def check_structure(pts, tri, opp, flat_ok=False):
                 hull_edges += 1
                 continue
             m, k = r >> 4, r & 3
-            assert opp[m, k] == (t << 4) | e
+            assert opp[m, k] & ~4 == (t << 4) | e          # bit 2: a constraint edge
             assert {tri[t, (e + 1) % 3], tri[t, (e + 2) % 3]} == {tri[m, (k + 1) % 3], tri[m, (k + 2) % 3]}
     used = len(np.unique(tri))
     assert nt == 2 * used - hull_edges - 2, "Euler: 2n - h - 2"
@@ -260,3 +260,122 @@ def test_extracted_sites(gpu, orc):
     check_structure(sites, tri, opp, flat_ok=True)
     assert tri_area2(sites, tri) == hull_area2(sites)
     assert locally_delaunay(sites, tri, opp)
+
+
+# ---------------------------------------------------------------- constraints --
+def edge_set(tri):
+    e = set()
+    for a, b, c in tri:
+        for u, v in ((a, b), (b, c), (c, a)):
+            e.add((min(u, v), max(u, v)))
+    return e
+
+
+def constrained_delaunay(pts, tri, opp):
+    """every interior edge that is not a constraint (bit 2) between triangles
+    of positive area: the opposite vertex is not inside the circle"""
+    from cvtamd.delaunay import predicates
+    q = []
+    for t in range(len(tri)):
+        for e in range(3):
+            r = opp[t, e]
+            if r < 0 or r & 4:
+                continue
+            m, k = r >> 4, r & 3
+            v = tri[t]
+            q.append(np.concatenate([pts[v[0]], pts[v[1]], pts[v[2]], pts[tri[m, k]]]))
+    if not q:
+        return True
+    q = np.array(q)
+    o, ic = predicates(q)
+    return (ic[o != 0] <= 0).all()
+
+
+def check_constrained(pts, segs, tri, opp):
+    check_structure(pts, tri, opp, flat_ok=True)
+    assert tri_area2(pts, tri) == hull_area2(pts)
+    es = edge_set(tri)
+    for a, b in segs:
+        assert (min(a, b), max(a, b)) in es, (a, b)
+    # bit 2 on both sides of exactly the segment edges
+    seg_set = {(min(a, b), max(a, b)) for a, b in segs}
+    for t in range(len(tri)):
+        for e in range(3):
+            r = opp[t, e]
+            u, v = tri[t, (e + 1) % 3], tri[t, (e + 2) % 3]
+            is_seg = (min(u, v), max(u, v)) in seg_set
+            if r >= 0:
+                assert bool(r & 4) == is_seg, (t, e)
+                m, k = r >> 4, r & 3
+                assert opp[m, k] == (t << 4) | e | (r & 4)
+    assert constrained_delaunay(pts, tri, opp)
+
+
+def star_polygon(g, n, c=1000.0, r0=200.0, r1=900.0):
+    """a star-shaped polygon about (c, c): its edges never cross"""
+    ang = np.sort(g.random(n)) * 2 * np.pi
+    rad = r0 + g.random(n) * (r1 - r0)
+    pts = np.stack([c + rad * np.cos(ang), c + rad * np.sin(ang)], 1)
+    pts = pts.astype(np.float32).astype(np.float64)
+    segs = np.stack([np.arange(n), (np.arange(n) + 1) % n], 1)
+    return pts, segs
+
+
+@pytest.mark.parametrize("case", ["star", "star_inside", "lattice_chords", "square_boundary"])
+def test_constrained(gpu, case):
+    from cvtamd.delaunay import Delaunay
+    g = np.random.default_rng(23)
+    if case == "star":
+        pts, segs = star_polygon(g, 400)
+    elif case == "star_inside":
+        poly, segs = star_polygon(g, 300)
+        inner = 1000.0 + (g.random((4000, 2)) - 0.5) * 2000.0
+        pts = np.concatenate([poly, inner.astype(np.float32).astype(np.float64)])
+    elif case == "lattice_chords":
+        xs, ys = np.meshgrid(np.arange(40.0), np.arange(40.0))
+        pts = np.stack([xs.ravel(), ys.ravel()], 1)
+        idx = lambda x, y: y * 40 + x
+        # parallel chords of slopes 5/13 and -2/5: no lattice point inside
+        # them (coprime steps), none crossing
+        segs = [(idx(0, y), idx(13, y + 5)) for y in range(0, 30, 4)]
+        segs += [(idx(39, y), idx(39 - 5, y + 2)) for y in range(1, 30, 3)]
+        segs = np.array(segs)
+    else:
+        # the domain boundary as constraint points and segments (delaunayInput,
+        # delaunay.h:60-79): collinear points along the hull
+        side = np.linspace(0.0, 100.0, 21)
+        ring = np.concatenate([np.stack([side, 0 * side], 1)[:-1],
+                               np.stack([0 * side + 100.0, side], 1)[:-1],
+                               np.stack([side[::-1], 0 * side + 100.0], 1)[:-1],
+                               np.stack([0 * side, side[::-1]], 1)[:-1]])
+        inner = 1.0 + g.random((2000, 2)) * 98.0
+        pts = np.concatenate([inner.astype(np.float32).astype(np.float64), ring])
+        nr = len(ring)
+        segs = np.stack([2000 + np.arange(nr), 2000 + (np.arange(nr) + 1) % nr], 1)
+    segs = np.asarray(segs, np.int32)
+    print(f"\ncase {case} n={len(pts)} segments={len(segs)}", flush=True)
+    with Delaunay(len(pts)) as d:
+        tri, opp = d.compute(pts, segments=segs)
+        print(d.stats(), flush=True)
+        tri0, _ = d.compute(pts)
+    missing = [s for s in segs if (min(s), max(s)) not in edge_set(tri0)]
+    print(f"{len(missing)} segments were not Delaunay edges", flush=True)
+    if case in ("star", "star_inside", "lattice_chords"):
+        assert len(missing) > 0
+    check_constrained(pts, segs, tri, opp)
+
+
+def test_constrained_errors(gpu):
+    import cvtamd
+    from cvtamd.delaunay import Delaunay
+    pts = np.array([[0.0, 0.0], [10.0, 0.0], [10.0, 10.0], [0.0, 10.0], [5.0, 1.0], [5.0, 9.0],
+                    [1.0, 5.0], [9.0, 5.0]])
+    with Delaunay(16) as d:
+        with pytest.raises(cvtamd.CvtError):
+            d.compute(pts, segments=np.array([[4, 5], [6, 7]], np.int32))       # they cross
+        with pytest.raises(cvtamd.CvtError):
+            d.compute(pts, segments=np.array([[0, 8]], np.int32))               # out of range
+        tri, opp = d.compute(pts, segments=np.array([[4, 5]], np.int32))
+        check_constrained(pts, [(4, 5)], tri, opp)
+        tri, opp = d.compute(pts, segments=np.zeros((0, 2), np.int32))
+        assert tri_set(tri) == tri_set(d.compute(pts)[0])
