@@ -280,6 +280,27 @@ def delaunay_sample(n: int, with_cpu: bool, sites=None) -> dict:
     ms = min(ts) * 1e3
     res = {"points": n, "triangles": int(len(tri)), "gpu_ms": round(ms, 3),
            "mpoints_per_s": round(n / ms / 1e3, 2), **st}
+    # the same points inside a square boundary of 4 x 1024 collinear constraint
+    # points joined by segments (delaunayInput's constraint points and edges)
+    side = np.arange(1024) * 4.0
+    ring = np.concatenate([np.stack([side, 0 * side - 1.0], 1),
+                           np.stack([0 * side + 4097.0, side], 1),
+                           np.stack([4096.0 - side, 0 * side + 4097.0], 1),
+                           np.stack([0 * side - 1.0, 4096.0 - side], 1)])
+    cpts = np.concatenate([pts, ring])
+    nr = len(ring)
+    segs = np.stack([n + np.arange(nr), n + (np.arange(nr) + 1) % nr], 1).astype(np.int32)
+    dc, ds = torch.from_numpy(cpts).cuda(), torch.from_numpy(segs).cuda()
+    with Delaunay(len(cpts)) as d:
+        d.compute(dc, segments=ds)
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tri_c, _ = d.compute(dc, segments=ds)
+            ts.append(time.perf_counter() - t0)
+    res["constrained"] = {"points": int(len(cpts)), "segments": int(nr),
+                          "triangles": int(len(tri_c)), "gpu_ms": round(min(ts) * 1e3, 3)}
     if sites is not None:
         sp = torch.from_numpy(np.ascontiguousarray(sites, np.float64)).cuda()
         with Delaunay(len(sites)) as d:

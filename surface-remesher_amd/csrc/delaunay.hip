@@ -11,8 +11,9 @@
 //
 // Rounds of parallel insertion, each followed by flipping to Delaunay:
 //   vote      every uninserted point bids for its triangle (the point closest
-//             to the centroid of a finite triangle, the farthest beyond the
-//             edge of an infinite one; ties by index)
+//             to the centroid of a finite triangle; beyond the edge of an
+//             infinite one, a point far from the edge and projecting near
+//             its middle; ties by index)
 //   split     each triangle with a bid splits 1 -> 3 around the winner; the
 //             two new triangles go to slots after the current ones, placed
 //             by a prefix sum over the splitting triangles
@@ -335,9 +336,17 @@ __device__ __forceinline__ unsigned long long point_bid(const Pt *P, int n, cons
         const double cy = (P[v[0]].y + P[v[1]].y + P[v[2]].y) / 3.0 - P[p].y;
         key = (float)(cx * cx + cy * cy);
     } else {
+        // beyond a hull edge u -> w: far from the edge (the hull grows fast)
+        // and projecting near its middle (points along a line, such as a
+        // side of the domain, are halved every round instead of taken one
+        // at a time from an end)
         const Pt &u = P[v[(k + 1) % 3]], &w = P[v[(k + 2) % 3]];
-        const double d = (w.x - u.x) * (P[p].y - u.y) - (w.y - u.y) * (P[p].x - u.x);
-        key = d > 0.0 ? 1.0f / (1.0f + (float)d) : 1.0f;
+        const double ex = w.x - u.x, ey = w.y - u.y;
+        const double len2 = ex * ex + ey * ey;
+        const double qx = P[p].x - u.x, qy = P[p].y - u.y;
+        const double t = len2 > 0.0 ? (qx * ex + qy * ey) / len2 : 0.5;
+        const double d = len2 > 0.0 ? fmax(ex * qy - ey * qx, 0.0) / len2 : 0.0;
+        key = (float)(1.0 / (1.0 + (d + 1e-6) / (1.0 + fabs(2.0 * t - 1.0))));
     }
     if (!(key >= 0.0f)) key = 3.0e38f;
     return ((unsigned long long)__float_as_uint(key) << 32) | (unsigned)p;
@@ -643,14 +652,14 @@ __global__ void k_dt_flip_fix(const Pt *__restrict__ P, int n, int T, int *__res
 // Constraint segments (gDel2D's doInsertConstraints, GpuDelaunay.cu:638-747).
 // A missing segment is walked through the triangulation: the triangles it
 // crosses (its cavity) form a polygon that the segment splits in two.  The
-// segment claims its cavity and the cavity's neighbours (lowest segment
-// index wins each triangle); a segment holding all of them replaces its
+// segment claims its cavity and the cavity's neighbours (the lowest of a
+// per-round hashed priority wins each triangle); a segment holding all of them replaces its
 // cavity by the segment plus a Delaunay triangulation of each side (the
 // apex of each base is the chain vertex whose circle with the base holds no
 // other chain vertex, applied recursively: Anglada, "An improved
 // incremental algorithm for constructing restricted Delaunay
-// triangulations", 1997), in the same triangle slots.  The lowest missing
-// segment always proceeds, so every round inserts at least one.  Present
+// triangulations", 1997), in the same triangle slots.  The missing segment
+// of lowest priority always proceeds, so every round inserts at least one.  Present
 // segments get bit 2 on both sides' links; the Lawson passes that follow
 // never flip them.
 // ---------------------------------------------------------------------------
@@ -754,18 +763,34 @@ __device__ __forceinline__ int cavity_tri(const int2 *cr, int k, int j, const in
     return to[3 * l.x + l.y] >> 4;
 }
 
-// claims (lowest segment index) on each cavity triangle and its neighbours
+// a segment's claim for this round: a hashed priority (so that a chain of
+// touching cavities frees about a third of its segments per round, not
+// only its lowest), then the segment index
+__device__ __forceinline__ unsigned long long claim_key(int s, int round)
+{
+    unsigned h = (unsigned)s * 0x9E3779B1u ^ (unsigned)round * 0x85EBCA6Bu;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return ((unsigned long long)h << 32) | (unsigned)s;
+}
+
+// claims (lowest key) on each cavity triangle and its neighbours
 __global__ void k_dt_claim(const int *__restrict__ to, const int2 *__restrict__ cross,
-                           const int *__restrict__ ncr, int nseg, int *__restrict__ own)
+                           const int *__restrict__ ncr, int nseg, int round,
+                           unsigned long long *__restrict__ own)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg || ncr[s] <= 0) return;
     const int k = ncr[s];
+    const unsigned long long key = claim_key(s, round);
     const int2 *cr = cross + (long)s * kCavMax;
     for (int j = 0; j <= k; ++j) {
         const int t = cavity_tri(cr, k, j, to);
-        atomicMin(&own[t], s);
-        for (int e = 0; e < 3; ++e) atomicMin(&own[to[3 * t + e] >> 4], s);
+        atomicMin(&own[t], key);
+        for (int e = 0; e < 3; ++e) atomicMin(&own[to[3 * t + e] >> 4], key);
     }
 }
 
@@ -804,22 +829,24 @@ __device__ int triangulate_chain(const Pt *P, const int *ch, int len, int *nt, i
 __global__ void k_dt_cavity(const Pt *__restrict__ P, int n, int *__restrict__ tv,
                             int *__restrict__ to, const int *__restrict__ loc,
                             const int *__restrict__ seg, const int2 *__restrict__ cross,
-                            const int *__restrict__ ncr, int nseg, const int *__restrict__ own,
+                            const int *__restrict__ ncr, int nseg, int round,
+                            const unsigned long long *__restrict__ own,
                             int *__restrict__ ws_all, int *__restrict__ cnt)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg || ncr[s] <= 0) return;
     int *err = cnt + C_ERR;
     const int k = ncr[s];
+    const unsigned long long key = claim_key(s, round);
     const int2 *cr = cross + (long)s * kCavMax;
     int *ws = ws_all + (long)s * kWs;
     int *C = ws + kWsC, *OV = ws + kWsOV, *OL = ws + kWsOL, *UL = ws + kWsUL, *UR = ws + kWsUR,
         *NT = ws + kWsNT, *ST = ws + kWsST;
     for (int j = 0; j <= k; ++j) {
         const int t = cavity_tri(cr, k, j, to);
-        if (own[t] != s) return;
+        if (own[t] != key) return;
         for (int e = 0; e < 3; ++e)
-            if (own[to[3 * t + e] >> 4] != s) return;
+            if (own[to[3 * t + e] >> 4] != key) return;
         C[j] = t;
     }
     const int a = kept_vertex(loc, seg[2 * s]), b = kept_vertex(loc, seg[2 * s + 1]);
@@ -937,7 +964,8 @@ struct DtCtx {
     // segments [2], crossings [kCavMax], crossing counts, workspace [kWs]
     char *cons = nullptr;
     int segCap = 0;
-    int *seg = nullptr, *ncr = nullptr, *ws = nullptr, *own = nullptr;
+    int *seg = nullptr, *ncr = nullptr, *ws = nullptr;
+    unsigned long long *own = nullptr;
     int2 *cross = nullptr;
     int *part = nullptr;                   // [3 kParts] first-triangle search
     double *partv = nullptr;               // [kParts]
@@ -1057,12 +1085,13 @@ static int insert_constraints(DtCtx *c, int n, int T, int nseg, int &round)
         hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_FLIPS, 0, 1L);
         hipLaunchKernelGGL(k_dt_walk, dim3(nb(nseg)), dim3(256), 0, st, c->P, n, T, c->tv, c->to,
                            c->vt, c->loc, c->seg, nseg, c->cross, c->ncr, c->cnt);
-        hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(T)), dim3(256), 0, st, c->own, 0x7fffffff,
-                           (long)T);
+        hipLaunchKernelGGL(k_dt_fill<unsigned long long>, dim3(nb(T)), dim3(256), 0, st, c->own,
+                           ~0ull, (long)T);
         hipLaunchKernelGGL(k_dt_claim, dim3(nb(nseg)), dim3(256), 0, st, c->to, c->cross, c->ncr,
-                           nseg, c->own);
-        hipLaunchKernelGGL(k_dt_cavity, dim3((nseg + 63) / 64), dim3(64), 0, st, c->P, n, c->tv, c->to,
-                           c->loc, c->seg, c->cross, c->ncr, nseg, c->own, c->ws, c->cnt);
+                           nseg, outer, c->own);
+        hipLaunchKernelGGL(k_dt_cavity, dim3((nseg + 63) / 64), dim3(64), 0, st, c->P, n, c->tv,
+                           c->to, c->loc, c->seg, c->cross, c->ncr, nseg, outer, c->own, c->ws,
+                           c->cnt);
         DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "counters");
         DCHK(hipStreamSynchronize(st), "constraints");
         ++c->stats[1];
