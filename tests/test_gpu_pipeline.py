@@ -5,10 +5,15 @@ compared with the oracle's chain on the same inputs:
   vertex weights (weighting.h:37-97) -> density raster (discretization.h:
   69-128) -> constraint mask (gcvt.h:143-159) and reference site
   initialisation (gcvt.h:76-122, host) -> gCVT (gcvt.cu:1087-1156) -> CDT
-  input points (delaunay.h:48-58) -> the site lift of recover()
-  (recover.h:63-125).
+  input points (delaunay.h:48-58) -> the constrained Delaunay triangulation
+  of the sites and the patch boundary (delaunay.h:60-79, GpuDel::compute at
+  main.cpp:236) -> the site lift and the triangle filter of recover()
+  (recover.h:63-141).
 
-Bar: every stage bit-exact; the next stage takes the GPU's output."""
+Bar: every stage before the triangulation bit-exact; the triangulation
+checked as a constrained Delaunay triangulation (every boundary edge
+present, hull tiled, empty circles); the next stage takes the GPU's
+output."""
 import numpy as np
 import pytest
 
@@ -51,3 +56,30 @@ def test_pipeline_mesh_to_lifted_sites(gpu, orc, n, num, m, seed):
     assert np.array_equal(face, f_r) and np.array_equal(wts, w_r) and np.array_equal(xyz, x_r)
     assert np.all(face >= 0)
     assert np.abs(xyz[:, 2] - 0.3 * np.sin(3 * xyz[:, 0]) * np.cos(2 * xyz[:, 1])).max() < 0.05
+    # the CDT input: the sites, then the boundary vertices as constraint
+    # points joined by the boundary edges (delaunayInput, delaunay.h:60-79)
+    from cvtamd.delaunay import delaunay
+    from test_gpu_delaunay import check_constrained
+    ring = ([i for i in range(m)] + [j * m + m - 1 for j in range(1, m)] +
+            [(m - 1) * m + i for i in range(m - 2, -1, -1)] + [j * m for j in range(m - 2, 0, -1)])
+    bpts = uv[ring]
+    # a site exactly on a boundary vertex would be the same point twice
+    dup = (sites[:, None, :] == bpts[None, :, :]).all(-1).any(1)
+    cs = sites[~dup]
+    cpts = np.concatenate([cs, bpts])
+    segs = np.stack([len(cs) + np.arange(len(ring)), len(cs) + (np.arange(len(ring)) + 1) % len(ring)],
+                    1).astype(np.int32)
+    ctri, copp = delaunay(cpts, segments=segs)
+    check_constrained(cpts, segs, ctri, copp)
+    # recover() keeps the triangles whose centroid lies in the mesh
+    # (recover.h:134-141).  The patch is convex, so every triangle of positive
+    # area keeps its centroid inside it; the zero-area ones along the
+    # collinear boundary have theirs on the boundary, where the barycentric
+    # test may round either way (recover() would drop those)
+    cen = cpts[ctri].mean(axis=1)
+    cface = recover.locate_points(uv, tri, cen)[0]
+    a, b_, c = cpts[ctri[:, 0]], cpts[ctri[:, 1]], cpts[ctri[:, 2]]
+    area2 = (b_[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b_[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+    inside = (cen[:, 0] > 0) & (cen[:, 0] < 1) & (cen[:, 1] > 0) & (cen[:, 1] < 1)
+    assert np.all(cface[inside & (area2 > 0)] >= 0)
+    assert (area2 > 0).sum() >= len(ctri) - 4 * (m - 1)
