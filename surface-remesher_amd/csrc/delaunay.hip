@@ -671,14 +671,19 @@ static constexpr int kWsC = 0, kWsOV = kCavMax + 1, kWsOL = 4 * (kCavMax + 1),
                      kWsNT = kWsUR + kCavMax + 2, kWsST = kWsNT + 3 * (kCavMax + 1),
                      kWs = kWsST + 2 * (kCavMax + 2);
 
-// vt[v] = a triangle incident to vertex v
-__global__ void k_dt_vertex_tri(const int *__restrict__ tv, int T, int n, int *__restrict__ vt)
+// vt[v] = a triangle incident to vertex v; deg[v] += its incident triangles
+// (deg zeroed before)
+__global__ void k_dt_vertex_tri(const int *__restrict__ tv, int T, int n, int *__restrict__ vt,
+                                int *__restrict__ deg)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     for (int j = 0; j < 3; ++j) {
         const int v = tv[3 * t + j];
-        if (v != n) vt[v] = t;
+        if (v != n) {
+            vt[v] = t;
+            atomicAdd(&deg[v], 1);
+        }
     }
 }
 
@@ -689,18 +694,26 @@ __device__ __forceinline__ int kept_vertex(const int *loc, int v)
 }
 
 // one thread per segment: marks it when present (ncr = 0), else lists the
-// edges it crosses from a to b (ncr of them; -1 beyond kCavMax)
+// edges it crosses from a to b (ncr of them; -1 beyond kCavMax).  a is the
+// end of lower degree (the turn around it is the shorter), kept in sa.
 __global__ void k_dt_walk(const Pt *__restrict__ P, int n, int T, const int *__restrict__ tv,
                           int *__restrict__ to, const int *__restrict__ vt,
-                          const int *__restrict__ loc, const int *__restrict__ seg, int nseg,
-                          int2 *__restrict__ cross, int *__restrict__ ncr, int *__restrict__ cnt)
+                          const int *__restrict__ deg, const int *__restrict__ loc,
+                          const int *__restrict__ seg, int nseg, int2 *__restrict__ cross,
+                          int *__restrict__ ncr, int *__restrict__ sa, int *__restrict__ cnt)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     bool missing = false;
     if (s < nseg) {
         int *err = cnt + C_ERR;
         int count = 0;
-        const int a = kept_vertex(loc, seg[2 * s]), b = kept_vertex(loc, seg[2 * s + 1]);
+        int a = kept_vertex(loc, seg[2 * s]), b = kept_vertex(loc, seg[2 * s + 1]);
+        if (deg[b] < deg[a]) {
+            const int x = a;
+            a = b;
+            b = x;
+        }
+        sa[s] = a;
         if (a != b) {
             int t = vt[a], ct = -1, ce = -1;
             // around a, counterclockwise, until the edge ab or the wedge holding b
@@ -831,7 +844,8 @@ __global__ void k_dt_cavity(const Pt *__restrict__ P, int n, int *__restrict__ t
                             const int *__restrict__ seg, const int2 *__restrict__ cross,
                             const int *__restrict__ ncr, int nseg, int round,
                             const unsigned long long *__restrict__ own,
-                            int *__restrict__ ws_all, int *__restrict__ cnt)
+                            const int *__restrict__ sa, int *__restrict__ ws_all,
+                            int *__restrict__ cnt)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg || ncr[s] <= 0) return;
@@ -849,7 +863,9 @@ __global__ void k_dt_cavity(const Pt *__restrict__ P, int n, int *__restrict__ t
             if (own[to[3 * t + e] >> 4] != key) return;
         C[j] = t;
     }
-    const int a = kept_vertex(loc, seg[2 * s]), b = kept_vertex(loc, seg[2 * s + 1]);
+    // the walk's direction: from sa[s] to the other end
+    const int e0 = kept_vertex(loc, seg[2 * s]), e1 = kept_vertex(loc, seg[2 * s + 1]);
+    const int a = sa[s], b = a == e0 ? e1 : e0;
     for (int j = 0; j <= k; ++j)
         for (int e = 0; e < 3; ++e) {
             OV[3 * j + e] = tv[3 * C[j] + e];
@@ -964,7 +980,7 @@ struct DtCtx {
     // segments [2], crossings [kCavMax], crossing counts, workspace [kWs]
     char *cons = nullptr;
     int segCap = 0;
-    int *seg = nullptr, *ncr = nullptr, *ws = nullptr;
+    int *seg = nullptr, *ncr = nullptr, *ws = nullptr, *sa = nullptr, *deg = nullptr;
     unsigned long long *own = nullptr;
     int2 *cross = nullptr;
     int *part = nullptr;                   // [3 kParts] first-triangle search
@@ -1080,18 +1096,20 @@ static int insert_constraints(DtCtx *c, int n, int T, int nseg, int &round)
     hipStream_t st = c->st;
     int *h = c->hcnt;
     for (int outer = 0;; ++outer) {
-        hipLaunchKernelGGL(k_dt_vertex_tri, dim3(nb(T)), dim3(256), 0, st, c->tv, T, n, c->vt);
+        hipLaunchKernelGGL(k_dt_fill<int>, dim3(nb(n)), dim3(256), 0, st, c->deg, 0, (long)n);
+        hipLaunchKernelGGL(k_dt_vertex_tri, dim3(nb(T)), dim3(256), 0, st, c->tv, T, n, c->vt,
+                           c->deg);
         hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_MISS, 0, 1L);
         hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_FLIPS, 0, 1L);
         hipLaunchKernelGGL(k_dt_walk, dim3(nb(nseg)), dim3(256), 0, st, c->P, n, T, c->tv, c->to,
-                           c->vt, c->loc, c->seg, nseg, c->cross, c->ncr, c->cnt);
+                           c->vt, c->deg, c->loc, c->seg, nseg, c->cross, c->ncr, c->sa, c->cnt);
         hipLaunchKernelGGL(k_dt_fill<unsigned long long>, dim3(nb(T)), dim3(256), 0, st, c->own,
                            ~0ull, (long)T);
         hipLaunchKernelGGL(k_dt_claim, dim3(nb(nseg)), dim3(256), 0, st, c->to, c->cross, c->ncr,
                            nseg, outer, c->own);
         hipLaunchKernelGGL(k_dt_cavity, dim3((nseg + 63) / 64), dim3(64), 0, st, c->P, n, c->tv,
-                           c->to, c->loc, c->seg, c->cross, c->ncr, nseg, outer, c->own, c->ws,
-                           c->cnt);
+                           c->to, c->loc, c->seg, c->cross, c->ncr, nseg, outer, c->own, c->sa,
+                           c->ws, c->cnt);
         DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "counters");
         DCHK(hipStreamSynchronize(st), "constraints");
         ++c->stats[1];
@@ -1138,7 +1156,8 @@ static int dt_run(cvt_dt *dt, const double *points, int num_points, const int *s
             };
             auto layout = [&](bool set) {
                 at = 0;
-                carve(c->seg, 2 * S, set); carve(c->ncr, S, set);
+                carve(c->seg, 2 * S, set); carve(c->ncr, S, set); carve(c->sa, S, set);
+                carve(c->deg, (size_t)c->maxN, set);
                 carve(c->cross, S * kCavMax, set); carve(c->ws, S * kWs, set);
                 carve(c->own, (size_t)c->Tmax, set);
             };
