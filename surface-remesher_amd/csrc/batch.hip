@@ -175,6 +175,8 @@ struct Batch {
     int *d_tile_patch = nullptr, *d_blk_patch = nullptr;
     float *rho = nullptr;
     uint32_t *seed = nullptr, *lab0 = nullptr, *lab1 = nullptr;
+    uint32_t *first = nullptr;     // the seed scatter's output, `none` between JFAs
+    int *d_site_patch = nullptr;   // patch of each site slot (-1: padding, sentinel)
     float2 *sites = nullptr;
     uint8_t *fixed = nullptr;
     Slot *slots = nullptr;
@@ -196,7 +198,7 @@ struct Batch {
         if (graph) (void)hipGraphDestroy(graph);
         void *b[] = {d_patch, d_units, d_tile_patch, d_blk_patch, rho, seed, lab0, lab1, sites,
                      fixed, slots, ovf, ovf_head, blockpart, hist, d_escale, hist_len, dstate,
-                     dflags};
+                     dflags, first, d_site_patch};
         for (void *p : b)
             if (p) (void)hipFree(p);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -206,6 +208,7 @@ struct Batch {
     int plan();
     int upload_patches();
     int enqueue_iteration(float omega);
+    void enqueue_jfa();
     unsigned pass_modes(int k) const;
 };
 
@@ -298,23 +301,39 @@ int Batch::upload_patches()
 }
 
 // step classes (bit 0: >= 4, bit 1: 1, bit 2: 2) that batched pass k runs
+// densely; bit 3: some patch starts its seed scatter at k
 unsigned Batch::pass_modes(int k) const
 {
     unsigned m = 0;
     for (const BatchPatch &P : hp) {
         const int local = k - (pmax - P.passes);
         if (local < 0) continue;
+        if (local == 0 && P.passes >= 3) {
+            m |= 8u;
+            continue;
+        }
         const int step = (local < P.passes - 1) ? (P.first_step >> local) : 1;
         m |= 1u << (step == 1 ? 1 : step == 2 ? 2 : 0);
     }
     return m;
 }
 
+void Batch::enqueue_jfa()
+{
+    for (int k = 0; k < pmax; ++k) {
+        const unsigned modes = pass_modes(k);
+        if (modes & 8u)
+            launch_jfa_batch_seeds(stream, d_patch, d_site_patch, total_sites, seed, first, sites, k,
+                                   pmax);
+        if (modes & 7u)
+            launch_jfa_batch(stream, d_patch, d_units, (int)units.size(), seed, first, lab0, lab1,
+                             sites, k, pmax, lds_bytes, modes & 7u);
+    }
+}
+
 int Batch::enqueue_iteration(float omega)
 {
-    for (int k = 0; k < pmax; ++k)
-        launch_jfa_batch(stream, d_patch, d_units, (int)units.size(), seed, lab0, lab1, sites, k,
-                         pmax, lds_bytes, pass_modes(k));
+    enqueue_jfa();
     hipLaunchKernelGGL(k_moments_batch, dim3(ntiles), dim3(256), 0, stream, d_patch, d_tile_patch,
                        lab0, lab1, rho, sites, slots, (long)total_sites, ovf, ovf_cap, ovf_head,
                        dflags, dstate);
@@ -388,6 +407,8 @@ int cvt_batch_create(cvt_batch **out, int num_patches, const int *widths, const 
               hipMalloc(&b->d_patch, sizeof(BatchPatch) * num_patches) == hipSuccess &&
               hipMalloc(&b->rho, sizeof(float) * pix) == hipSuccess &&
               hipMalloc(&b->seed, sizeof(uint32_t) * pix) == hipSuccess &&
+              hipMalloc(&b->first, sizeof(uint32_t) * pix) == hipSuccess &&
+              hipMalloc(&b->d_site_patch, sizeof(int) * soff) == hipSuccess &&
               hipMalloc(&b->lab0, sizeof(uint32_t) * pix) == hipSuccess &&
               hipMalloc(&b->lab1, sizeof(uint32_t) * pix) == hipSuccess &&
               hipMalloc(&b->sites, sizeof(float2) * soff) == hipSuccess &&
@@ -408,12 +429,18 @@ int cvt_batch_create(cvt_batch **out, int num_patches, const int *widths, const 
     hipStream_t s = b->stream;
     // seed map empty: every pixel holds its patch's `none`
     std::vector<float2> sent(soff, make_float2(0.0f, 0.0f));
+    std::vector<int> site_patch(soff, -1);
     for (int p = 0; p < num_patches; ++p) {
         const BatchPatch &P = b->hp[p];
         sent[P.site_off + P.cap] = make_float2(__builtin_inff(), __builtin_inff());
+        for (int i = 0; i < P.cap; ++i) site_patch[P.site_off + i] = p;
         hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, s, b->seed + P.pix_off,
                            8u * (uint32_t)(P.site_off + P.cap), (long)P.W * P.H);
+        hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, s, b->first + P.pix_off,
+                           8u * (uint32_t)(P.site_off + P.cap), (long)P.W * P.H);
     }
+    (void)hipMemcpyAsync(b->d_site_patch, site_patch.data(), sizeof(int) * soff,
+                         hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(b->sites, sent.data(), sizeof(float2) * soff, hipMemcpyHostToDevice, s);
     (void)hipMemsetAsync(b->rho, 0, sizeof(float) * pix, s);
     (void)hipMemsetAsync(b->fixed, 0, soff, s);
@@ -566,9 +593,7 @@ int cvt_batch_voronoi(cvt_batch *bh)
     Batch *b = reinterpret_cast<Batch *>(bh);
     int rc = batch_prepare(b);
     if (rc) return rc;
-    for (int k = 0; k < b->pmax; ++k)
-        launch_jfa_batch(b->stream, b->d_patch, b->d_units, (int)b->units.size(), b->seed, b->lab0,
-                         b->lab1, b->sites, k, b->pmax, b->lds_bytes, b->pass_modes(k));
+    b->enqueue_jfa();
     b->seeds_valid = false;
     b->labels_fresh = true;
     return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "voronoi");

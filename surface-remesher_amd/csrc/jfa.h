@@ -76,8 +76,18 @@ struct BatchUnit {
 // lds_bytes) is staged in LDS.  One launch per step class in `modes` (bit 0:
 // step >= 4, bit 1: step 1, bit 2: step 2); a unit runs in the launch of its
 // patch's class.
+// A patch of three passes or more starts with a scatter of its seeds into
+// `first` (launch_jfa_batch_seeds, at the batched pass where its own pass 0
+// falls; its units skip that pass), reads `first` in its pass 1, which
+// resets the seed map at its seed pixels, and resets `first` at the seeds'
+// receivers in pass 2: both maps hold their patch's `none` between JFAs.
 void launch_jfa_batch(hipStream_t st, const BatchPatch *patches, const BatchUnit *units, int nunits,
-                      uint32_t *seed, uint32_t *lab0, uint32_t *lab1, const float2 *sites, int pass,
-                      int pmax, size_t lds_bytes, unsigned modes);
+                      uint32_t *seed, uint32_t *first, uint32_t *lab0, uint32_t *lab1,
+                      const float2 *sites, int pass, int pmax, size_t lds_bytes, unsigned modes);
+// site_patch[i]: the patch owning site slot i of the concatenated table (-1
+// for padding and sentinels)
+void launch_jfa_batch_seeds(hipStream_t st, const BatchPatch *patches, const int *site_patch,
+                            int total_sites, const uint32_t *seed, uint32_t *first,
+                            const float2 *sites, int pass, int pmax);
 
 }  // namespace cvt

@@ -828,6 +828,7 @@ template <bool SITES_IN_LDS, int MODE>
 __global__ __launch_bounds__(1024) void k_jfa_batch(const BatchPatch *__restrict__ patches,
                                                     const BatchUnit *__restrict__ units,
                                                     uint32_t *__restrict__ seed,
+                                                    uint32_t *__restrict__ first,
                                                     uint32_t *__restrict__ lab0,
                                                     uint32_t *__restrict__ lab1,
                                                     const float2 *__restrict__ sites, int pass,
@@ -838,13 +839,35 @@ __global__ __launch_bounds__(1024) void k_jfa_batch(const BatchPatch *__restrict
     const BatchPatch P = patches[u.patch];
     const int local = pass - (pmax - P.passes);
     if (local < 0) return;                                   // uniform per block
+    const bool scatter = P.passes >= 3;
+    if (scatter && local == 0) return;                       // the seed scatter's pass
     const int step = (local < P.passes - 1) ? (P.first_step >> local) : 1;
     if ((step == 1 ? STEP_1 : step == 2 ? STEP_2 : STEP_ANY) != MODE) return;
     uint32_t *bufs[2] = {lab0 + P.pix_off, lab1 + P.pix_off};
-    const uint32_t *in = (local == 0) ? seed + P.pix_off : bufs[(local - 1) & 1];
+    const uint32_t *in = (local == 0)                 ? seed + P.pix_off
+                         : (local == 1 && scatter) ? first + P.pix_off
+                                                   : bufs[(local - 1) & 1];
     uint32_t *out = bufs[local & 1];
-    uint32_t *clear = (local == 1) ? seed + P.pix_off : nullptr;
+    uint32_t *clear = (local == 1 && !scatter) ? seed + P.pix_off : nullptr;
     const uint32_t none = 8u * (uint32_t)(P.site_off + P.cap);
+    if (scatter && (local == 1 || local == 2)) {
+        // this unit's share of the sparse reset: the seed pixels (pass 1) or
+        // the seeds' receivers at the first step (pass 2); neither map is
+        // read or written by this pass
+        uint32_t *map = (local == 1 ? seed : first) + P.pix_off;
+        const int R = local == 2 ? 9 : 1;
+        const int s0 = (int)((long)P.S * u.b / u.nb), s1 = (int)((long)P.S * (u.b + 1) / u.nb);
+        for (int t = threadIdx.x; t < R * (s1 - s0); t += blockDim.x) {
+            const int i = s0 + t / R, r = t - R * (t / R);
+            const float2 v = sites[P.site_off + i];
+            int x = seed_pixel(v.x, P.W), y = seed_pixel(v.y, P.H);
+            if (R == 9) {
+                x += (r % 3 - 1) * P.first_step;
+                y += (r / 3 - 1) * P.first_step;
+            }
+            if (x >= 0 && x < P.W && y >= 0 && y < P.H) map[(long)y * P.W + x] = none;
+        }
+    }
     const MapGeo m = map_geo(in, out, clear, P.W, P.H, step, none);
     // part b of the pass's runs (the planner gives a patch at most one part
     // per 16 runs, one per wave)
@@ -870,8 +893,9 @@ __global__ __launch_bounds__(1024) void k_jfa_batch(const BatchPatch *__restrict
 
 template <bool LDS, int MODE>
 static void launch_batch_mode(hipStream_t st, const BatchPatch *patches, const BatchUnit *units,
-                              int nunits, uint32_t *seed, uint32_t *lab0, uint32_t *lab1,
-                              const float2 *sites, int pass, int pmax, size_t lds_bytes)
+                              int nunits, uint32_t *seed, uint32_t *first, uint32_t *lab0,
+                              uint32_t *lab1, const float2 *sites, int pass, int pmax,
+                              size_t lds_bytes)
 {
     if (LDS) {
         static bool attr = false;
@@ -882,21 +906,21 @@ static void launch_batch_mode(hipStream_t st, const BatchPatch *patches, const B
         }
     }
     hipLaunchKernelGGL((k_jfa_batch<LDS, MODE>), dim3(nunits), dim3(1024), LDS ? lds_bytes : 0, st,
-                       patches, units, seed, lab0, lab1, sites, pass, pmax);
+                       patches, units, seed, first, lab0, lab1, sites, pass, pmax);
 }
 
 void launch_jfa_batch(hipStream_t st, const BatchPatch *patches, const BatchUnit *units, int nunits,
-                      uint32_t *seed, uint32_t *lab0, uint32_t *lab1, const float2 *sites, int pass,
-                      int pmax, size_t lds_bytes, unsigned modes)
+                      uint32_t *seed, uint32_t *first, uint32_t *lab0, uint32_t *lab1,
+                      const float2 *sites, int pass, int pmax, size_t lds_bytes, unsigned modes)
 {
 #define CVT_BATCH_MODE(M)                                                                      \
     if (modes & (1u << (M))) {                                                                 \
         if (lds_bytes)                                                                         \
-            launch_batch_mode<true, M>(st, patches, units, nunits, seed, lab0, lab1, sites, pass, \
-                                       pmax, lds_bytes);                                       \
+            launch_batch_mode<true, M>(st, patches, units, nunits, seed, first, lab0, lab1,    \
+                                       sites, pass, pmax, lds_bytes);                          \
         else                                                                                   \
-            launch_batch_mode<false, M>(st, patches, units, nunits, seed, lab0, lab1, sites,    \
-                                        pass, pmax, 0);                                        \
+            launch_batch_mode<false, M>(st, patches, units, nunits, seed, first, lab0, lab1,   \
+                                        sites, pass, pmax, 0);                                 \
     }
     CVT_BATCH_MODE(STEP_ANY)
     CVT_BATCH_MODE(STEP_1)
@@ -1081,6 +1105,52 @@ __global__ __launch_bounds__(256) void k_jfa_seeds(const uint32_t *__restrict__ 
         best = (k == 0) ? key : __builtin_fmin(best, key);
     }
     out[(long)py * W + px] = (uint32_t)__double2loint(best);
+}
+
+// pass 0 of the patches whose own pass 0 falls at batched pass `pass`: the
+// seed scatter of k_jfa_seeds, one thread per (site slot, receiver)
+__global__ __launch_bounds__(256) void k_jfa_batch_seeds(const BatchPatch *__restrict__ patches,
+                                                         const int *__restrict__ site_patch,
+                                                         int total, const uint32_t *__restrict__ seed,
+                                                         uint32_t *__restrict__ first,
+                                                         const float2 *__restrict__ sites,
+                                                         int pass, int pmax)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gi = t / 9, r = t - 9 * (t / 9);
+    if (gi >= total) return;
+    const int pid = site_patch[gi];
+    if (pid < 0) return;
+    const BatchPatch P = patches[pid];
+    if (pass - (pmax - P.passes) != 0 || P.passes < 3 || gi - P.site_off >= P.S) return;
+    const int W = P.W, H = P.H, step = P.first_step;
+    const float2 s = sites[gi];
+    const int qx = seed_pixel(s.x, W), qy = seed_pixel(s.y, H);
+    const int px = qx - (r % 3 - 1) * step, py = qy - (r / 3 - 1) * step;
+    if (px < 0 || px >= W || py < 0 || py >= H) return;
+    const uint32_t *sm = seed + P.pix_off;
+    const char *sb = reinterpret_cast<const char *>(sites);
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    double best = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int ox = (k % 3 - 1) * step, oy = (k / 3 - 1) * step;
+        const int tx = (px + ox >= 0 && px + ox < W) ? px + ox : px;
+        const int ty = (py + oy >= 0 && py + oy < H) ? py + oy : py;
+        const uint32_t l = sm[(long)ty * W + tx];
+        const double key = cand(*reinterpret_cast<const float2 *>(sb + l), l, fx, fy);
+        best = (k == 0) ? key : __builtin_fmin(best, key);
+    }
+    first[P.pix_off + (long)py * W + px] = (uint32_t)__double2loint(best);
+}
+
+void launch_jfa_batch_seeds(hipStream_t st, const BatchPatch *patches, const int *site_patch,
+                            int total_sites, const uint32_t *seed, uint32_t *first,
+                            const float2 *sites, int pass, int pmax)
+{
+    if (total_sites > 0)
+        hipLaunchKernelGGL(k_jfa_batch_seeds, dim3((9 * total_sites + 255) / 256), dim3(256), 0,
+                           st, patches, site_patch, total_sites, seed, first, sites, pass, pmax);
 }
 
 void launch_jfa_seed_pass(hipStream_t st, const uint32_t *seed, uint32_t *out,
