@@ -379,3 +379,25 @@ def test_constrained_errors(gpu):
         check_constrained(pts, [(4, 5)], tri, opp)
         tri, opp = d.compute(pts, segments=np.zeros((0, 2), np.int32))
         assert tri_set(tri) == tri_set(d.compute(pts)[0])
+
+
+@pytest.mark.parametrize("n,seed", [(2000, 31), (20000, 32)])
+def test_constrained_by_a_full_triangulation(gpu, n, seed):
+    """all edges of another triangulation of the same points (Qhull's Delaunay
+    triangulation of the points stretched 3x in y: an affine map keeps it
+    planar) as constraints: the constrained triangulation must be that
+    triangulation, every edge a constraint"""
+    from scipy.spatial import Delaunay as QDelaunay
+    from cvtamd.delaunay import Delaunay
+    g = np.random.default_rng(seed)
+    pts = (g.random((n, 2)) * 1000.0).astype(np.float32).astype(np.float64)
+    other = QDelaunay(pts * np.array([1.0, 3.0])).simplices
+    segs = np.array(sorted(edge_set(other)), np.int32)
+    with Delaunay(n) as d:
+        tri0, _ = d.compute(pts)
+        tri, opp = d.compute(pts, segments=segs)
+        print(f"\n{n} points, {len(segs)} segments, "
+              f"{len(edge_set(other) - edge_set(tri0))} not Delaunay edges", flush=True)
+    assert tri_set(tri) == tri_set(other)
+    assert ((opp < 0) | (opp & 4 != 0)).all()          # every interior edge constrained
+    check_structure(pts, tri, opp)
