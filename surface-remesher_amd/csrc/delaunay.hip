@@ -54,7 +54,7 @@ namespace cvt {
 using namespace dt;
 
 enum { C_UNINS = 0, C_DUP = 1, C_ERR = 2, C_FLIPS = 3, C_A = 4, C_B = 5, C_C = 6, C_FTOT = 7,
-       C_MISS = 8, C_NCROSS = 9, NUM_C = 12 };
+       C_MISS = 8, C_NCROSS = 9, C_WSLOT = 10, NUM_C = 12 };
 static constexpr int kParts = 256;              // workgroups of the first-triangle search
 static constexpr unsigned long long NO_BID = ~0ull;
 static constexpr int NO_FLIP = 0x7fffffff;
@@ -664,8 +664,10 @@ __global__ void k_dt_flip_fix(const Pt *__restrict__ P, int n, int T, int *__res
 // never flip them.
 // ---------------------------------------------------------------------------
 static constexpr int kCavMax = 128;            // edges one segment may cross
-// per-segment workspace (ints): cavity slots, their saved vertices and
-// links, the two chains, the new triangles and the recursion stack
+static constexpr int kWsSlots = 16384;         // cavities re-triangulated per round, at most
+// workspace of one cavity re-triangulation (ints, 7.2 KB): cavity slots,
+// their saved vertices and links, the two chains, the new triangles and the
+// recursion stack; kWsSlots of them, taken by the round's winners
 static constexpr int kWsC = 0, kWsOV = kCavMax + 1, kWsOL = 4 * (kCavMax + 1),
                      kWsUL = 7 * (kCavMax + 1), kWsUR = kWsUL + kCavMax + 2,
                      kWsNT = kWsUR + kCavMax + 2, kWsST = kWsNT + 3 * (kCavMax + 1),
@@ -844,7 +846,7 @@ __global__ void k_dt_cavity(const Pt *__restrict__ P, int n, int *__restrict__ t
                             const int *__restrict__ seg, const int2 *__restrict__ cross,
                             const int *__restrict__ ncr, int nseg, int round,
                             const unsigned long long *__restrict__ own,
-                            const int *__restrict__ sa, int *__restrict__ ws_all,
+                            const int *__restrict__ sa, int *__restrict__ ws_all, int nslots,
                             int *__restrict__ cnt)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -853,16 +855,19 @@ __global__ void k_dt_cavity(const Pt *__restrict__ P, int n, int *__restrict__ t
     const int k = ncr[s];
     const unsigned long long key = claim_key(s, round);
     const int2 *cr = cross + (long)s * kCavMax;
-    int *ws = ws_all + (long)s * kWs;
-    int *C = ws + kWsC, *OV = ws + kWsOV, *OL = ws + kWsOL, *UL = ws + kWsUL, *UR = ws + kWsUR,
-        *NT = ws + kWsNT, *ST = ws + kWsST;
     for (int j = 0; j <= k; ++j) {
         const int t = cavity_tri(cr, k, j, to);
         if (own[t] != key) return;
         for (int e = 0; e < 3; ++e)
             if (own[to[3 * t + e] >> 4] != key) return;
-        C[j] = t;
     }
+    // a workspace slot; a segment left without one waits for the next round
+    const int slot = atomicAdd(cnt + C_WSLOT, 1);
+    if (slot >= nslots) return;
+    int *ws = ws_all + (long)slot * kWs;
+    int *C = ws + kWsC, *OV = ws + kWsOV, *OL = ws + kWsOL, *UL = ws + kWsUL, *UR = ws + kWsUR,
+        *NT = ws + kWsNT, *ST = ws + kWsST;
+    for (int j = 0; j <= k; ++j) C[j] = cavity_tri(cr, k, j, to);
     // the walk's direction: from sa[s] to the other end
     const int e0 = kept_vertex(loc, seg[2 * s]), e1 = kept_vertex(loc, seg[2 * s + 1]);
     const int a = sa[s], b = a == e0 ? e1 : e0;
@@ -977,7 +982,8 @@ struct DtCtx {
     int *cnt = nullptr;                    // [NUM_C]
     int *vt = nullptr;                     // [maxN] a triangle per vertex (constraints)
     // constraint buffers, sized by the largest segment count seen:
-    // segments [2], crossings [kCavMax], crossing counts, workspace [kWs]
+    // per segment: ends [2], crossings [kCavMax], crossing count; then
+    // min(segments, kWsSlots) workspaces [kWs]
     char *cons = nullptr;
     int segCap = 0;
     int *seg = nullptr, *ncr = nullptr, *ws = nullptr, *sa = nullptr, *deg = nullptr;
@@ -1101,6 +1107,7 @@ static int insert_constraints(DtCtx *c, int n, int T, int nseg, int &round)
                            c->deg);
         hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_MISS, 0, 1L);
         hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_FLIPS, 0, 1L);
+        hipLaunchKernelGGL(k_dt_fill<int>, dim3(1), dim3(64), 0, st, c->cnt + C_WSLOT, 0, 1L);
         hipLaunchKernelGGL(k_dt_walk, dim3(nb(nseg)), dim3(256), 0, st, c->P, n, T, c->tv, c->to,
                            c->vt, c->deg, c->loc, c->seg, nseg, c->cross, c->ncr, c->sa, c->cnt);
         hipLaunchKernelGGL(k_dt_fill<unsigned long long>, dim3(nb(T)), dim3(256), 0, st, c->own,
@@ -1109,7 +1116,7 @@ static int insert_constraints(DtCtx *c, int n, int T, int nseg, int &round)
                            nseg, outer, c->own);
         hipLaunchKernelGGL(k_dt_cavity, dim3((nseg + 63) / 64), dim3(64), 0, st, c->P, n, c->tv,
                            c->to, c->loc, c->seg, c->cross, c->ncr, nseg, outer, c->own, c->sa,
-                           c->ws, c->cnt);
+                           c->ws, std::min(nseg, kWsSlots), c->cnt);
         DCHK(hipMemcpyAsync(h, c->cnt, sizeof(int) * NUM_C, hipMemcpyDeviceToHost, st), "counters");
         DCHK(hipStreamSynchronize(st), "constraints");
         ++c->stats[1];
@@ -1158,7 +1165,8 @@ static int dt_run(cvt_dt *dt, const double *points, int num_points, const int *s
                 at = 0;
                 carve(c->seg, 2 * S, set); carve(c->ncr, S, set); carve(c->sa, S, set);
                 carve(c->deg, (size_t)c->maxN, set);
-                carve(c->cross, S * kCavMax, set); carve(c->ws, S * kWs, set);
+                carve(c->cross, S * kCavMax, set);
+                carve(c->ws, std::min<size_t>(S, kWsSlots) * kWs, set);
                 carve(c->own, (size_t)c->Tmax, set);
             };
             layout(false);
