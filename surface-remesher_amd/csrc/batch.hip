@@ -44,9 +44,8 @@ static __global__ __launch_bounds__(256) void k_moments_batch(
     const int tile = blockIdx.x - P.tile_off;
     const int tX = tile % P.tiles_x, tY = tile / P.tiles_x;
     const uint32_t *lab = ((P.passes - 1) & 1 ? lab1 : lab0) + P.pix_off;
-    tile_lds_init(L);
     tile_moments<64>(L, lab, rho + P.pix_off, sites + P.site_off, P.site_off, P.S, P.W, P.H,
-                     P.rho_scale, tX * T, tY * T, T);
+                     P.rho_scale, tX * T, tY * T, T, nullptr);
     tile_flush(L, sites, P.site_off, T, tX, tY, P.tiles_x, P.tiles_y, dstate[ST_EPOCH], slots, nrows, ovf,
                ovf_cap, ovf_head, dstate, dflags);
 }
@@ -69,7 +68,8 @@ static __global__ __launch_bounds__(256) void k_update_batch(
         float2 s = sites[gi];
         const int pinned = fixed[gi];
         double w, dx, dy, ee;
-        cvt_site_totals(gi, dstate[ST_EPOCH], slots, nrows, ovf, ovf_head, w, dx, dy, ee);
+        if (cvt_site_totals(gi, dstate[ST_EPOCH], slots, nrows, ovf, ovf_head, w, dx, dy, ee) >= 0)
+            ovf_head[gi] = -1;
         e = ee;
         g = dx * dx + dy * dy;
         if ((mode & UPD_MOVE) && w > 0.0 && pinned == 0) {

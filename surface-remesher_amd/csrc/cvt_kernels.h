@@ -198,15 +198,23 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long m)
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// Adds the runs of the tile [tx0, tx0 + T) x [ty0, ty0 + T) to the hash.
-// lab: labels (byte offsets); a label's site index is (lab >> 3) - key_base,
-// valid when below S; sites: the S sites of this map.
+// Adds the runs of the tile [tx0, tx0 + T) x [ty0, ty0 + T) to the hash
+// (which it clears first).  lab: labels (byte offsets); a label's site index
+// is (lab >> 3) - key_base, valid when below S; sites: the S sites of this
+// map.  A wave's first label and density loads are issued before the hash is
+// cleared.  stop (may be null): the stop word of cvt_lloyd_until, read with
+// those loads; when it is set the routine returns true before adding
+// anything.
+struct MomLoad {
+    uint32_t lv[4], rv[4];
+};
+
 template <int SEG, int HC>
-__device__ __forceinline__ void tile_moments(TileLds<HC> &L, const uint32_t *__restrict__ lab,
+__device__ __forceinline__ bool tile_moments(TileLds<HC> &L, const uint32_t *__restrict__ lab,
                                              const float *__restrict__ rho,
                                              const float2 *__restrict__ sites, int key_base,
                                              int S, int W, int H, float rho_scale, int tx0,
-                                             int ty0, int T)
+                                             int ty0, int T, const int *__restrict__ stop)
 {
     constexpr int SL = SEG / 4;                    // lanes per segment
     constexpr int SPW = 64 / SL;                   // segments per wave step
@@ -220,38 +228,58 @@ __device__ __forceinline__ void tile_moments(TileLds<HC> &L, const uint32_t *__r
     const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float *>(rho), (short)0, (int)map_bytes, 0x00020000);
     TailRec *q = L.q[wave];
-
-    for (int g = wave; g < steps; g += 4) {
+    // the pixels of step g: row y, columns x0 .. x0 + 3 of this lane
+    auto step_pos = [&](int g, int &x0, int &y) {
         const int seg = g * SPW + lane / SL;
         const int r = seg / spr;
-        const int x0 = tx0 + (seg - r * spr) * SEG + 4 * seg_lane;
-        const int y = ty0 + r;
+        x0 = tx0 + (seg - r * spr) * SEG + 4 * seg_lane;
+        y = ty0 + r;
+    };
+    auto step_load = [&](int g, MomLoad &ld) {
+        int x0, y;
+        step_pos(g, x0, y);
+        const uint32_t off = 4u * ((uint32_t)min(y, H - 1) * (uint32_t)W + (uint32_t)x0);
+        if ((W & 3) == 0) {
+            typedef unsigned u4 __attribute__((ext_vector_type(4)));
+            const u4 a4 = __builtin_amdgcn_raw_buffer_load_b128(rl, off, 0, 0);
+            const u4 b4 = __builtin_amdgcn_raw_buffer_load_b128(rr, off, 0, 0);
+            ld.lv[0] = a4.x; ld.lv[1] = a4.y; ld.lv[2] = a4.z; ld.lv[3] = a4.w;
+            ld.rv[0] = b4.x; ld.rv[1] = b4.y; ld.rv[2] = b4.z; ld.rv[3] = b4.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                ld.lv[j] = __builtin_amdgcn_raw_buffer_load_b32(rl, off + 4u * j, 0, 0);
+                ld.rv[j] = __builtin_amdgcn_raw_buffer_load_b32(rr, off + 4u * j, 0, 0);
+            }
+        }
+    };
+
+    // a vector load, so it returns in order with the step loads (a null stop
+    // is an empty buffer range, which reads 0)
+    const int halt = (int)__builtin_amdgcn_raw_buffer_load_b32(
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(stop), (short)0, stop ? 4 : 0,
+                                          0x00020000),
+        0, 0, 0);
+    MomLoad ld;
+    if (wave < steps) step_load(wave, ld);
+    tile_lds_init(L);
+    if (halt) return true;
+
+    for (int g = wave; g < steps; g += 4) {
+        if (g > wave) step_load(g, ld);
+        int x0, y;
+        step_pos(g, x0, y);
         int key[4];
         float rs[4];
         {
             const bool rowok = y < H;
-            const uint32_t off = 4u * ((uint32_t)min(y, H - 1) * (uint32_t)W + (uint32_t)x0);
-            uint32_t lv[4], rv[4];
-            if ((W & 3) == 0) {
-                typedef unsigned u4 __attribute__((ext_vector_type(4)));
-                const u4 a4 = __builtin_amdgcn_raw_buffer_load_b128(rl, off, 0, 0);
-                const u4 b4 = __builtin_amdgcn_raw_buffer_load_b128(rr, off, 0, 0);
-                lv[0] = a4.x; lv[1] = a4.y; lv[2] = a4.z; lv[3] = a4.w;
-                rv[0] = b4.x; rv[1] = b4.y; rv[2] = b4.z; rv[3] = b4.w;
-            } else {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    lv[j] = __builtin_amdgcn_raw_buffer_load_b32(rl, off + 4u * j, 0, 0);
-                    rv[j] = __builtin_amdgcn_raw_buffer_load_b32(rr, off + 4u * j, 0, 0);
-                }
-            }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int l = (int)(lv[j] >> 3) - key_base;
-                const float d = __uint_as_float(rv[j]);
-                const bool ok = rowok && (x0 + j < W) && (unsigned)l < (unsigned)S && d != 0.0f;
+                const int l = (int)(ld.lv[j] >> 3) - key_base;
+                const float dv = __uint_as_float(ld.rv[j]);
+                const bool ok = rowok && (x0 + j < W) && (unsigned)l < (unsigned)S && dv != 0.0f;
                 key[j] = ok ? l : -1;
-                rs[j] = ok ? d * rho_scale : 0.0f;
+                rs[j] = ok ? dv * rho_scale : 0.0f;
             }
         }
         // run heads and run starts (positions 4 * lane + j within the wave)
@@ -373,6 +401,7 @@ __device__ __forceinline__ void tile_moments(TileLds<HC> &L, const uint32_t *__r
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    return false;
 }
 
 // Writes the tile's hash entries: a site whose home tile (the tile holding
@@ -430,11 +459,11 @@ static __global__ __launch_bounds__(256) void k_moments(
     int *__restrict__ ovf_head, int *__restrict__ dflags, int *__restrict__ dstate)
 {
     __shared__ TileLds<TileHash<T>::HC> L;
-    if (dstate[ST_STOP]) return;   // a stop rule held (cvt_lloyd_until)
     const int tile = blockIdx.x;
     const int tX = tile % tilesX, tY = tile / tilesX;
-    tile_lds_init(L);
-    tile_moments<(T < 64 ? T : 64)>(L, lab, rho, sites, 0, S, W, H, rho_scale, tX * T, tY * T, T);
+    if (tile_moments<(T < 64 ? T : 64)>(L, lab, rho, sites, 0, S, W, H, rho_scale, tX * T, tY * T,
+                                        T, dstate + ST_STOP))
+        return;   // a stop rule held (cvt_lloyd_until)
     tile_flush(L, sites, 0, T, tX, tY, tilesX, tilesY, dstate[ST_EPOCH], slots, nrows, ovf, ovf_cap,
                ovf_head, dstate, dflags);
 }
@@ -459,12 +488,12 @@ enum {
 };
 
 // Moment totals of site i: the 9 window slots of this epoch plus its overflow
-// chain, summed exactly in 128 bits.  Consumes the chain.
-__device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__restrict__ slots,
-                                                long nrows,
-                                                const OvfRec *__restrict__ ovf,
-                                                int *__restrict__ ovf_head, double &w, double &dx,
-                                                double &dy, double &ee)
+// chain, summed exactly in 128 bits.  Returns the chain's head (-1: none); the
+// caller consumes the chain (ovf_head[i] = -1) once it may write.
+__device__ __forceinline__ int cvt_site_totals(int i, int epoch, const Slot *__restrict__ slots,
+                                               long nrows, const OvfRec *__restrict__ ovf,
+                                               const int *__restrict__ ovf_head, double &w,
+                                               double &dx, double &dy, double &ee)
 {
     __int128 sw = 0, sx = 0, sy = 0, se = 0;
     // the chain head and all nine slots are loaded before any is used, so
@@ -485,11 +514,11 @@ __device__ __forceinline__ void cvt_site_totals(int i, int epoch, const Slot *__
     for (int k = head; k >= 0; k = ovf[k].next) {
         sw += ovf[k].w; sx += ovf[k].dx; sy += ovf[k].dy; se += ovf[k].e;
     }
-    ovf_head[i] = -1;
     w = cvt_i128_to_double(sw, MOM_FW);
     dx = cvt_i128_to_double(sx, MOM_FX);
     dy = cvt_i128_to_double(sy, MOM_FX);
     ee = cvt_i128_to_double(se, MOM_FE);
+    return head;
 }
 
 
@@ -518,18 +547,28 @@ static __global__ __launch_bounds__(256) void k_update(
 {
     __shared__ double red_e[256], red_g[256];
     __shared__ int last;
-    if (dstate[ST_STOP]) return;   // a stop rule held (cvt_lloyd_until)
+    // the stop word of cvt_lloyd_until is loaded with the other state words
+    // and tested once this thread's loads are issued: a stopped update
+    // returns before its first write
+    const int halt = dstate[ST_STOP];
+    const int epoch = dstate[ST_EPOCH];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int live = (mode & UPD_DEV_S) ? dstate[ST_NSITES] : S;
     const float omega = domega ? *domega : omega_arg;
     double e = 0.0, g = 0.0;
+    // the site and its fixed flag (the context's flags, never null) are
+    // loaded with the moment slots; nothing waits for them until used
+    float2 s = make_float2(0.0f, 0.0f);
+    int pinned = 0, head = -1;
+    double w = 0.0, dx = 0.0, dy = 0.0, ee = 0.0;
     if (i < live) {
-        // the site and its fixed flag (the context's flags, never null) are
-        // loaded with the moment slots; nothing waits for them until used
-        float2 s = sites[i];
-        const int pinned = fixed[i];
-        double w, dx, dy, ee;
-        cvt_site_totals(i, dstate[ST_EPOCH], slots, nrows, ovf, ovf_head, w, dx, dy, ee);
+        s = sites[i];
+        pinned = fixed[i];
+        head = cvt_site_totals(i, epoch, slots, nrows, ovf, ovf_head, w, dx, dy, ee);
+    }
+    if (halt) return;   // uniform over the grid
+    if (i < live) {
+        if (head >= 0) ovf_head[i] = -1;
         e = ee;
         g = dx * dx + dy * dy;
         if (mom_out) {

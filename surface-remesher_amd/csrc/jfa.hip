@@ -443,11 +443,19 @@ __device__ __forceinline__ void eval_item(const Item<MODE> &it, const MapGeo &m,
     }
 }
 
+// Work a pass does once its first label loads and the table staging are
+// issued: PRO() returns true when the pass must stop there.  NoPrologue is
+// the batched pass's; SparseReset is the single-map pass's (below).
+struct NoPrologue {
+    __device__ bool operator()() const { return false; }
+};
+
 // Rows [y0, y1) of one map by all waves of the block.  Items (row, segment)
 // are dealt to the waves round robin; a wave's first load is issued before
 // the site table is staged, so the two latencies overlap.
-template <int MODE, int TAB>
-__device__ __forceinline__ void quad_rows(const MapGeo &m, const SiteTab<TAB> &tab, int y0, int y1)
+template <int MODE, int TAB, class PRO>
+__device__ __forceinline__ void quad_rows(const MapGeo &m, const SiteTab<TAB> &tab, int y0, int y1,
+                                          const PRO &pro)
 {
     const int nseg = (m.W + 255) >> 8;
     const int nwaves = blockDim.x >> 6;
@@ -458,6 +466,7 @@ __device__ __forceinline__ void quad_rows(const MapGeo &m, const SiteTab<TAB> &t
     Item<MODE> A, B;
     load_item<MODE>(A, m, min(row, last), seg, lane4);
     tab.stage();
+    if (pro()) return;
     while (row < y1) {
         int r2 = row, g2 = seg + nwaves;
         while (g2 >= nseg) { g2 -= nseg; ++r2; }
@@ -645,10 +654,10 @@ __device__ __forceinline__ double key_max() { return __hiloint2double(0x7f800000
 // first two.  Rows past the grid end their chain: their outputs are not
 // stored and their loads not issued.  With `stage` the site table is staged
 // after the wave's first loads are issued, so the two latencies overlap.
-template <int MODE, int TAB>
+template <int MODE, int TAB, class PRO = NoPrologue>
 __device__ __forceinline__ void wave_runs(const MapGeo &m, const SiteTab<TAB> &tab,
                                           const RunGeo &g, int u, int u_end, int stride,
-                                          bool stage)
+                                          bool stage, const PRO &pro = PRO())
 {
     const int lane4 = 4 * (threadIdx.x & 63);
     const int k = m.step, H = m.H, W = m.W;
@@ -660,6 +669,7 @@ __device__ __forceinline__ void wave_runs(const MapGeo &m, const SiteTab<TAB> &t
         if (y < H) load_row<MODE>(R1, m, y, seg, lane4);
     }
     if (stage) tab.stage();
+    if (pro()) return;
     while (u < u_end) {
         const int un = u + stride;
         int yn = 0, sn = 0;
@@ -723,12 +733,13 @@ __device__ __forceinline__ void wave_runs(const MapGeo &m, const SiteTab<TAB> &t
 }
 
 // Runs [u0, u1) of one map by all waves of the block, dealt round robin.
-template <int MODE, int TAB>
-__device__ __forceinline__ void quad_runs(const MapGeo &m, const SiteTab<TAB> &tab, int u0, int u1)
+template <int MODE, int TAB, class PRO>
+__device__ __forceinline__ void quad_runs(const MapGeo &m, const SiteTab<TAB> &tab, int u0, int u1,
+                                          const PRO &pro)
 {
     const int nwaves = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    wave_runs<MODE, TAB>(m, tab, run_geo(m), u0 + wave, u1, nwaves, true);
+    wave_runs<MODE, TAB>(m, tab, run_geo(m), u0 + wave, u1, nwaves, true, pro);
 }
 
 // Single map, site table staged at LDS byte 0 in a static array of ENTRIES
@@ -739,16 +750,17 @@ __device__ __forceinline__ void quad_runs(const MapGeo &m, const SiteTab<TAB> &t
 #define JFA_THREADS(E) ((E) > 5120 || (E) < 0 ? 1024 : 256)
 #define JFA_MINB(E) ((E) > 5120 || (E) < 0 ? 1 : 4)
 
-template <int MODE, int TAB>
-__device__ __forceinline__ void quad_layout(const MapGeo &m, const SiteTab<TAB> &tab, bool runs)
+template <int MODE, int TAB, class PRO>
+__device__ __forceinline__ void quad_layout(const MapGeo &m, const SiteTab<TAB> &tab, bool runs,
+                                            const PRO &pro)
 {
     int a, b;
     if (runs) {
         band_of(blockIdx.x, gridDim.x, run_geo(m).count, a, b);
-        quad_runs<MODE, TAB>(m, tab, a, b);
+        quad_runs<MODE, TAB>(m, tab, a, b, pro);
     } else {
         band_of(blockIdx.x, gridDim.x, m.H, a, b);
-        quad_rows<MODE, TAB>(m, tab, a, b);
+        quad_rows<MODE, TAB>(m, tab, a, b, pro);
     }
 }
 
@@ -760,33 +772,67 @@ __device__ __forceinline__ int seed_pixel(float v, int n)
     return (int)f;
 }
 
-// nclear > 0: clear is a map whose only entries sit at the seed pixels q of
-// sites [0, nclear) (rstep == 0), or at q + rstep * {-1, 0, 1}^2 (rstep > 0,
-// the receivers of a seed pass); they are reset one store per pixel of that
-// set instead of W * H stores (the pass does not read that map).
+// The single-map pass's prologue.  halt: the stop word of cvt_lloyd_until,
+// loaded at kernel entry and tested here, so its latency overlaps the first
+// loads (a stopped pass writes nothing).  nclear > 0: clear is a map whose
+// only entries sit at the seed pixels q of sites [0, nclear) (rstep == 0),
+// or at q + rstep * {-1, 0, 1}^2 (rstep > 0, the receivers of a seed pass);
+// they are reset one store per pixel of that set instead of W * H stores
+// (the pass does not read that map).
+// The stop word (0 without one), as a vector load: it returns in order with
+// the pass's first label loads, so testing it after them costs no wait of its
+// own (a scalar load would be waited for with the kernel arguments).  A null
+// stop gives an empty buffer range, whose load returns 0.
+__device__ __forceinline__ int stop_word(const int *stop)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(stop, stop ? 4u : 0u), 0, 0, 0);
+}
+
+struct SparseReset {
+    int halt;
+    uint32_t *clear;
+    const float2 *sites;
+    int W, H, nclear, rstep;
+    uint32_t none;
+    __device__ bool operator()() const
+    {
+        if (halt) return true;
+        if (clear && nclear > 0) {
+            const __amdgpu_buffer_rsrc_t rc = buf_rsrc(clear, 4u * (uint32_t)W * (uint32_t)H);
+            const int R = rstep > 0 ? 9 : 1;
+            for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < R * nclear;
+                 t += gridDim.x * blockDim.x) {
+                const int i = t / R, r = t - R * (t / R);
+                const float2 v = sites[i];
+                int x = seed_pixel(v.x, W), y = seed_pixel(v.y, H);
+                if (R == 9) {
+                    x += (r % 3 - 1) * rstep;
+                    y += (r / 3 - 1) * rstep;
+                }
+                if (x >= 0 && x < W && y >= 0 && y < H)
+                    __builtin_amdgcn_raw_buffer_store_b32(none, rc, 4u * (uint32_t)(y * W + x), 0,
+                                                          0);
+            }
+        }
+        return false;
+    }
+};
+
 template <int MODE, int ENTRIES, bool RUNS>
 __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa_quad(
     const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
     const float2 *__restrict__ sites, int table_f4, int W, int H, int step,
     uint32_t *__restrict__ clear, uint32_t none, int nclear, int rstep, const int *__restrict__ stop)
 {
-    if (stop && *stop) return;
-    if (clear && nclear > 0) {
-        const __amdgpu_buffer_rsrc_t rc = buf_rsrc(clear, 4u * (uint32_t)W * (uint32_t)H);
-        const int R = rstep > 0 ? 9 : 1;
-        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < R * nclear;
-             t += gridDim.x * blockDim.x) {
-            const int i = t / R, r = t - R * (t / R);
-            const float2 v = sites[i];
-            int x = seed_pixel(v.x, W), y = seed_pixel(v.y, H);
-            if (R == 9) {
-                x += (r % 3 - 1) * rstep;
-                y += (r / 3 - 1) * rstep;
-            }
-            if (x >= 0 && x < W && y >= 0 && y < H)
-                __builtin_amdgcn_raw_buffer_store_b32(none, rc, 4u * (uint32_t)(y * W + x), 0, 0);
-        }
-    }
+    SparseReset pro;
+    pro.halt = stop_word(stop);
+    pro.clear = clear;
+    pro.sites = sites;
+    pro.W = W;
+    pro.H = H;
+    pro.nclear = nclear;
+    pro.rstep = rstep;
+    pro.none = none;
     const MapGeo m = map_geo(in, out, nclear > 0 ? nullptr : clear, W, H, step, none);
     if constexpr (ENTRIES > 0) {
         __shared__ float4 lds_tab[ENTRIES / 2];
@@ -795,7 +841,7 @@ __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa
         tab.src = reinterpret_cast<const f32x4 *>(sites);
         tab.n4 = table_f4;
         tab.base = 0;
-        quad_layout<MODE, TAB_LDS0>(m, tab, RUNS);
+        quad_layout<MODE, TAB_LDS0>(m, tab, RUNS, pro);
     } else if constexpr (ENTRIES < 0) {
         __shared__ float4 lds_tab[-ENTRIES / 2];
         // entries 0 .. -ENTRIES - 3 staged, then a zero pair; the rest of the
@@ -808,13 +854,13 @@ __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa
         tab.g = buf_rsrc(sites + kept, 8u * (uint32_t)(2 * table_f4 - kept));
         tab.split = 8u * (uint32_t)kept;
         tab.base = 0;
-        quad_layout<MODE, TAB_SPLIT>(m, tab, RUNS);
+        quad_layout<MODE, TAB_SPLIT>(m, tab, RUNS, pro);
     } else {
         SiteTab<TAB_GLOBAL> tab;
         tab.g = buf_rsrc(sites, 16u * (uint32_t)table_f4);
         tab.base = 0;
         tab.n4 = 0;
-        quad_layout<MODE, TAB_GLOBAL>(m, tab, RUNS);
+        quad_layout<MODE, TAB_GLOBAL>(m, tab, RUNS, pro);
     }
 }
 
@@ -1084,11 +1130,12 @@ __global__ __launch_bounds__(256) void k_jfa_seeds(const uint32_t *__restrict__ 
                                                    const float2 *__restrict__ sites, int S, int W,
                                                    int H, int step, const int *__restrict__ stop)
 {
-    if (stop && *stop) return;
+    const int halt = stop_word(stop);   // tested once the site's load is in flight
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / 9, r = t - 9 * (t / 9);
     if (i >= S) return;
     const float2 s = sites[i];
+    if (halt) return;
     const int qx = seed_pixel(s.x, W), qy = seed_pixel(s.y, H);
     const int px = qx - (r % 3 - 1) * step, py = qy - (r / 3 - 1) * step;
     if (px < 0 || px >= W || py < 0 || py >= H) return;
