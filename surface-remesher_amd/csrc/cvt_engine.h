@@ -73,8 +73,13 @@ struct Ctx {
     void clear_seed_map();         // every pixel back to none()
     int enqueue_jfa(hipEvent_t after_first = nullptr);   // event recorded after pass 0
     int enqueue_moments();
+    // defer: the update leaves its finalize (energy sums, history, stop rule,
+    // epoch) to the next seed pass or enqueue_finalize
     int enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg,
-                       const float *domega = nullptr, CompatArgs ca = CompatArgs{});
+                       const float *domega = nullptr, CompatArgs ca = CompatArgs{},
+                       int defer = 0);
+    FinalArgs final_args(int record, double *out_eg) const;
+    void enqueue_finalize();       // a deferred finalize, if one is pending
     int enqueue_iteration(float omega);
     int build_graph(float omega);
     int lloyd(int iterations, float omega);

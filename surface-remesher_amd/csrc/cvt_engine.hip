@@ -164,9 +164,16 @@ int Ctx::enqueue_jfa(hipEvent_t after_first)
     // sites do not move during a JFA; compaction only shrinks the count).
     const bool scatter = np >= 3;
     const int step0 = step;
+    // A deferred finalize of the previous iteration (the Lloyd iteration's
+    // site update leaves it pending) runs as one more workgroup of the seed
+    // scatter, or as its own launch when there is no scatter; either is a
+    // no-op when nothing is pending.
+    const FinalArgs fa = final_args(1, nullptr);
+    if (!scatter) enqueue_finalize();
     for (int k = 0; k < np; ++k) {
         if (k == 0 && scatter) {
-            launch_jfa_seed_pass(stream, seed, first, sites, S, W, H, step, dstate + ST_STOP);
+            launch_jfa_seed_pass(stream, seed, first, sites, S, W, H, step, dstate + ST_STOP,
+                                 blockpart, dstate, &fa);
             if (after_first) hipEventRecord(after_first, stream);
             in = first;
             if (step > 1) step >>= 1;
@@ -186,10 +193,8 @@ int Ctx::enqueue_jfa(hipEvent_t after_first)
     return hipGetLastError() == hipSuccess ? CVT_OK : fail(CVT_ERR_HIP, "jfa launch failed");
 }
 
-int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg,
-                        const float *domega, CompatArgs ca)
+FinalArgs Ctx::final_args(int record, double *out_eg) const
 {
-    const int nb = std::max(1, (S + 255) / 256);
     const double a = (double)W * H;
     FinalArgs fa;
     fa.hist = hist;
@@ -199,6 +204,22 @@ int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, doub
     fa.g_scale = 2.0 * ldexp(1.0, rho_exp) / a;
     fa.out_eg = out_eg;
     fa.until = until;
+    fa.defer = 0;
+    return fa;
+}
+
+void Ctx::enqueue_finalize()
+{
+    hipLaunchKernelGGL(k_finalize_pending, dim3(1), dim3(256), 0, stream, blockpart, dstate,
+                       final_args(1, nullptr));
+}
+
+int Ctx::enqueue_update(int mode, float omega, double *mom_out, int record, double *out_eg,
+                        const float *domega, CompatArgs ca, int defer)
+{
+    const int nb = std::max(1, (S + 255) / 256);
+    FinalArgs fa = final_args(record, out_eg);
+    fa.defer = defer;
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, stream, sites, fixed, S, slots, (long)capS, ovf,
                        ovf_head, dflags, dstate, omega, domega, mode, ldexp(1.0, rho_exp), mom_out,
                        blockpart, seed, W, H, ca, fa);
@@ -211,7 +232,8 @@ int Ctx::enqueue_iteration(float omega)
     int rc;
     if ((rc = enqueue_jfa())) return rc;
     if ((rc = enqueue_moments())) return rc;
-    return enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr);
+    return enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr, nullptr, CompatArgs{},
+                          1);
 }
 
 int Ctx::build_graph(float omega)
@@ -278,6 +300,10 @@ int Ctx::lloyd_until(int max_iter, float omega, double rel_tol, int window, doub
     }
     hipLaunchKernelGGL(k_until, dim3(1), dim3(64), 0, stream, dstate, until, 0, rel_tol, target,
                        window, ran);
+    // A stop raised by the finalize inside a seed scatter can leave part of
+    // that scatter's receivers in `first`, which holds none between JFAs:
+    // clear it.
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, stream, first, none(), (long)W * H);
     int n = 0;
     if (hipMemcpyAsync(&n, ran, sizeof n, hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
@@ -302,6 +328,8 @@ int Ctx::lloyd(int iterations, float omega)
         for (int k = 0; k < iterations; ++k)
             if ((rc = enqueue_iteration(omega))) return rc;
     }
+    // the last iteration's finalize
+    enqueue_finalize();
     labels_fresh = true;
     return CVT_OK;
 }
@@ -686,9 +714,12 @@ int cvt_profile(cvt_ctx *ctx, int iterations, float omega, double *ms)
         hipEventRecord(ev[NE * it + 2], c->stream);
         if (!rc) rc = c->enqueue_moments();
         hipEventRecord(ev[NE * it + 3], c->stream);
-        if (!rc) rc = c->enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr);
+        if (!rc)
+            rc = c->enqueue_update(UPD_MOVE | UPD_SEED, omega, nullptr, 1, nullptr, nullptr,
+                                   CompatArgs{}, 1);
         hipEventRecord(ev[NE * it + 4], c->stream);
     }
+    c->enqueue_finalize();
     double acc[5] = {0, 0, 0, 0, 0};
     if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(CVT_ERR_HIP, "profile");
     for (int it = 0; it < iterations && !rc; ++it) {

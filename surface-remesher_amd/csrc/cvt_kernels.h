@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cvt_finalize.h"
 #include "cvt_types.h"
 
 // Each translation unit that includes this header launches only some of its
@@ -523,20 +524,6 @@ __device__ __forceinline__ int cvt_site_totals(int i, int epoch, const Slot *__r
 
 
 
-// The fixed pairwise tree v[t] += v[t + o], o = 32, 16, ..., 1, over the 64
-// lanes of a wave (lane 0 ends with the sum).  Shuffles instead of LDS
-// steps: the same additions in the same order, so the same bits as the LDS
-// tree it replaces.
-__device__ __forceinline__ double wave_tree_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double u = __shfl_down(v, o, 64);
-        if ((threadIdx.x & 63) < (unsigned)o) v += u;
-    }
-    return v;
-}
-
 static __global__ __launch_bounds__(256) void k_update(
     float2 *__restrict__ sites, const uint8_t *__restrict__ fixed, int S,
     const Slot *__restrict__ slots, long nrows, const OvfRec *__restrict__ ovf,
@@ -624,6 +611,15 @@ static __global__ __launch_bounds__(256) void k_update(
     }
     // every chain was consumed above: the record pool restarts empty
     if (blockIdx.x == 0 && threadIdx.x == 0) dstate[ST_OVFN] = 0;
+    if (fa.defer) {
+        // the partials wait for the next seed pass (or a finalize launch)
+        if (threadIdx.x == 0) {
+            blockpart[2 * blockIdx.x] = be;
+            blockpart[2 * blockIdx.x + 1] = bg;
+            if (blockIdx.x == 0) dstate[ST_PEND] = (int)gridDim.x;
+        }
+        return;
+    }
     // publish this block's partials; the last block to finish adds all of
     // them in block order (the release / acquire pair at agent scope makes
     // every block's partials visible to it, whichever XCD wrote them)
@@ -637,48 +633,17 @@ static __global__ __launch_bounds__(256) void k_update(
     __syncthreads();
     if (!last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // thread t sums partials t, t + 256, ...; then the tree of 256 (steps
-    // 128 and 64 in LDS, 32 .. 1 in wave 0)
-    double te = 0.0, tg = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) {
-        te += __hip_atomic_load(&blockpart[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tg += __hip_atomic_load(&blockpart[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    red_e[threadIdx.x] = te;
-    red_g[threadIdx.x] = tg;
-    __syncthreads();
-    for (int o = 128; o >= 64; o >>= 1) {
-        if (threadIdx.x < o) {
-            red_e[threadIdx.x] += red_e[threadIdx.x + o];
-            red_g[threadIdx.x] += red_g[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x >= 64) return;
-    const double se = wave_tree_sum(red_e[threadIdx.x]);
-    const double sg = wave_tree_sum(red_g[threadIdx.x]);
-    if (threadIdx.x == 0) {
-        const double E = se * fa.e_scale;
-        const double G = sqrt(sg) * fa.g_scale;
-        if (fa.record) {
-            const int h = dstate[ST_HIST];
-            if (h < fa.hist_cap) { fa.hist[2 * h] = E; fa.hist[2 * h + 1] = G; }
-            dstate[ST_HIST] = h + 1;
-            // stop rule of cvt_lloyd_until: this iteration's energy is at or
-            // below the target, or fell by less than rel_tol * E over the
-            // last `window` iterations of the call
-            if (dstate[ST_UACTIVE] && fa.until) {
-                const int k = h - dstate[ST_UBASE], win = dstate[ST_UWIN];
-                bool stop = E <= fa.until[1];
-                if (win > 0 && k >= win && h < fa.hist_cap)
-                    stop = stop || (fa.hist[2 * (h - win)] - E) < fa.until[0] * E;
-                if (stop) dstate[ST_STOP] = 1;
-            }
-        }
-        if (fa.out_eg) { fa.out_eg[0] = E; fa.out_eg[1] = G; }
-        dstate[ST_EPOCH] = dstate[ST_EPOCH] + 1;
-        dstate[ST_DONE] = 0;
-    }
+    cvt_finalize_block(blockpart, (int)gridDim.x, dstate, fa, red_e, red_g);
+}
+
+// A deferred finalize (one 256-thread workgroup), when one is pending.
+static __global__ __launch_bounds__(256) void k_finalize_pending(const double *__restrict__ blockpart,
+                                                                 int *__restrict__ dstate,
+                                                                 FinalArgs fa)
+{
+    __shared__ double red_e[256], red_g[256];
+    const int nb = dstate[ST_PEND];
+    if (nb > 0) cvt_finalize_block(blockpart, nb, dstate, fa, red_e, red_g);
 }
 
 // cvt_lloyd_until: arm the stop rule (relative decrease over `window`

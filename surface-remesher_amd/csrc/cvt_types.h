@@ -34,15 +34,17 @@ enum { FLAG_FULL = 1, FLAG_BAD = 2, NUM_FLAGS = 4 };
 // device state words
 // (ST_DONE counts the finished blocks of a site-update launch; the last block
 // to finish resets it.  ST_STOP, when set, makes every kernel of a Lloyd
-// iteration return at entry: cvt_lloyd_until arms the stop rule (ST_UACTIVE,
-// the history index it started at ST_UBASE, the window ST_UWIN) and the
-// update's last block raises ST_STOP when the rule holds.)
+// iteration return before its first write: cvt_lloyd_until arms the stop rule
+// (ST_UACTIVE, the history index it started at ST_UBASE, the window ST_UWIN)
+// and the iteration's finalize raises ST_STOP when the rule holds.  ST_PEND:
+// the number of blocks whose energy partials a deferred finalize still has to
+// add, 0 when none is pending.)
 enum {
     ST_EPOCH = 0, ST_HIST = 1, ST_OVFN = 2, ST_NSITES = 3, ST_DONE = 4,
-    ST_STOP = 5, ST_UBASE = 6, ST_UWIN = 7, ST_UACTIVE = 8, NUM_STATE = 9
+    ST_STOP = 5, ST_UBASE = 6, ST_UWIN = 7, ST_UACTIVE = 8, ST_PEND = 9, NUM_STATE = 10
 };
 
-// what the last block of a site update does with the energy partials
+// what the finalize of a site update does with the energy partials
 struct FinalArgs {
     double *hist;          // history (E, |grad E|) [hist_cap][2]
     int hist_cap;
@@ -50,6 +52,7 @@ struct FinalArgs {
     double e_scale, g_scale;
     double *out_eg;        // optional (E, |grad E|) copy
     const double *until;   // stop rule (relative decrease, target energy), read when ST_UACTIVE
+    int defer;             // leave the finalize to the next seed pass (or a finalize launch)
 };
 
 

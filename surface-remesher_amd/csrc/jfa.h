@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cvt_types.h"
+
 namespace cvt {
 
 // Launch shape of a JFA pass for one grid: the quad kernel (4 pixels per
@@ -37,7 +39,8 @@ int jfa_runs_mask(const JfaPlan &p, int W, int H);
 // (rstep == 0) or q + rstep * {-1, 0, 1}^2 (rstep > 0); otherwise pixel by
 // pixel.  The site table must hold capS + 1 entries rounded up to an even
 // count.
-// A non-null `stop` makes the pass return at entry while *stop is set.
+// A non-null `stop` makes the pass write nothing while *stop is set (it is
+// read with the pass's first loads).
 void launch_jfa_pass(hipStream_t st, const JfaPlan &plan, const uint32_t *in, uint32_t *out,
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
                      int nclear = 0, int rstep = 0, const int *stop = nullptr);
@@ -46,10 +49,12 @@ void launch_jfa_pass(hipStream_t st, const JfaPlan &plan, const uint32_t *in, ui
 // [0, S), none elsewhere), computed from the seeds into a map `out` that
 // holds none everywhere: the same output as launch_jfa_pass with in = seed.
 // Only the receivers q + step * {-1, 0, 1}^2 of the seed pixels q are
-// written.
+// written.  With a non-null fa, one more workgroup runs the previous site
+// update's deferred finalize (cvt_finalize.h) if one is pending in dstate.
 void launch_jfa_seed_pass(hipStream_t st, const uint32_t *seed, uint32_t *out,
                           const float2 *sites, int S, int W, int H, int step,
-                          const int *stop = nullptr);
+                          const int *stop = nullptr, const double *blockpart = nullptr,
+                          int *dstate = nullptr, const FinalArgs *fa = nullptr);
 
 // A patch of a batch, as the batched kernels see it.  Maps are concatenated
 // (pix_off), site tables too (site_off; entry site_off + cap is the patch's

@@ -28,6 +28,7 @@
 #include <stdint.h>
 
 #include "jfa.h"
+#include "cvt_finalize.h"
 
 #include <algorithm>
 
@@ -1128,10 +1129,19 @@ void launch_jfa_pass(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint3
 __global__ __launch_bounds__(256) void k_jfa_seeds(const uint32_t *__restrict__ seed,
                                                    uint32_t *__restrict__ out,
                                                    const float2 *__restrict__ sites, int S, int W,
-                                                   int H, int step, const int *__restrict__ stop)
+                                                   int H, int step, const int *__restrict__ stop,
+                                                   int fin, const double *__restrict__ blockpart,
+                                                   int *__restrict__ dstate, FinalArgs fa)
 {
+    if (fin && blockIdx.x == 0) {
+        // the previous iteration's deferred finalize, beside the scatter
+        __shared__ double red_e[256], red_g[256];
+        const int nb = dstate[ST_PEND];
+        if (nb > 0) cvt_finalize_block(blockpart, nb, dstate, fa, red_e, red_g);
+        return;
+    }
     const int halt = stop_word(stop);   // tested once the site's load is in flight
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (blockIdx.x - fin) * blockDim.x + threadIdx.x;
     const int i = t / 9, r = t - 9 * (t / 9);
     if (i >= S) return;
     const float2 s = sites[i];
@@ -1201,11 +1211,14 @@ void launch_jfa_batch_seeds(hipStream_t st, const BatchPatch *patches, const int
 }
 
 void launch_jfa_seed_pass(hipStream_t st, const uint32_t *seed, uint32_t *out,
-                          const float2 *sites, int S, int W, int H, int step, const int *stop)
+                          const float2 *sites, int S, int W, int H, int step, const int *stop,
+                          const double *blockpart, int *dstate, const FinalArgs *fa)
 {
-    if (S > 0)
-        hipLaunchKernelGGL(k_jfa_seeds, dim3((9 * S + 255) / 256), dim3(256), 0, st, seed, out,
-                           sites, S, W, H, step, stop);
+    const int fin = fa ? 1 : 0;
+    const int nb = (9 * S + 255) / 256 + fin;
+    if (nb > 0)
+        hipLaunchKernelGGL(k_jfa_seeds, dim3(nb), dim3(256), 0, st, seed, out, sites, S, W, H, step,
+                           stop, fin, blockpart, dstate, fa ? *fa : FinalArgs());
 }
 
 }  // namespace cvt

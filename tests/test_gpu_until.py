@@ -77,3 +77,28 @@ def test_cap_without_stop(gpu):
     with engine(w) as e:
         assert e.lloyd_until(25) == 25          # no window, no target
         assert len(e.history()) == 25
+
+
+def test_new_sites_after_a_stop(gpu, orc):
+    """a stop raised while the next iteration's seed scatter runs may leave
+    part of that scatter behind; new sites set afterwards still get the
+    oracle's exact label map, and a Lloyd run from them the same history as
+    on a fresh context"""
+    import cvtamd
+    w = workload("config2_1024")
+    g = np.random.default_rng(7)
+    other = (g.random((w.num_sites, 2)) * w.n).astype(np.float32)
+    with engine(w) as e:
+        n = e.lloyd_until(200, rel_tol=1e-3, window=5)
+        assert 5 < n < 200
+        e.set_sites(other)
+        e.voronoi()
+        assert np.array_equal(e.labels(), orc.jfa(w.n, w.n, other))
+        e.set_sites(other)
+        e.lloyd(5)
+        h = e.history()[-5:, 0]
+    with cvtamd.Engine(w.n, w.n, w.num_sites) as f:
+        f.set_density(w.density)
+        f.set_sites(other)
+        f.lloyd(5)
+        assert np.array_equal(h, f.history()[:, 0])
