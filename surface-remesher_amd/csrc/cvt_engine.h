@@ -65,8 +65,10 @@ struct Ctx {
     bool use_graph = true;
     bool graph_valid = false;
     float graph_omega = 0.0f;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;            // one iteration
     hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph_n = nullptr;          // kGraphIters iterations
+    hipGraphExec_t graph_exec_n = nullptr;
 
     ~Ctx();
     int enqueue_seed();
