@@ -63,6 +63,7 @@ struct Ctx {
 
 
     bool use_graph = true;
+    bool use_tail = true;          // fused last five JFA passes where the plan allows
     bool graph_valid = false;
     float graph_omega = 0.0f;
     hipGraph_t graph = nullptr;            // one iteration

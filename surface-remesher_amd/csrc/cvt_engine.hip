@@ -170,7 +170,14 @@ int Ctx::enqueue_jfa(hipEvent_t after_first)
     // no-op when nothing is pending.
     const FinalArgs fa = final_args(1, nullptr);
     if (!scatter) enqueue_finalize();
+    // with a small site table the last five passes run fused (launch_jfa_tail)
+    const bool tail = use_tail && jfa_tail_ok(jplan, np);
     for (int k = 0; k < np; ++k) {
+        if (tail && k == np - 5) {
+            launch_jfa_tail(stream, jplan, in, lab[cur], sites, W, H, dstate + ST_STOP);
+            cur ^= 1;
+            break;
+        }
         if (k == 0 && scatter) {
             launch_jfa_seed_pass(stream, seed, first, sites, S, W, H, step, dstate + ST_STOP,
                                  blockpart, dstate, &fa);

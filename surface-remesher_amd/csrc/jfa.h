@@ -45,6 +45,15 @@ void launch_jfa_pass(hipStream_t st, const JfaPlan &plan, const uint32_t *in, ui
                      const float2 *sites, int W, int H, int step, uint32_t *clear, uint32_t none,
                      int nclear = 0, int rstep = 0, const int *stop = nullptr);
 
+// The last five passes (steps 8, 4, 2, 1, 1) fused into one launch over
+// 32 x 32 tiles with a 16-pixel halo staged in LDS, for plans whose site
+// table is staged whole in a 16 or 40 KB array and JFAs of eight passes or
+// more: out = those five passes applied to `in` (the step-16 map), bit for
+// bit.  `stop` as in launch_jfa_pass.
+bool jfa_tail_ok(const JfaPlan &plan, int passes);
+void launch_jfa_tail(hipStream_t st, const JfaPlan &plan, const uint32_t *in, uint32_t *out,
+                     const float2 *sites, int W, int H, const int *stop = nullptr);
+
 // The first pass from a seed map (labels only at the seed pixels of sites
 // [0, S), none elsewhere), computed from the seeds into a map `out` that
 // holds none everywhere: the same output as launch_jfa_pass with in = seed.

@@ -866,6 +866,107 @@ __global__ __launch_bounds__(JFA_THREADS(ENTRIES), JFA_MINB(ENTRIES)) void k_jfa
 }
 
 // ---------------------------------------------------------------------------
+// Fused tail: the last five passes (steps 8, 4, 2, 1 and the extra 1) of one
+// map in one launch, for site tables that fit in LDS beside the tile.  Each
+// workgroup owns a kTailT x kTailT tile of the output.  It loads the tile and
+// a kTailHalo-pixel halo of the step-16 map into LDS and runs the five
+// passes there, each over the tile grown by the steps still to come (8, 4,
+// 2, 1, 0 pixels), so every tap a pass reads was written by the pass before
+// it.  Taps off the grid are redirected to the pixel's own row or column,
+// as in the dense pass, so the grid edges need no halo.  Each pixel takes
+// the (d, label) minimum over its nine taps with the dense pass's distance
+// formula: the map equals five dense passes bit for bit.
+// ---------------------------------------------------------------------------
+static constexpr int kTailT = 32, kTailHalo = 16, kTailP = kTailT + 2 * kTailHalo;
+
+template <int ENTRIES>
+__global__ __launch_bounds__(1024) void k_jfa_tail(const uint32_t *__restrict__ in,
+                                                   uint32_t *__restrict__ out,
+                                                   const float2 *__restrict__ sites, int table_f4,
+                                                   int W, int H, int tilesX,
+                                                   const int *__restrict__ stop)
+{
+    // the site table at LDS byte 0, then the two tile buffers
+    __shared__ float4 lds_all[ENTRIES / 2 + kTailP * kTailP / 2];
+    uint32_t *const buf0 = reinterpret_cast<uint32_t *>(lds_all + ENTRIES / 2);
+    uint32_t *const buf1 = buf0 + kTailP * kTailP;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tY = blockIdx.x / tilesX, tX = blockIdx.x - tY * tilesX;
+    const int X0 = tX * kTailT, Y0 = tY * kTailT;
+    const int bx = X0 - kTailHalo, by = Y0 - kTailHalo;   // the buffers' origin on the grid
+    const int halt = stop_word(stop);
+    {
+        const int x0 = max(bx, 0), x1 = min(X0 + kTailT + kTailHalo, W);
+        const int y0 = max(by, 0), y1 = min(Y0 + kTailT + kTailHalo, H);
+        const int w = x1 - x0, n = w * (y1 - y0);
+        for (int i = tid; i < n; i += nt) {
+            const int r = i / w, x = x0 + (i - r * w), y = y0 + r;
+            buf0[(y - by) * kTailP + (x - bx)] = in[(long)y * W + x];
+        }
+    }
+    SiteTab<TAB_LDS0> tab;
+    tab.lds = (lds_cchar *)lds_all;
+    tab.src = reinterpret_cast<const f32x4 *>(sites);
+    tab.n4 = table_f4;
+    tab.base = 0;
+    tab.stage();                     // ends with a barrier: buf0 is complete too
+    if (halt) return;
+    uint32_t *cur = buf0, *nxt = buf1;
+#pragma unroll 1
+    for (int k = 0; k < 5; ++k) {
+        const int s = k == 0 ? 8 : k == 1 ? 4 : k == 2 ? 2 : 1;
+        const int g = k == 0 ? 8 : k == 1 ? 4 : k == 2 ? 2 : k == 3 ? 1 : 0;   // halo after
+        const int x0 = max(X0 - g, 0), x1 = min(X0 + kTailT + g, W);
+        const int y0 = max(Y0 - g, 0), y1 = min(Y0 + kTailT + g, H);
+        const int w = x1 - x0, n = w * (y1 - y0);
+        for (int i = tid; i < n; i += nt) {
+            const int r = i / w, x = x0 + (i - r * w), y = y0 + r;
+            // tap columns and rows, off-grid ones redirected to the pixel's own
+            const int cx[3] = {x - s >= 0 ? x - s : x, x, x + s < W ? x + s : x};
+            const int cy[3] = {y - s >= 0 ? y - s : y, y, y + s < H ? y + s : y};
+            const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+            double best = 0.0;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const uint32_t l = cur[(cy[t / 3] - by) * kTailP + (cx[t % 3] - bx)];
+                const double key = cand(tab(l), l, px, py);
+                best = (t == 0) ? key : __builtin_fmin(best, key);
+            }
+            nxt[(y - by) * kTailP + (x - bx)] = (uint32_t)__double2loint(best);
+        }
+        __syncthreads();
+        uint32_t *const t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    const int x1 = min(X0 + kTailT, W), y1 = min(Y0 + kTailT, H);
+    const int w = x1 - X0, n = w * (y1 - Y0);
+    for (int i = tid; i < n; i += nt) {
+        const int r = i / w, x = X0 + (i - r * w), y = Y0 + r;
+        out[(long)y * W + x] = cur[(y - by) * kTailP + (x - bx)];
+    }
+}
+
+bool jfa_tail_ok(const JfaPlan &p, int passes)
+{
+    // the dense passes 1 and 2 (which reset the seed and first-pass maps)
+    // stay before the tail; the table fits beside the tile buffers
+    return passes >= 8 && (p.lds_entries == 2048 || p.lds_entries == 5120);
+}
+
+void launch_jfa_tail(hipStream_t st, const JfaPlan &p, const uint32_t *in, uint32_t *out,
+                     const float2 *sites, int W, int H, const int *stop)
+{
+    const int tx = (W + kTailT - 1) / kTailT, ty = (H + kTailT - 1) / kTailT;
+    if (p.lds_entries == 2048)
+        hipLaunchKernelGGL(k_jfa_tail<2048>, dim3(tx * ty), dim3(1024), 0, st, in, out, sites,
+                           p.table_f4, W, H, tx, stop);
+    else
+        hipLaunchKernelGGL(k_jfa_tail<5120>, dim3(tx * ty), dim3(1024), 0, st, in, out, sites,
+                           p.table_f4, W, H, tx, stop);
+}
+
+// ---------------------------------------------------------------------------
 // Batched pass: one launch over many patches.  Each workgroup takes one work
 // unit (patch, band of rows); patches run their passes right-aligned, so pass
 // k of the batch is pass k - (Pmax - P) of a patch with P passes (skipped
