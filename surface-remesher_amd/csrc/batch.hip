@@ -29,7 +29,11 @@
 namespace cvt {
 
 // moments of one tile of one patch (tile side T >= 64): the single-map tile
-// routine with 64-pixel segments, so a patch's sums equal its single-map run
+// routine with 64-pixel segments, so a patch's sums equal its single-map run.
+// HC: the hash capacity of the batch's largest tile (256, 512 or 1024 labels
+// for tiles of 64, 128 or 256, as in the single-map kernel): the smaller the
+// hash, the more tiles each CU keeps in flight.
+template <int HC>
 static __global__ __launch_bounds__(256) void k_moments_batch(
     const BatchPatch *__restrict__ patches, const int *__restrict__ tile_patch,
     const uint32_t *__restrict__ lab0, const uint32_t *__restrict__ lab1,
@@ -37,7 +41,7 @@ static __global__ __launch_bounds__(256) void k_moments_batch(
     long nrows, OvfRec *__restrict__ ovf, int ovf_cap, int *__restrict__ ovf_head, int *__restrict__ dflags,
     int *__restrict__ dstate)
 {
-    __shared__ TileLds<CVT_HASH_CAP> L;
+    __shared__ TileLds<HC> L;
     const int pid = tile_patch[blockIdx.x];
     const BatchPatch P = patches[pid];
     const int T = P.tile;
@@ -182,6 +186,7 @@ struct Batch {
     Slot *slots = nullptr;
     OvfRec *ovf = nullptr;
     int ovf_cap = 0;
+    int hash_cap = CVT_HASH_CAP;   // moment hash of the largest tile (set with the tiles)
     int *ovf_head = nullptr;
     double *blockpart = nullptr, *hist = nullptr, *d_escale = nullptr;
     int *hist_len = nullptr;
@@ -246,8 +251,12 @@ int Batch::plan()
     nblk = 0;
     tile_patch.clear();
     blk_patch.clear();
+    hash_cap = 0;
     for (int p = 0; p < np; ++p) {
         BatchPatch &P = hp[p];
+        hash_cap = std::max(hash_cap, P.tile == 64    ? TileHash<64>::HC
+                                      : P.tile == 128 ? TileHash<128>::HC
+                                                      : CVT_HASH_CAP);
         P.tiles_x = (P.W + P.tile - 1) / P.tile;
         P.tiles_y = (P.H + P.tile - 1) / P.tile;
         P.tile_off = ntiles;
@@ -334,9 +343,17 @@ void Batch::enqueue_jfa()
 int Batch::enqueue_iteration(float omega)
 {
     enqueue_jfa();
-    hipLaunchKernelGGL(k_moments_batch, dim3(ntiles), dim3(256), 0, stream, d_patch, d_tile_patch,
-                       lab0, lab1, rho, sites, slots, (long)total_sites, ovf, ovf_cap, ovf_head,
-                       dflags, dstate);
+#define CVT_MOMENTS_BATCH(HC)                                                                    \
+    hipLaunchKernelGGL(k_moments_batch<HC>, dim3(ntiles), dim3(256), 0, stream, d_patch,          \
+                       d_tile_patch, lab0, lab1, rho, sites, slots, (long)total_sites, ovf,       \
+                       ovf_cap, ovf_head, dflags, dstate)
+    if (hash_cap <= TileHash<64>::HC)
+        CVT_MOMENTS_BATCH(TileHash<64>::HC);
+    else if (hash_cap <= TileHash<128>::HC)
+        CVT_MOMENTS_BATCH(TileHash<128>::HC);
+    else
+        CVT_MOMENTS_BATCH(CVT_HASH_CAP);
+#undef CVT_MOMENTS_BATCH
     hipLaunchKernelGGL(k_update_batch, dim3(nblk), dim3(256), 0, stream, d_patch, d_blk_patch, sites,
                        fixed, slots, (long)total_sites, ovf, ovf_head, dstate, omega,
                        UPD_MOVE | UPD_SEED, blockpart,
@@ -607,7 +624,8 @@ int cvt_batch_synchronize(cvt_batch *bh)
     if (hipMemcpyAsync(f, b->dflags, sizeof f, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess)
         return fail(CVT_ERR_HIP, "synchronize");
-    if (f[FLAG_FULL]) return fail(CVT_ERR_CAPACITY, "a moment tile met more than %d sites", CVT_HASH_CAP);
+    if (f[FLAG_FULL])
+        return fail(CVT_ERR_CAPACITY, "a moment tile met more than %d sites", b->hash_cap);
     return CVT_OK;
 }
 
