@@ -63,6 +63,9 @@ def check_moments(m, ref, W_tol=1e-6):
     (704, 608, 20471, 18),    # runs with the smallest split table
     (256, 320, 900, 19),      # one 256-pixel segment per row
     (32, 40, 20, 20),         # runs on a grid narrower than one segment
+    (8, 200, 30, 21),         # fused tail (last five passes in LDS) on a grid one lane wide
+    (200, 8, 30, 22),         # fused tail on a grid eight rows high
+    (100, 70, 5000, 23),      # fused tail with the 40 KB table, a site on most pixels
 ])
 def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
     xy = rand_sites(seed, W, H, S)
