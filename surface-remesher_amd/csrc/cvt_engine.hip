@@ -302,8 +302,10 @@ int Ctx::lloyd_until(int max_iter, float omega, double rel_tol, int window, doub
     // Chunks of iterations are enqueued with a copy of the stop flag behind
     // each; before enqueuing chunk j + 1 the host waits for chunk j - 1 only
     // (chunk j is still queued, so the device never idles) and stops
-    // enqueuing once a copy reads set.  Iterations enqueued after the stop
-    // return at entry.
+    // enqueuing once a copy reads set.  The first chunk alone is waited for
+    // before the second is enqueued: a run that stops within it (a warm
+    // start close to convergence) then leaves no chunk of no-op iterations
+    // behind.  Iterations enqueued after the stop write nothing.
     constexpr int kChunk = kGraphIters;
     int rc = CVT_OK, done = 0;
     for (int j = 0; done < max_iter; ++j) {
@@ -316,12 +318,13 @@ int Ctx::lloyd_until(int max_iter, float omega, double rel_tol, int window, doub
             rc = fail(CVT_ERR_HIP, "stop-flag copy");
             break;
         }
-        if (j >= 1) {
-            if (hipEventSynchronize(until_ev[(j - 1) & 1]) != hipSuccess) {
+        if (j != 1) {   // chunk 0 alone, then chunk j - 1 while chunk j is queued
+            const int w = j == 0 ? 0 : j - 1;
+            if (hipEventSynchronize(until_ev[w & 1]) != hipSuccess) {
                 rc = fail(CVT_ERR_HIP, "stop-flag wait");
                 break;
             }
-            if (until_host[(j - 1) & 1]) break;
+            if (until_host[w & 1]) break;
         }
     }
     hipLaunchKernelGGL(k_until, dim3(1), dim3(64), 0, stream, dstate, until, 0, rel_tol, target,
