@@ -199,6 +199,11 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long m)
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+// the labels and densities of a lane's four pixels in one wave step
+struct MomLoad {
+    uint32_t lv[4], rv[4];
+};
+
 // Adds the runs of the tile [tx0, tx0 + T) x [ty0, ty0 + T) to the hash
 // (which it clears first).  lab: labels (byte offsets); a label's site index
 // is (lab >> 3) - key_base, valid when below S; sites: the S sites of this
@@ -206,10 +211,6 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long m)
 // cleared.  stop (may be null): the stop word of cvt_lloyd_until, read with
 // those loads; when it is set the routine returns true before adding
 // anything.
-struct MomLoad {
-    uint32_t lv[4], rv[4];
-};
-
 template <int SEG, int HC>
 __device__ __forceinline__ bool tile_moments(TileLds<HC> &L, const uint32_t *__restrict__ lab,
                                              const float *__restrict__ rho,
