@@ -773,13 +773,6 @@ __device__ __forceinline__ int seed_pixel(float v, int n)
     return (int)f;
 }
 
-// The single-map pass's prologue.  halt: the stop word of cvt_lloyd_until,
-// loaded at kernel entry and tested here, so its latency overlaps the first
-// loads (a stopped pass writes nothing).  nclear > 0: clear is a map whose
-// only entries sit at the seed pixels q of sites [0, nclear) (rstep == 0),
-// or at q + rstep * {-1, 0, 1}^2 (rstep > 0, the receivers of a seed pass);
-// they are reset one store per pixel of that set instead of W * H stores
-// (the pass does not read that map).
 // The stop word (0 without one), as a vector load: it returns in order with
 // the pass's first label loads, so testing it after them costs no wait of its
 // own (a scalar load would be waited for with the kernel arguments).  A null
@@ -789,6 +782,13 @@ __device__ __forceinline__ int stop_word(const int *stop)
     return (int)__builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(stop, stop ? 4u : 0u), 0, 0, 0);
 }
 
+// The single-map pass's prologue.  halt: the stop word of cvt_lloyd_until,
+// loaded at kernel entry and tested here, so its latency overlaps the first
+// loads (a stopped pass writes nothing).  nclear > 0: clear is a map whose
+// only entries sit at the seed pixels q of sites [0, nclear) (rstep == 0),
+// or at q + rstep * {-1, 0, 1}^2 (rstep > 0, the receivers of a seed pass);
+// they are reset one store per pixel of that set instead of W * H stores
+// (the pass does not read that map).
 struct SparseReset {
     int halt;
     uint32_t *clear;
