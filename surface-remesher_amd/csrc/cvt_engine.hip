@@ -66,7 +66,7 @@ int Ctx::enqueue_moments() { return launch_moments_on(this, labels_map()); }
 static int launch_moments_on(Ctx *c, const uint32_t *lab)
 {
     const float rs = (float)ldexp(1.0, -c->rho_exp);
-    dim3 grid(c->tilesX * c->tilesY), block(256);
+    dim3 grid(c->tilesX * c->tilesY), block(c->tile >= 64 ? 512 : 256);
     switch (c->tile) {
     case 32:
         hipLaunchKernelGGL(k_moments<32>, grid, block, 0, c->stream, lab, c->rho, c->sites, c->S,

@@ -159,21 +159,24 @@ __device__ __forceinline__ float seg_add(float v, int t, int src, int start)
 // scan of the lanes' carries (4 DPP steps).  Each run end becomes a record in
 // the wave's LDS queue (ballot + mbcnt compaction); the wave then converts
 // the records to site-relative fp64 sums and adds them, as int64 fixed point,
-// to the tile's LDS hash with integer atomics (exact, order-free).
+// to the tile's LDS hash with integer atomics (exact, order-free), whenever
+// its queue could overflow and at the end of each step.
 // ---------------------------------------------------------------------------
 struct TailRec {
     int key;
     float a, b, c;
     int xy;          // run start x | y << 16
 };
-#define CVT_TAILQ 256   // records per wave and step (4 pixels x 64 lanes)
+// queued records per wave: the queue is drained into the hash before a
+// sub-step (64 lanes, at most one record each) could overflow it
+#define CVT_TAILQ 128
 
 // LDS of one tile: a hash of HC labels (a power of two <= CVT_HASH_CAP) with
 // their fixed-point sums, and the waves' run-end queues
 template <int HC> struct TileLds {
     int keys[HC];
     unsigned long long acc[HC][4];
-    TailRec q[4][CVT_TAILQ];
+    TailRec q[8][CVT_TAILQ];   // one queue per wave, up to 8 waves
     int full;
 };
 
@@ -267,7 +270,8 @@ __device__ __forceinline__ bool tile_moments(TileLds<HC> &L, const uint32_t *__r
     tile_lds_init(L);
     if (halt) return true;
 
-    for (int g = wave; g < steps; g += 4) {
+    const int nw = blockDim.x >> 6;
+    for (int g = wave; g < steps; g += nw) {
         if (g > wave) step_load(g, ld);
         int x0, y;
         step_pos(g, x0, y);
@@ -348,12 +352,50 @@ __device__ __forceinline__ bool tile_moments(TileLds<HC> &L, const uint32_t *__r
         tl[1] = key[2] != key[1];
         tl[2] = key[3] != key[2];
         tl[3] = (seg_lane == SL - 1) || (knext != key[3]);
+        auto drain = [&](int nrec) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // one lane per record: site-relative fp64 sums into the hash
+            for (int e = lane; e < nrec; e += 64) {
+                const TailRec t = q[e];
+                const float2 sv = sites[t.key];
+                const double A = t.a, B = t.b, Cc = t.c;
+                const double ox = ((double)(t.xy & 0xffff) + 0.5) - (double)sv.x;
+                const double oy = ((double)(t.xy >> 16) + 0.5) - (double)sv.y;
+                const double DX = B + A * ox;
+                const double DY = A * oy;
+                const double E = Cc + 2.0 * ox * B + A * (ox * ox + oy * oy);
+                const unsigned hsh = ((unsigned)t.key * 2654435761u) >> (32 - __builtin_ctz(HC));
+                int slot = -1;
+                for (int probe = 0; probe < HC; ++probe) {
+                    const int h = (hsh + probe) & (HC - 1);
+                    const int old = atomicCAS(&L.keys[h], -1, t.key);
+                    if (old == -1 || old == t.key) { slot = h; break; }
+                }
+                if (slot >= 0) {
+                    atomicAdd(&L.acc[slot][0], (unsigned long long)cvt_fix(A, MOM_FW));
+                    atomicAdd(&L.acc[slot][1], (unsigned long long)cvt_fix(DX, MOM_FX));
+                    atomicAdd(&L.acc[slot][2], (unsigned long long)cvt_fix(DY, MOM_FX));
+                    atomicAdd(&L.acc[slot][3], (unsigned long long)cvt_fix(E, MOM_FE));
+                } else {
+                    L.full = 1;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
         // queue the run ends: records ordered by (j, lane)
         int base = 0;
         bool first_piece = true;                 // no head yet in this lane at <= j
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (hd[j]) first_piece = false;
+            if (base > CVT_TAILQ - 64) {   // uniform: base is the wave's count
+                drain(base);
+                base = 0;
+            }
             const bool m = tl[j] && key[j] >= 0;
             const unsigned long long bal = __ballot(m);
             if (m) {
@@ -370,38 +412,7 @@ __device__ __forceinline__ bool tile_moments(TileLds<HC> &L, const uint32_t *__r
             }
             base += __popcll(bal);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // one lane per record: site-relative fp64 sums into the hash
-        for (int e = lane; e < base; e += 64) {
-            const TailRec t = q[e];
-            const float2 sv = sites[t.key];
-            const double A = t.a, B = t.b, Cc = t.c;
-            const double ox = ((double)(t.xy & 0xffff) + 0.5) - (double)sv.x;
-            const double oy = ((double)(t.xy >> 16) + 0.5) - (double)sv.y;
-            const double DX = B + A * ox;
-            const double DY = A * oy;
-            const double E = Cc + 2.0 * ox * B + A * (ox * ox + oy * oy);
-            const unsigned hsh = ((unsigned)t.key * 2654435761u) >> (32 - __builtin_ctz(HC));
-            int slot = -1;
-            for (int probe = 0; probe < HC; ++probe) {
-                const int h = (hsh + probe) & (HC - 1);
-                const int old = atomicCAS(&L.keys[h], -1, t.key);
-                if (old == -1 || old == t.key) { slot = h; break; }
-            }
-            if (slot >= 0) {
-                atomicAdd(&L.acc[slot][0], (unsigned long long)cvt_fix(A, MOM_FW));
-                atomicAdd(&L.acc[slot][1], (unsigned long long)cvt_fix(DX, MOM_FX));
-                atomicAdd(&L.acc[slot][2], (unsigned long long)cvt_fix(DY, MOM_FX));
-                atomicAdd(&L.acc[slot][3], (unsigned long long)cvt_fix(E, MOM_FE));
-            } else {
-                L.full = 1;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        drain(base);
     }
     return false;
 }
@@ -452,9 +463,11 @@ __device__ __forceinline__ void tile_flush(TileLds<HC> &L, const float2 *__restr
 }
 
 // moments (reference replacement for pbaCVDComputeCentroid + the energy
-// texture, source/gcvt.cu:562-732, :788-802): one workgroup per T x T tile
+// texture, source/gcvt.cu:562-732, :788-802): one workgroup per T x T tile,
+// eight waves for tiles of 64 pixels or more (four for 32: a 32-pixel tile
+// has four wave steps)
 template <int T>
-static __global__ __launch_bounds__(256) void k_moments(
+static __global__ __launch_bounds__(512) void k_moments(
     const uint32_t *__restrict__ lab, const float *__restrict__ rho,
     const float2 *__restrict__ sites, int S, int W, int H, float rho_scale, int tilesX,
     int tilesY, Slot *__restrict__ slots, long nrows, OvfRec *__restrict__ ovf, int ovf_cap,
