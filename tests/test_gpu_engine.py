@@ -66,6 +66,7 @@ def check_moments(m, ref, W_tol=1e-6):
     (8, 200, 30, 21),         # fused tail (last five passes in LDS) on a grid one lane wide
     (200, 8, 30, 22),         # fused tail on a grid eight rows high
     (100, 70, 5000, 23),      # fused tail with the 40 KB table, a site on most pixels
+    (68, 40, 50, 24),         # the smallest JFA with a fused tail (eight passes)
 ])
 def test_jfa_labels_bit_exact(gpu, orc, W, H, S, seed):
     xy = rand_sites(seed, W, H, S)
@@ -360,3 +361,30 @@ def test_full_size_labels_and_mass(gpu, orc, cfg):
     assert m[:, 0].sum() == pytest.approx(float(w.density.astype(np.float64).sum()), rel=1e-9)
     ref = orc.moments(w.density, lab, w.sites)
     check_moments(m, ref)
+
+
+# ------------------------------------------------- iteration launch shapes --
+@pytest.mark.parametrize("counts", [(1, 7, 8, 9, 17), (16, 3, 8)])
+def test_graph_chunks_equal_single_iterations(gpu, counts):
+    """cvt_lloyd replays graphs of eight iterations and of one; any split of
+    the same iteration count gives the same sites and history bit for bit,
+    and so do the event-timed iterations of cvt_profile"""
+    n, S = 320, 300
+    xy = rand_sites(31, n, n, S)
+    rho = (0.5 + np.random.default_rng(32).random((n, n))).astype(np.float32)
+    total = sum(counts)
+    with make_engine(n, xy, rho) as a:
+        for c in counts:
+            a.lloyd(c)
+        ha, sa = a.history(), a.sites()
+    with make_engine(n, xy, rho) as b:
+        for _ in range(total):
+            b.lloyd(1)
+        hb, sb = b.history(), b.sites()
+    with make_engine(n, xy, rho) as p:
+        p.lloyd(total - 5)
+        p.profile(5)
+        hp, sp = p.history(), p.sites()
+    assert len(ha) == len(hb) == len(hp) == total
+    assert np.array_equal(ha, hb) and np.array_equal(sa, sb)
+    assert np.array_equal(ha, hp) and np.array_equal(sa, sp)
