@@ -136,22 +136,26 @@ __global__ __launch_bounds__(256) void k_conv3x3_t(const uint16_t *__restrict__ 
 // Weight-gradient partials of one tap on a BCO x BCI tile of (co, ci):
 // part[split][tap][co][ci] = sum over the split's pixels m of
 // g[m][co] * in[m + off(tap)][ci].  grid (9 * Cout/BCO * Cin/BCI, splits),
-// block 256.  The reduction runs over pixels in stages of 32; each stage's
+// block 256.  The reduction runs over pixels in stages of KM; each stage's
 // g and input tiles are staged in LDS transposed to [channel][pixel] (the
-// MFMA operands take 8 consecutive pixels per lane): a thread loads 4
-// pixels x 4 channels and writes 4 eight-byte rows.  The next stage's loads
-// are issued before this stage's MFMAs (two LDS buffers, one barrier per
-// stage).  Requires W % 4 == 0 and chunk % 32 == 0.
-template <int BCO, int BCI>
+// MFMA operands take 8 consecutive pixels per lane): a staging item is 4
+// pixels x 4 channels, written as 4 eight-byte rows, and every thread holds
+// IG items of g and IX of the input.  Each wave then runs KM / 32 MFMA
+// k-steps on each of its tiles, in pixel order, so the sums do not depend on
+// KM.  The next stage's loads are issued before this stage's MFMAs (two LDS
+// buffers, one barrier per stage).  Requires W % 4 == 0 and chunk % 4 == 0.
+template <int BCO, int BCI, int KM>
 __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict__ in,
                                                      const uint16_t *__restrict__ g,
                                                      float *__restrict__ part, int N, int H, int W,
                                                      int Cin, int Cout, long chunk)
 {
-    constexpr int KM = 32, LP = KM + 8;            // pixels per stage, LDS row pitch
+    constexpr int LP = KM + 8;                     // LDS row pitch (bf16)
     constexpr int NT = (BCO / 16) * (BCI / 16);    // 16x16 output tiles
     constexpr int TPW = NT / 4;                    // tiles per wave
-    constexpr int NG = 8 * (BCO / 4), NX = 8 * (BCI / 4);   // staging threads
+    constexpr int IG = (KM / 4) * (BCO / 4) / 256, IX = (KM / 4) * (BCI / 4) / 256;
+    static_assert(IG * 256 == (KM / 4) * (BCO / 4) && IX * 256 == (KM / 4) * (BCI / 4),
+                  "staging items must fill the block");
     __shared__ uint16_t sG[2][BCO * LP];
     __shared__ uint16_t sX[2][BCI * LP];
     const int nco = Cout / BCO, nci = Cin / BCI;
@@ -162,41 +166,43 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict_
     const long mb = (long)blockIdx.y * chunk, me = min(M, mb + chunk);
     const int ky = tap / 3 - 1, kx = tap % 3 - 1;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    // staging role: G rows (co) or X rows (ci); 4 pixels x 4 channels
-    const bool isG = t < NG, isX = !isG && t < NG + NX;
-    const int tt = isG ? t : t - NG;
-    const int cw = isG ? BCO / 4 : BCI / 4;
-    const int cgrp = (isG || isX) ? tt % cw : 0, pg = (isG || isX) ? tt / cw : 0;
-    const int ch = 4 * cgrp;
-    uint2 v[4];
+    // item r of a thread: pixel group (t + 256 r) / (channels / 4), channel
+    // group (t + 256 r) % (channels / 4)
+    uint2 vg[IG][4], vx[IX][4];
     auto load = [&](long mc) {
 #pragma unroll
-        for (int p = 0; p < 4; ++p) v[p] = make_uint2(0, 0);
-        const long m = mc + 4 * pg;                 // 4 pixels of one row (W % 4 == 0)
-        if (!(isG || isX) || m >= me) return;
-        const int n = (int)(m / ((long)H * W));
-        const int r = (int)(m - (long)n * H * W);
-        const int y = r / W, x = r - (r / W) * W;
-        if (isG) {
+        for (int r = 0; r < IG; ++r) {
+            const int it = t + 256 * r, ch = 4 * (it % (BCO / 4));
+            const long m = mc + 4 * (it / (BCO / 4));
 #pragma unroll
             for (int p = 0; p < 4; ++p)
-                if (m + p < me) v[p] = *reinterpret_cast<const uint2 *>(g + (m + p) * Cout + co0 + ch);
-        } else {
+                vg[r][p] = (m + p < me)
+                               ? *reinterpret_cast<const uint2 *>(g + (m + p) * Cout + co0 + ch)
+                               : make_uint2(0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < IX; ++r) {
+            const int it = t + 256 * r, ch = 4 * (it % (BCI / 4));
+            const long m = mc + 4 * (it / (BCI / 4));   // 4 pixels of one row
+#pragma unroll
+            for (int p = 0; p < 4; ++p) vx[r][p] = make_uint2(0, 0);
+            if (m >= me) continue;
+            const int n = (int)(m / ((long)H * W));
+            const int rr = (int)(m - (long)n * H * W);
+            const int y = rr / W, x = rr - (rr / W) * W;
             const int yy = y + ky;
-            if (yy < 0 || yy >= H) return;
+            if (yy < 0 || yy >= H) continue;
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
                 const int xx = x + p + kx;
                 if (m + p < me && xx >= 0 && xx < W)
-                    v[p] = *reinterpret_cast<const uint2 *>(
+                    vx[r][p] = *reinterpret_cast<const uint2 *>(
                         in + (((long)n * H + yy) * W + xx) * Cin + ci0 + ch);
             }
         }
     };
-    auto store = [&](int buf) {
-        if (!(isG || isX)) return;
-        uint16_t *dst = isG ? sG[buf] : sX[buf];
-        // channel q of the 4: its values at the 4 pixels, one 8-byte row piece
+    // channel q of an item: its values at the 4 pixels, one 8-byte row piece
+    auto put = [&](uint16_t *dst, const uint2 (&v)[4], int ch, int pg) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint32_t e0 = ((q < 2 ? v[0].x : v[0].y) >> (16 * (q & 1))) & 0xffffu;
@@ -205,6 +211,18 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict_
             const uint32_t e3 = ((q < 2 ? v[3].x : v[3].y) >> (16 * (q & 1))) & 0xffffu;
             *reinterpret_cast<uint2 *>(&dst[(ch + q) * LP + 4 * pg]) =
                 make_uint2(e0 | (e1 << 16), e2 | (e3 << 16));
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < IG; ++r) {
+            const int it = t + 256 * r;
+            put(sG[buf], vg[r], 4 * (it % (BCO / 4)), it / (BCO / 4));
+        }
+#pragma unroll
+        for (int r = 0; r < IX; ++r) {
+            const int it = t + 256 * r;
+            put(sX[buf], vx[r], 4 * (it % (BCI / 4)), it / (BCI / 4));
         }
     };
     f32x4 acc[TPW];
@@ -218,15 +236,17 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict_
         const bool more = mc + KM < me;
         if (more) load(mc + KM);
 #pragma unroll
-        for (int k = 0; k < TPW; ++k) {
-            const int tile = wave + 4 * k;
-            const int tco = tile / (BCI / 16), tci = tile % (BCI / 16);
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
-                &sG[buf][(tco * 16 + (lane & 15)) * LP + 8 * (lane >> 4)]);
-            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(
-                &sX[buf][(tci * 16 + (lane & 15)) * LP + 8 * (lane >> 4)]);
-            acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[k], 0, 0, 0);
-        }
+        for (int kk = 0; kk < KM / 32; ++kk)
+#pragma unroll
+            for (int k = 0; k < TPW; ++k) {
+                const int tile = wave + 4 * k;
+                const int tco = tile / (BCI / 16), tci = tile % (BCI / 16);
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
+                    &sG[buf][(tco * 16 + (lane & 15)) * LP + 32 * kk + 8 * (lane >> 4)]);
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(
+                    &sX[buf][(tci * 16 + (lane & 15)) * LP + 32 * kk + 8 * (lane >> 4)]);
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[k], 0, 0, 0);
+            }
         if (more) store(buf ^ 1);
         __syncthreads();
         buf ^= 1;
@@ -271,6 +291,7 @@ __global__ void k_sum_splits(const float *__restrict__ part, int splits, long n,
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = 0.0f;
+#pragma unroll 8
     for (int k = 0; k < splits; ++k) s += part[(long)k * n + i];
     out[i] = accumulate ? out[i] + s : s;
 }
@@ -721,14 +742,14 @@ void nn_conv3x3(hipStream_t s, const uint16_t *in, const uint16_t *w, const floa
                            bias, out, N, H, W, Cin, Cout, relu);
 }
 
-// pixel splits of a weight gradient: about 2048 blocks per launch, at most
-// 64 splits and at least 1024 pixels per split
+// pixel splits of a weight gradient: about 8192 blocks per launch, at most
+// 256 splits and at least 1024 pixels per split
 int nn_dw_splits(long M, int Cin, int Cout)
 {
     const int bco = (Cout % 64 == 0) ? 64 : 32, bci = (Cin % 64 == 0) ? 64 : 32;
     const long per_split = 9L * (Cout / bco) * (Cin / bci);
-    const long want = std::max<long>(1, 2048 / per_split);
-    return (int)std::max<long>(1, std::min<long>({64, want, M / 1024}));
+    const long want = std::max<long>(1, 8192 / per_split);
+    return (int)std::max<long>(1, std::min<long>({256, want, M / 1024}));
 }
 
 void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *dw, float *work,
@@ -736,21 +757,23 @@ void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *
 {
     const long M = (long)N * H * W;
     const int splits = nn_dw_splits(M, Cin, Cout);
-    const long chunk = ((M + splits - 1) / splits + 31) / 32 * 32;
     const int bco = (Cout % 64 == 0) ? 64 : 32, bci = (Cin % 64 == 0) ? 64 : 32;
+    // pixels per stage: 128, or 64 for the 64 x 64 tile (LDS for 4 blocks per CU)
+    const long km = (bco == 64 && bci == 64) ? 64 : 128;
+    const long chunk = ((M + splits - 1) / splits + km - 1) / km * km;
     const dim3 grid(9 * (Cout / bco) * (Cin / bci), splits);
     if (bco == 64 && bci == 64)
-        hipLaunchKernelGGL((k_conv3x3_dw2<64, 64>), grid, dim3(256), 0, s, in, g, work, N, H, W, Cin,
-                           Cout, chunk);
+        hipLaunchKernelGGL((k_conv3x3_dw2<64, 64, 64>), grid, dim3(256), 0, s, in, g, work, N, H, W,
+                           Cin, Cout, chunk);
     else if (bco == 64)
-        hipLaunchKernelGGL((k_conv3x3_dw2<64, 32>), grid, dim3(256), 0, s, in, g, work, N, H, W, Cin,
-                           Cout, chunk);
+        hipLaunchKernelGGL((k_conv3x3_dw2<64, 32, 128>), grid, dim3(256), 0, s, in, g, work, N, H, W,
+                           Cin, Cout, chunk);
     else if (bci == 64)
-        hipLaunchKernelGGL((k_conv3x3_dw2<32, 64>), grid, dim3(256), 0, s, in, g, work, N, H, W, Cin,
-                           Cout, chunk);
+        hipLaunchKernelGGL((k_conv3x3_dw2<32, 64, 128>), grid, dim3(256), 0, s, in, g, work, N, H, W,
+                           Cin, Cout, chunk);
     else
-        hipLaunchKernelGGL((k_conv3x3_dw2<32, 32>), grid, dim3(256), 0, s, in, g, work, N, H, W, Cin,
-                           Cout, chunk);
+        hipLaunchKernelGGL((k_conv3x3_dw2<32, 32, 128>), grid, dim3(256), 0, s, in, g, work, N, H, W,
+                           Cin, Cout, chunk);
     const long n = 9L * Cout * Cin;
     hipLaunchKernelGGL(k_sum_splits, dim3(nblk(n)), dim3(256), 0, s, work, splits, n, dw, accumulate);
 }
