@@ -32,11 +32,14 @@ void nn_up_concat(hipStream_t s, const uint16_t *low, const uint16_t *skip, uint
                   int H, int W, int C1, int C2);
 void nn_up_concat_bwd(hipStream_t s, const uint16_t *gout, uint16_t *glow, uint16_t *gskip, int N,
                       int H, int W, int C1, int C2, int acc_skip);
-// 1x1 head C -> 2 (fp32 out); dwh_db = [dwh (2C), dbh (2)]; work 256 * (2C + 2)
+// 1x1 head NN_HEAD_C -> 2 (fp32 out); dwh_db = [dwh (2C), dbh (2)]; work
+// nn_head_splits(M) * (2C + 2) floats
+constexpr int NN_HEAD_C = 32;
+int nn_head_splits(long M);
 void nn_head_fwd(hipStream_t s, const uint16_t *x, const float *wh, const float *bh, float *f,
-                 long M, int C);
+                 long M);
 void nn_head_bwd(hipStream_t s, const uint16_t *x, const float *gf, const float *wh, uint16_t *gx,
-                 float *dwh_db, float *work, long M, int C);
+                 float *dwh_db, float *work, long M);
 // bilinear sample of (nb, R, R, 2) at uv [nb][S][2]; loss partials per 256
 // sites; gfield (zeroed by the caller) receives the gradient
 void nn_sample_loss(hipStream_t s, const float *field, int R, const float *uv, const float *target,

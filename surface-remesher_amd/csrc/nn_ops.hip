@@ -490,54 +490,88 @@ __global__ void k_up_concat_bwd(const uint16_t *__restrict__ gout, uint16_t *__r
     }
 }
 
-// head: 1x1 conv C -> 2 in fp32: f[m][j] = b[j] + sum_c x[m][c] wh[j][c]
+// 1x1 head, one pixel per thread: f[m] = bh + sum_c x[m][c] * wh[.][c], the
+// pixel's NN_HEAD_C bf16 channels read as 16-byte words, summed in channel
+// order
 __global__ void k_head_fwd(const uint16_t *__restrict__ x, const float *__restrict__ wh,
-                           const float *__restrict__ bh, float *__restrict__ f, long M, int C)
+                           const float *__restrict__ bh, float *__restrict__ f, long M)
 {
+    constexpr int C = NN_HEAD_C;
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
     float a = bh[0], b = bh[1];
-    for (int c = 0; c < C; ++c) {
-        const float v = bf2f(x[m * C + c]);
-        a += v * wh[c];
-        b += v * wh[C + c];
+    const uint4 *src = reinterpret_cast<const uint4 *>(x + m * C);
+#pragma unroll
+    for (int q = 0; q < C / 8; ++q) {
+        const uint4 u = src[q];
+        const uint32_t wd[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int c = 8 * q + e;
+            const float v = bf2f((uint16_t)(wd[e >> 1] >> (16 * (e & 1))));
+            a += v * wh[c];
+            b += v * wh[C + c];
+        }
     }
     f[2 * m] = a;
     f[2 * m + 1] = b;
 }
 
 // head gradients: gx[m][c] = sum_j gf[m][j] wh[j][c]; per-split partials of
-// dwh[j][c] = sum_m gf[m][j] x[m][c] and dbh[j] = sum_m gf[m][j]
+// dwh[j][c] = sum_m gf[m][j] x[m][c] and dbh[j] = sum_m gf[m][j].  A thread
+// takes every 256th pixel of the block's split (16-byte loads and stores of
+// the pixel's channels), then each partial is summed by a fixed butterfly
+// over the wave and the four waves in order (deterministic).
 __global__ __launch_bounds__(256) void k_head_bwd(const uint16_t *__restrict__ x,
                                                   const float *__restrict__ gf,
                                                   const float *__restrict__ wh,
                                                   uint16_t *__restrict__ gx, float *__restrict__ part,
-                                                  long M, int C, long chunk)
+                                                  long M, long chunk)
 {
-    __shared__ float red[256];
+    constexpr int C = NN_HEAD_C, NP = 2 * C + 2;
+    __shared__ float red[4][NP];
     const long mb = (long)blockIdx.x * chunk, me = min(M, mb + chunk);
-    float accw[2 * 32 + 2];
-    for (int k = 0; k < 2 * C + 2; ++k) accw[k] = 0.0f;
+    float accw[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) accw[k] = 0.0f;
     for (long m = mb + threadIdx.x; m < me; m += 256) {
         const float g0 = gf[2 * m], g1 = gf[2 * m + 1];
-        for (int c = 0; c < C; ++c) {
-            const float v = bf2f(x[m * C + c]);
-            gx[m * C + c] = f2bf(g0 * wh[c] + g1 * wh[C + c]);
-            accw[c] += g0 * v;
-            accw[C + c] += g1 * v;
+        const uint4 *src = reinterpret_cast<const uint4 *>(x + m * C);
+        uint4 *dst = reinterpret_cast<uint4 *>(gx + m * C);
+#pragma unroll
+        for (int q = 0; q < C / 8; ++q) {
+            const uint4 u = src[q];
+            const uint32_t wd[4] = {u.x, u.y, u.z, u.w};
+            uint32_t o[4];
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+                const int c = 8 * q + e;
+                const float v0 = bf2f((uint16_t)(wd[e >> 1] & 0xffffu));
+                const float v1 = bf2f((uint16_t)(wd[e >> 1] >> 16));
+                o[e >> 1] = (uint32_t)f2bf(g0 * wh[c] + g1 * wh[C + c]) |
+                            ((uint32_t)f2bf(g0 * wh[c + 1] + g1 * wh[C + c + 1]) << 16);
+                accw[c] += g0 * v0;
+                accw[C + c] += g1 * v0;
+                accw[c + 1] += g0 * v1;
+                accw[C + c + 1] += g1 * v1;
+            }
+            dst[q] = make_uint4(o[0], o[1], o[2], o[3]);
         }
         accw[2 * C] += g0;
         accw[2 * C + 1] += g1;
     }
-    for (int k = 0; k < 2 * C + 2; ++k) {
-        red[threadIdx.x] = accw[k];
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[(long)blockIdx.x * (2 * C + 2) + k] = red[0];
-        __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        float v = accw[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NP) {
+        const int k = threadIdx.x;
+        part[(long)blockIdx.x * NP + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
     }
 }
 
@@ -827,19 +861,21 @@ void nn_up_concat_bwd(hipStream_t s, const uint16_t *gout, uint16_t *glow, uint1
 }
 
 void nn_head_fwd(hipStream_t s, const uint16_t *x, const float *wh, const float *bh, float *f,
-                 long M, int C)
+                 long M)
 {
-    hipLaunchKernelGGL(k_head_fwd, dim3(nblk(M)), dim3(256), 0, s, x, wh, bh, f, M, C);
+    hipLaunchKernelGGL(k_head_fwd, dim3(nblk(M)), dim3(256), 0, s, x, wh, bh, f, M);
 }
 
+int nn_head_splits(long M) { return (int)std::min<long>(1024, std::max<long>(1, M / 1024)); }
+
 void nn_head_bwd(hipStream_t s, const uint16_t *x, const float *gf, const float *wh, uint16_t *gx,
-                 float *dwh_db, float *work, long M, int C)
+                 float *dwh_db, float *work, long M)
 {
-    const int splits = (int)std::min<long>(256, std::max<long>(1, M / 1024));
+    const int splits = nn_head_splits(M);
     const long chunk = (M + splits - 1) / splits;
-    hipLaunchKernelGGL(k_head_bwd, dim3(splits), dim3(256), 0, s, x, gf, wh, gx, work, M, C, chunk);
-    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(2 * C + 2)), dim3(256), 0, s, work, splits,
-                       (long)(2 * C + 2), dwh_db, 0);
+    constexpr long NP = 2 * NN_HEAD_C + 2;
+    hipLaunchKernelGGL(k_head_bwd, dim3(splits), dim3(256), 0, s, x, gf, wh, gx, work, M, chunk);
+    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(NP)), dim3(256), 0, s, work, splits, NP, dwh_db, 0);
 }
 
 void nn_sample_loss(hipStream_t s, const float *field, int R, const float *uv, const float *target,

@@ -29,7 +29,7 @@
 namespace cvt {
 
 static const int IN_C = 32;   // input channels padded to the conv's K granularity
-static const int HEAD_C = 32;
+static const int HEAD_C = NN_HEAD_C;
 
 struct ConvSpec {
     int cin, cout, level;     // level: resolution R >> level
@@ -128,14 +128,14 @@ void UNet::forward(int nb)
     nn_up_concat(stream, a[11], a[1], up[2], nb, R1, R1, 64, 32);
     conv(12, up[2]);
     conv(13, a[12]);
-    nn_head_fwd(stream, a[13], p + whoff, p + bhoff, field, npix(0, nb), HEAD_C);
+    nn_head_fwd(stream, a[13], p + whoff, p + bhoff, field, npix(0, nb));
 }
 
 void UNet::backward(int nb)
 {
     const int R0 = res(0), R1 = res(1), R2 = res(2), R3 = res(3);
     // head: gfield -> ga[13], dwh, dbh
-    nn_head_bwd(stream, a[13], gfield, p + whoff, ga[13], g + whoff, work, npix(0, nb), HEAD_C);
+    nn_head_bwd(stream, a[13], gfield, p + whoff, ga[13], g + whoff, work, npix(0, nb));
     // one conv layer: mask the output gradient by ReLU', weight / bias
     // gradients, then the input gradient into gin (if any)
     auto layer = [&](int l, const uint16_t *in, uint16_t *gin) {
@@ -228,7 +228,7 @@ int cvt_unet_create(cvt_unet **out, int R, int max_batch, unsigned long long see
     alloc((void **)&u->field, sizeof(float) * 2 * u->npix(0, B));
     alloc((void **)&u->gfield, sizeof(float) * 2 * u->npix(0, B));
     // split-K workspace: the largest conv weight gradient and the head partials
-    long wn = 256L * (2 * HEAD_C + 2);
+    long wn = (long)nn_head_splits(u->npix(0, B)) * (2 * HEAD_C + 2);
     for (int l = 0; l < 14; ++l)
         wn = std::max({wn,
                        (long)nn_dw_splits(u->npix(SPEC[l].level, B), SPEC[l].cin, SPEC[l].cout) * 9L *
