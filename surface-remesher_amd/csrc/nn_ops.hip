@@ -133,41 +133,43 @@ __global__ __launch_bounds__(256) void k_conv3x3_t(const uint16_t *__restrict__ 
     }
 }
 
-// Weight-gradient partials of one tap on a BCO x BCI tile of (co, ci):
+// Weight-gradient partials of TG taps on a BCO x BCI tile of (co, ci):
 // part[split][tap][co][ci] = sum over the split's pixels m of
-// g[m][co] * in[m + off(tap)][ci].  grid (9 * Cout/BCO * Cin/BCI, splits),
-// block 256.  The reduction runs over pixels in stages of KM; each stage's
-// g and input tiles are staged in LDS transposed to [channel][pixel] (the
-// MFMA operands take 8 consecutive pixels per lane): a staging item is 4
-// pixels x 4 channels, written as 4 eight-byte rows, and every thread holds
-// IG items of g and IX of the input.  Each wave then runs KM / 32 MFMA
-// k-steps on each of its tiles, in pixel order, so the sums do not depend on
-// KM.  The next stage's loads are issued before this stage's MFMAs (two LDS
-// buffers, one barrier per stage).  Requires W % 4 == 0 and chunk % 4 == 0.
-template <int BCO, int BCI, int KM>
+// g[m][co] * in[m + off(tap)][ci].  grid (9 / TG * Cout/BCO * Cin/BCI,
+// splits), block 256; TG = 3 takes the three taps of one kernel row, which
+// share the g tile and read the same input rows.  The reduction runs over
+// pixels in stages of KM; each stage's g tile and TG tap-shifted input tiles
+// are staged in LDS transposed to [channel][pixel] (the MFMA operands take 8
+// consecutive pixels per lane): a staging item is 4 pixels x 4 channels,
+// written as 4 eight-byte rows, dealt to the 256 threads in turn.  Each wave
+// then runs KM / 32 MFMA k-steps on each of its tiles, in pixel order, so
+// the sums depend on neither KM nor TG.  The next stage's loads are issued
+// before this stage's MFMAs (two LDS buffers, one barrier per stage).
+// Requires W % 4 == 0 and chunk % 4 == 0.
+template <int BCO, int BCI, int KM, int TG>
 __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict__ in,
                                                      const uint16_t *__restrict__ g,
                                                      float *__restrict__ part, int N, int H, int W,
                                                      int Cin, int Cout, long chunk)
 {
     constexpr int LP = KM + 8;                     // LDS row pitch (bf16)
-    constexpr int NT = (BCO / 16) * (BCI / 16);    // 16x16 output tiles
-    constexpr int TPW = NT / 4;                    // tiles per wave
-    constexpr int IG = (KM / 4) * (BCO / 4) / 256, IX = (KM / 4) * (BCI / 4) / 256;
-    static_assert(IG * 256 == (KM / 4) * (BCO / 4) && IX * 256 == (KM / 4) * (BCI / 4),
-                  "staging items must fill the block");
+    constexpr int NTT = (BCO / 16) * (BCI / 16);   // 16x16 output tiles per tap
+    constexpr int TPW = TG * NTT / 4;              // tiles per wave
+    constexpr int NGI = (KM / 4) * (BCO / 4);      // g items per stage
+    constexpr int NXT = (KM / 4) * (BCI / 4);      // input items per tap and stage
+    constexpr int IG = (NGI + 255) / 256, IX = (TG * NXT + 255) / 256;
+    static_assert(TPW * 4 == TG * NTT, "tiles must fill the waves");
     __shared__ uint16_t sG[2][BCO * LP];
-    __shared__ uint16_t sX[2][BCI * LP];
+    __shared__ uint16_t sX[2][TG][BCI * LP];
     const int nco = Cout / BCO, nci = Cin / BCI;
     const int bx = blockIdx.x;
-    const int tap = bx / (nco * nci);
+    const int tap0 = (bx / (nco * nci)) * TG;
     const int co0 = ((bx / nci) % nco) * BCO, ci0 = (bx % nci) * BCI;
     const long M = (long)N * H * W;
     const long mb = (long)blockIdx.y * chunk, me = min(M, mb + chunk);
-    const int ky = tap / 3 - 1, kx = tap % 3 - 1;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    // item r of a thread: pixel group (t + 256 r) / (channels / 4), channel
-    // group (t + 256 r) % (channels / 4)
+    // g item it: pixel group it / (BCO / 4), channel group it % (BCO / 4);
+    // input item it: tap tap0 + it / NXT, then the same split of it % NXT
     uint2 vg[IG][4], vx[IX][4];
     auto load = [&](long mc) {
 #pragma unroll
@@ -176,17 +178,19 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict_
             const long m = mc + 4 * (it / (BCO / 4));
 #pragma unroll
             for (int p = 0; p < 4; ++p)
-                vg[r][p] = (m + p < me)
+                vg[r][p] = (it < NGI && m + p < me)
                                ? *reinterpret_cast<const uint2 *>(g + (m + p) * Cout + co0 + ch)
                                : make_uint2(0, 0);
         }
 #pragma unroll
         for (int r = 0; r < IX; ++r) {
-            const int it = t + 256 * r, ch = 4 * (it % (BCI / 4));
-            const long m = mc + 4 * (it / (BCI / 4));   // 4 pixels of one row
+            const int it = t + 256 * r, j = it / NXT, rest = it % NXT;
+            const int ch = 4 * (rest % (BCI / 4));
+            const long m = mc + 4 * (rest / (BCI / 4));   // 4 pixels of one row
 #pragma unroll
             for (int p = 0; p < 4; ++p) vx[r][p] = make_uint2(0, 0);
-            if (m >= me) continue;
+            if (j >= TG || m >= me) continue;
+            const int tap = tap0 + j, ky = tap / 3 - 1, kx = tap % 3 - 1;
             const int n = (int)(m / ((long)H * W));
             const int rr = (int)(m - (long)n * H * W);
             const int y = rr / W, x = rr - (rr / W) * W;
@@ -217,12 +221,12 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict_
 #pragma unroll
         for (int r = 0; r < IG; ++r) {
             const int it = t + 256 * r;
-            put(sG[buf], vg[r], 4 * (it % (BCO / 4)), it / (BCO / 4));
+            if (it < NGI) put(sG[buf], vg[r], 4 * (it % (BCO / 4)), it / (BCO / 4));
         }
 #pragma unroll
         for (int r = 0; r < IX; ++r) {
-            const int it = t + 256 * r;
-            put(sX[buf], vx[r], 4 * (it % (BCI / 4)), it / (BCI / 4));
+            const int it = t + 256 * r, j = it / NXT, rest = it % NXT;
+            if (j < TG) put(sX[buf][j], vx[r], 4 * (rest % (BCI / 4)), rest / (BCI / 4));
         }
     };
     f32x4 acc[TPW];
@@ -239,12 +243,12 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict_
         for (int kk = 0; kk < KM / 32; ++kk)
 #pragma unroll
             for (int k = 0; k < TPW; ++k) {
-                const int tile = wave + 4 * k;
-                const int tco = tile / (BCI / 16), tci = tile % (BCI / 16);
+                const int tile = wave + 4 * k, j = tile / NTT, rem = tile % NTT;
+                const int tco = rem / (BCI / 16), tci = rem % (BCI / 16);
                 const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
                     &sG[buf][(tco * 16 + (lane & 15)) * LP + 32 * kk + 8 * (lane >> 4)]);
                 const bf16x8 b = *reinterpret_cast<const bf16x8 *>(
-                    &sX[buf][(tci * 16 + (lane & 15)) * LP + 32 * kk + 8 * (lane >> 4)]);
+                    &sX[buf][j][(tci * 16 + (lane & 15)) * LP + 32 * kk + 8 * (lane >> 4)]);
                 acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[k], 0, 0, 0);
             }
         if (more) store(buf ^ 1);
@@ -252,11 +256,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw2(const uint16_t *__restrict_
         buf ^= 1;
     }
     const long tap_sz = (long)Cout * Cin;
-    float *dst = part + (long)blockIdx.y * 9 * tap_sz + (long)tap * tap_sz;
 #pragma unroll
     for (int k = 0; k < TPW; ++k) {
-        const int tile = wave + 4 * k;
-        const int tco = tile / (BCI / 16), tci = tile % (BCI / 16);
+        const int tile = wave + 4 * k, j = tile / NTT, rem = tile % NTT;
+        const int tco = rem / (BCI / 16), tci = rem % (BCI / 16);
+        float *dst = part + (long)blockIdx.y * 9 * tap_sz + (long)(tap0 + j) * tap_sz;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int co = co0 + tco * 16 + 4 * (lane >> 4) + e;
@@ -792,22 +796,23 @@ void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *
     const long M = (long)N * H * W;
     const int splits = nn_dw_splits(M, Cin, Cout);
     const int bco = (Cout % 64 == 0) ? 64 : 32, bci = (Cin % 64 == 0) ? 64 : 32;
-    // pixels per stage: 128, or 64 for the 64 x 64 tile (LDS for 4 blocks per CU)
-    const long km = (bco == 64 && bci == 64) ? 64 : 128;
-    const long chunk = ((M + splits - 1) / splits + km - 1) / km * km;
-    const dim3 grid(9 * (Cout / bco) * (Cin / bci), splits);
+    // 64 pixels per stage; a kernel row of three taps per block, but one tap
+    // for the 64 x 64 tile (LDS for four blocks per CU)
+    const long chunk = ((M + splits - 1) / splits + 63) / 64 * 64;
+    const int tg = (bco == 64 && bci == 64) ? 1 : 3;
+    const dim3 grid(9 / tg * (Cout / bco) * (Cin / bci), splits);
     if (bco == 64 && bci == 64)
-        hipLaunchKernelGGL((k_conv3x3_dw2<64, 64, 64>), grid, dim3(256), 0, s, in, g, work, N, H, W,
-                           Cin, Cout, chunk);
+        hipLaunchKernelGGL((k_conv3x3_dw2<64, 64, 64, 1>), grid, dim3(256), 0, s, in, g, work, N, H,
+                           W, Cin, Cout, chunk);
     else if (bco == 64)
-        hipLaunchKernelGGL((k_conv3x3_dw2<64, 32, 128>), grid, dim3(256), 0, s, in, g, work, N, H, W,
-                           Cin, Cout, chunk);
+        hipLaunchKernelGGL((k_conv3x3_dw2<64, 32, 64, 3>), grid, dim3(256), 0, s, in, g, work, N, H,
+                           W, Cin, Cout, chunk);
     else if (bci == 64)
-        hipLaunchKernelGGL((k_conv3x3_dw2<32, 64, 128>), grid, dim3(256), 0, s, in, g, work, N, H, W,
-                           Cin, Cout, chunk);
+        hipLaunchKernelGGL((k_conv3x3_dw2<32, 64, 64, 3>), grid, dim3(256), 0, s, in, g, work, N, H,
+                           W, Cin, Cout, chunk);
     else
-        hipLaunchKernelGGL((k_conv3x3_dw2<32, 32, 128>), grid, dim3(256), 0, s, in, g, work, N, H, W,
-                           Cin, Cout, chunk);
+        hipLaunchKernelGGL((k_conv3x3_dw2<32, 32, 64, 3>), grid, dim3(256), 0, s, in, g, work, N, H,
+                           W, Cin, Cout, chunk);
     const long n = 9L * Cout * Cin;
     hipLaunchKernelGGL(k_sum_splits, dim3(nblk(n)), dim3(256), 0, s, work, splits, n, dw, accumulate);
 }
