@@ -16,13 +16,15 @@ void nn_conv3x3(hipStream_t s, const uint16_t *in, const uint16_t *w, const floa
 void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *dw, float *work,
                    int N, int H, int W, int Cin, int Cout, int accumulate);
 int nn_dw_splits(long M, int Cin, int Cout);
-// db[c] = sum_m g[m][c]; work holds nn_colsum_splits(M) * C floats
+// db[c] = sum_m g[m][c]; work holds nn_colsum_splits(M) * C floats.  With
+// relu_y, g is first masked in place by the ReLU derivative of the layer's
+// output relu_y (g = 0 where relu_y <= 0).
 // splits of the column sum over M rows (its workspace holds splits * C floats)
 int nn_colsum_splits(long M);
-void nn_colsum(hipStream_t s, const uint16_t *g, long M, int C, float *db, float *work);
+void nn_colsum(hipStream_t s, uint16_t *g, long M, int C, float *db, float *work,
+               const uint16_t *relu_y);
 // fp32 weights -> bf16 forward copy and flipped/transposed data-gradient copy
 void nn_weights_bf16(hipStream_t s, const float *w, int Cin, int Cout, uint16_t *wf, uint16_t *wb);
-void nn_relu_bwd(hipStream_t s, uint16_t *g, const uint16_t *y, long n);
 void nn_avgpool2(hipStream_t s, const uint16_t *in, uint16_t *out, int N, int H, int W, int C);
 void nn_avgpool2_bwd(hipStream_t s, const uint16_t *gout, uint16_t *gin, int N, int H, int W, int C,
                      int accumulate);

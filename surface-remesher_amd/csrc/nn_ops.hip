@@ -303,8 +303,13 @@ __global__ void k_sum_splits(const float *__restrict__ part, int splits, long n,
 // per-split column sums of g [M][C] (bf16, C % 8 == 0, C <= 256) ->
 // part[split][C]: a thread owns 8 consecutive channels (one 16-byte load per
 // row) and every (256 / (C / 8))-th row of the split; the threads of one
-// channel group are then added in thread order (deterministic)
-__global__ __launch_bounds__(256) void k_colsum(const uint16_t *__restrict__ g, long M, int C,
+// channel group are then added in thread order (deterministic).  With RELU,
+// g is first masked in place by the ReLU derivative of the layer's output y
+// (g = 0 where y <= 0), the backward of the activation, and the masked
+// values are summed.
+template <bool RELU>
+__global__ __launch_bounds__(256) void k_colsum(uint16_t *__restrict__ g,
+                                                const uint16_t *__restrict__ y, long M, int C,
                                                 long chunk, float *__restrict__ part)
 {
     __shared__ float red[256][9];
@@ -316,8 +321,19 @@ __global__ __launch_bounds__(256) void k_colsum(const uint16_t *__restrict__ g, 
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (r0 < rows) {
         for (long m = mb + r0; m < me; m += rows) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(g + m * C + grp * 8);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            uint4 *gp = reinterpret_cast<uint4 *>(g + m * C + grp * 8);
+            const uint4 v = *gp;
+            uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            if constexpr (RELU) {
+                const uint4 yv = *reinterpret_cast<const uint4 *>(y + m * C + grp * 8);
+                const uint32_t yw[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (!(__uint_as_float(yw[k] << 16) > 0.0f)) w[k] &= 0xffff0000u;
+                    if (!(__uint_as_float(yw[k] & 0xffff0000u) > 0.0f)) w[k] &= 0x0000ffffu;
+                }
+                *gp = make_uint4(w[0], w[1], w[2], w[3]);
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 acc[2 * k] += __uint_as_float(w[k] << 16);
@@ -371,23 +387,6 @@ __device__ __forceinline__ uint4 pack8(const float f[8])
 #pragma unroll
     for (int k = 0; k < 4; ++k) w[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
     return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
-// g = 0 where the ReLU output y is not positive (n % 8 == 0)
-__global__ void k_relu_bwd(uint16_t *__restrict__ g, const uint16_t *__restrict__ y, int n8)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n8) return;
-    const uint4 yv = reinterpret_cast<const uint4 *>(y)[i];
-    uint4 gv = reinterpret_cast<const uint4 *>(g)[i];
-    const uint32_t yw[4] = {yv.x, yv.y, yv.z, yv.w};
-    uint32_t gw[4] = {gv.x, gv.y, gv.z, gv.w};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (!(__uint_as_float(yw[k] << 16) > 0.0f)) gw[k] &= 0xffff0000u;
-        if (!(__uint_as_float(yw[k] & 0xffff0000u) > 0.0f)) gw[k] &= 0x0000ffffu;
-    }
-    reinterpret_cast<uint4 *>(g)[i] = make_uint4(gw[0], gw[1], gw[2], gw[3]);
 }
 
 // 2x2 average pool, NHWC (H, W even)
@@ -819,22 +818,23 @@ void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *
 
 int nn_colsum_splits(long M) { return (int)std::min<long>(1024, std::max<long>(1, M / 1024)); }
 
-void nn_colsum(hipStream_t s, const uint16_t *g, long M, int C, float *db, float *work)
+void nn_colsum(hipStream_t s, uint16_t *g, long M, int C, float *db, float *work,
+               const uint16_t *relu_y)
 {
     const int splits = nn_colsum_splits(M);
     const long chunk = (M + splits - 1) / splits;
-    hipLaunchKernelGGL(k_colsum, dim3(splits), dim3(256), 0, s, g, M, C, chunk, work);
+    if (relu_y)
+        hipLaunchKernelGGL(k_colsum<true>, dim3(splits), dim3(256), 0, s, g, relu_y, M, C, chunk,
+                           work);
+    else
+        hipLaunchKernelGGL(k_colsum<false>, dim3(splits), dim3(256), 0, s, g, nullptr, M, C, chunk,
+                           work);
     hipLaunchKernelGGL(k_sum_cols, dim3(C), dim3(256), 0, s, work, splits, C, db);
 }
 
 void nn_weights_bf16(hipStream_t s, const float *w, int Cin, int Cout, uint16_t *wf, uint16_t *wb)
 {
     hipLaunchKernelGGL(k_weights_bf16, dim3(nblk(9L * Cin * Cout)), dim3(256), 0, s, w, Cin, Cout, wf, wb);
-}
-
-void nn_relu_bwd(hipStream_t s, uint16_t *g, const uint16_t *y, long n)
-{
-    hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n / 8)), dim3(256), 0, s, g, y, (int)(n / 8));
 }
 
 void nn_avgpool2(hipStream_t s, const uint16_t *in, uint16_t *out, int N, int H, int W, int C)

@@ -136,15 +136,15 @@ void UNet::backward(int nb)
     const int R0 = res(0), R1 = res(1), R2 = res(2), R3 = res(3);
     // head: gfield -> ga[13], dwh, dbh
     nn_head_bwd(stream, a[13], gfield, p + whoff, ga[13], g + whoff, work, npix(0, nb));
-    // one conv layer: mask the output gradient by ReLU', weight / bias
-    // gradients, then the input gradient into gin (if any)
+    // one conv layer: mask the output gradient by ReLU' and take the bias
+    // gradient (one pass), the weight gradient, then the input gradient into
+    // gin (if any)
     auto layer = [&](int l, const uint16_t *in, uint16_t *gin) {
         const ConvSpec &c = SPEC[l];
         const int r = res(c.level);
         const long m = npix(c.level, nb);
-        nn_relu_bwd(stream, ga[l], a[l], m * c.cout);
+        nn_colsum(stream, ga[l], m, c.cout, g + boff[l], work, a[l]);
         nn_conv3x3_dw(stream, in, ga[l], g + woff[l], work, nb, r, r, c.cin, c.cout, 0);
-        nn_colsum(stream, ga[l], m, c.cout, g + boff[l], work);
         if (gin) nn_conv3x3(stream, ga[l], wb[l], nullptr, gin, nb, r, r, c.cout, c.cin, 0);
     };
     layer(13, a[12], ga[12]);
