@@ -288,16 +288,28 @@ __global__ __launch_bounds__(256) void k_sum_cols(const float *__restrict__ part
     if (threadIdx.x == 0) out[c] = red[0];
 }
 
-// out[i] = sum_s part[s][i] in split order (deterministic)
-__global__ void k_sum_splits(const float *__restrict__ part, int splits, long n,
-                             float *__restrict__ out, int accumulate)
+// out[i] = sum_s part[s][i] in a fixed order (deterministic): block 256
+// takes 64 consecutive i; its four waves sum the splits s = q, q + 4, ...
+// (q = wave) in order, then the four sums are added in wave order.
+// grid ceil(n / 64).
+__global__ __launch_bounds__(256) void k_sum_splits(const float *__restrict__ part, int splits,
+                                                    long n, float *__restrict__ out,
+                                                    int accumulate)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const long i = (long)blockIdx.x * 64 + lane;
     float s = 0.0f;
+    if (i < n) {
 #pragma unroll 8
-    for (int k = 0; k < splits; ++k) s += part[(long)k * n + i];
-    out[i] = accumulate ? out[i] + s : s;
+        for (int k = q; k < splits; k += 4) s += part[(long)k * n + i];
+    }
+    red[q][lane] = s;
+    __syncthreads();
+    if (q == 0 && i < n) {
+        const float t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+        out[i] = accumulate ? out[i] + t : t;
+    }
 }
 
 // per-split column sums of g [M][C] (bf16, C % 8 == 0, C <= 256) ->
@@ -813,7 +825,8 @@ void nn_conv3x3_dw(hipStream_t s, const uint16_t *in, const uint16_t *g, float *
         hipLaunchKernelGGL((k_conv3x3_dw2<32, 32, 64, 3>), grid, dim3(256), 0, s, in, g, work, N, H,
                            W, Cin, Cout, chunk);
     const long n = 9L * Cout * Cin;
-    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(n)), dim3(256), 0, s, work, splits, n, dw, accumulate);
+    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(n, 64)), dim3(256), 0, s, work, splits, n, dw,
+                       accumulate);
 }
 
 int nn_colsum_splits(long M) { return (int)std::min<long>(1024, std::max<long>(1, M / 1024)); }
@@ -880,7 +893,8 @@ void nn_head_bwd(hipStream_t s, const uint16_t *x, const float *gf, const float 
     const long chunk = (M + splits - 1) / splits;
     constexpr long NP = 2 * NN_HEAD_C + 2;
     hipLaunchKernelGGL(k_head_bwd, dim3(splits), dim3(256), 0, s, x, gf, wh, gx, work, M, chunk);
-    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(NP)), dim3(256), 0, s, work, splits, NP, dwh_db, 0);
+    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(NP, 64)), dim3(256), 0, s, work, splits, NP, dwh_db,
+                       0);
 }
 
 void nn_sample_loss(hipStream_t s, const float *field, int R, const float *uv, const float *target,
