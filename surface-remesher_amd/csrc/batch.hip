@@ -231,9 +231,10 @@ static int g_cu_count()
 
 int Batch::plan()
 {
-    // JFA work units: about four per CU over all pixels (the C4 batch: 391
-    // batched iterations/s with two, 426 with four, 418 with eight), at
-    // least one per patch, at least 16 row segments (one per wave) each
+    // JFA work units: about four per CU over all pixels (a batch of 64
+    // patches of 256^2 .. 2048^2: 391 batched iterations/s with two, 426
+    // with four, 418 with eight), at least one per patch, at least 16 row
+    // segments (one per wave) each
     const long target = std::max<long>(1, total_pix / (4L * g_cu_count()));
     units.clear();
     for (int p = 0; p < np; ++p) {
